@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""StreamGC headline benchmark on one MI355X.
+
+Workload (default "c2"): Chung-Lu graph of com-LiveJournal shape (3,997,962
+vertices, 34,681,189 edges, degree exponent 2.3, max degree ~15K, seed 42,
+random vertex permutation), churn stream at 50% insert (seed 42), epoch
+E = 32,768, histogram threshold 64, plain per-update GPU engine.  The init
+stream and its repair run first, in the reference's epoch order; the
+measured phase repairs and publishes a proper snapshot every epoch.
+
+A step is one batch of E updates: the per-update kernel over the batch, then
+the epoch (compact, drain, Tier-1, Tier-2 check, publish).  Inputs are
+resident in HBM before timing.  Every published snapshot is validated
+afterwards against the stream's live edge set (independent validator).
+
+Prints one JSON line (rank 0, one GPU).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "edge updates/s on 1 MI355X, served vs plain; p50/p99 update-to-snapshot ms"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c2": dict(name="Chung-Lu com-LiveJournal shape: 3,997,962 vertices, 34,681,189 edges, gamma 2.3, "
+                    "max degree 15K, permuted (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
+               kind="chung_lu", n=3997962, m=34681189, gamma=2.3, max_degree=15000.0),
+    "c1": dict(name="Delaunay 2^17 uniform points (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
+               kind="delaunay", points=1 << 17),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_graph(S, w):
+    if w["kind"] == "chung_lu":
+        return S.gen_chung_lu(w["n"], w["m"], w["gamma"], w["max_degree"], seed=42, permute=True)
+    return S.gen_delaunay(w["points"], seed=42)
+
+
+def kernel_bytes(name, n, S_slots, nnz, upd_tot):
+    """Algorithmic bytes of one launch (per-unit figures of the roofline model)."""
+    if name == "apply":
+        u = upd_tot
+        return (48 * u["updates"] + 4 * u["scan_slots"] + 4 * u["hd_endpoints"] + 4 * u["recolor_cap_slots"]
+                + 8 * u["recolor_live"] + 4 * u["recolors"]) / max(u["launches"], 1)
+    if name == "compact":
+        return 8 * S_slots
+    if name == "rebuild":
+        return 4 * S_slots + 4 * nnz + 8 * n
+    if name in ("tier1", "t2_check"):
+        return 4 * S_slots + 4 * nnz + 4 * n
+    if name == "snapshot":
+        return 6 * n
+    return 0
+
+
+def cpu_baseline(g, state, bench, E, batches, threads):
+    """The oracle (sequential updates, OpenMP sweeps) on the same batches from the same state."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    og = O.Csr(g.n, g.row_ptr, g.col_idx, g.degree)
+    cfg = O.default_config(flags=O.FLAG_REPAIR_EVERY_EPOCH)
+    eng = O.Engine(og, cfg, threads=threads)
+    words, col, bitmaps, hist = state
+    eng.set_col(col)
+    eng.set_words(words)
+    eng.set_bitmaps(bitmaps)
+    if hist.size:
+        eng.set_hist(hist)
+    sl = bench[:batches * E]
+    t0 = time.perf_counter()
+    eng.submit(sl["u"], sl["v"], sl["type"])
+    dt = time.perf_counter() - t0
+    return len(sl) / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-batches", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-validate", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this "
+              f"build", file=sys.stderr)
+        sys.exit(2)
+    import streamgc_amd as S
+    if S.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+
+    w = WORKLOADS[args.workload]
+    E = 32768
+    K, W = args.steps, args.warmup
+    t = time.perf_counter()
+    g = make_graph(S, w)
+    log(f"graph {g.n} vertices {g.num_edges // 2} edges max deg {int(g.degree.max())} ({time.perf_counter() - t:.1f}s)")
+    t = time.perf_counter()
+    init = S.generate_init_stream(g)
+    bench = S.generate_bench_stream_edges(g, (W + K) * E, 50, 42)
+    log(f"streams: init {len(init)}, churn {len(bench)} ({time.perf_counter() - t:.1f}s)")
+
+    eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E))
+    t = time.perf_counter()
+    ds_init = S.DeviceStream(init)
+    eng.process_stream(ds_init, max_epochs=0)
+    fin = eng.finish()
+    ds_init.close()
+    log(f"init phase: {time.perf_counter() - t:.1f}s, max colour {fin['max_color']}, t2_remaining {fin['t2_remaining']}")
+
+    eng.set_mode(repair_every_epoch=True)
+    eng.history_reserve(W + K)
+    ds = S.DeviceStream(bench)
+    for k in range(W):
+        eng.submit_device(ds.at(k * E), E)
+    state = None
+    if not args.no_cpu_baseline:
+        state = (eng.words(), eng.csr()[1], eng.bitmaps(), eng.hist())
+    tot0 = eng.totals()
+    eng.kernel_timing(True)
+    lat = []
+    epochs = []
+    t0 = time.perf_counter()
+    for k in range(W, W + K):
+        ts = time.perf_counter()
+        epochs += eng.submit_device(ds.at(k * E), E, max_epochs=1)
+        lat.append(time.perf_counter() - ts)
+    elapsed = time.perf_counter() - t0
+    ktimes = eng.kernel_times()
+    eng.kernel_timing(False)
+    tot1 = eng.totals()
+    value = K * E / elapsed
+    log(f"timed {K} steps: {elapsed * 1e3:.1f} ms, {value / 1e6:.2f} M updates/s")
+
+    # roofline of the dominant kernel
+    d = {k: tot1[k] - tot0[k] for k in tot1}
+    n = g.n
+    slots = int(np.sum(np.maximum(g.degree.astype(np.int64) * 4, 8)))   # slack 300%, floor 8
+    comp_launches = ktimes.get("compact", (0, 0))[1]
+    nnz_live = d["live_slots"] / comp_launches if comp_launches else g.num_edges
+    upd = dict(d)
+    upd["launches"] = ktimes.get("apply", (0, 1))[1]
+    per_kernel = {}
+    for name, (ms, launches) in ktimes.items():
+        b = kernel_bytes(name, n, slots, nnz_live, upd)
+        avg_ms = ms / max(launches, 1)
+        per_kernel[name] = dict(total_ms=round(ms, 3), launches=int(launches), avg_us=round(avg_ms * 1e3, 2),
+                                bytes_per_launch=int(b),
+                                gbs=round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 and b else None)
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["total_ms"])
+    pk = per_kernel[dom]
+    roofline = dict(bound="hbm", kernel=dom, achieved=pk["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(pk["gbs"] / HBM_PEAK_GBS, 4) if pk["gbs"] else None, traffic=None,
+                    bytes_per_launch=pk["bytes_per_launch"], avg_launch_us=pk["avg_us"])
+
+    parity = {"epochs_timed": len(epochs), "t2_remaining_max": max(e["t2_remaining"] for e in epochs),
+              "max_color_final": epochs[-1]["max_color"], "t1_iterations_mean":
+              round(float(np.mean([e["t1_iterations"] for e in epochs])), 2),
+              "tier2_runs": int(sum(e["t2_ran"] for e in epochs)), "grouped": int(d["grouped"]),
+              "overflow_events": int(d["overflow_events"])}
+    if not args.no_validate:
+        t = time.perf_counter()
+        iv = S.edge_intervals(g, bench)
+        pos = [(k + 1) * E for k in range(eng.history_count())]
+        bad = eng.validate_history(iv, pos)
+        parity.update(snapshots_validated=len(pos), snapshots_improper=int(np.count_nonzero(bad)),
+                      conflicting_edges_total=int(bad.sum()))
+        log(f"validated {len(pos)} snapshots against the live edge set: {int(bad.sum())} bad ({time.perf_counter() - t:.1f}s)")
+
+    cpu = None
+    if state is not None:
+        threads = min(16, os.cpu_count() or 1)
+        rate, dt = cpu_baseline(g, state, bench[W * E:], E, args.cpu_batches, threads)
+        cpu = dict(value=round(rate, 1), unit="updates/s", cores=threads, kind="port",
+                   sample=f"oracle engine, first {args.cpu_batches} measured batches ({args.cpu_batches * E} "
+                          f"updates incl. their epochs with Tier-1/Tier-2 and publish) from the same state; "
+                          f"{dt:.1f}s")
+        log(f"cpu baseline: {rate:.0f} updates/s on {threads} threads ({dt:.1f}s)")
+
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(elapsed * 1e3 / K, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": w["name"], "vertices": g.n, "edges": g.num_edges // 2, "epoch_size": E,
+                   "updates_timed": K * E, "maintenance": "full sweep", "epoch": "compact, drain, tier-1, "
+                   "tier-2 check (host tier-2 when needed), publish"},
+        "latency_ms": {"update_to_snapshot_p50": round(float(np.percentile(lat_ms, 50)), 3),
+                       "update_to_snapshot_p99": round(float(np.percentile(lat_ms, 99)), 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "kernels": per_kernel,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

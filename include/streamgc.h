@@ -1,0 +1,247 @@
+/* streamgc.h -- C ABI of the MI355X StreamGC engine (libstreamgc_amd.so).
+ *
+ * Plain pointers and sizes only.  Every call returns SGC_OK (0) or a negative
+ * error code and never exits the process; sgc_last_error() gives the text.
+ * Calls on one engine handle are serialised by the caller (one HIP stream per
+ * engine); sgc_query may run from another thread because it only reads the
+ * published snapshot.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout,
+ * A/ = algorithms/streamgc/):
+ *   EdgeUpdate / UpdateType ........ A/include/types.h:8-21
+ *   StreamGCConfig ................. A/include/streamgc.cuh:113-135
+ *   GPUPartition + allocate ........ A/include/streamgc.cuh:26-110, A/src/streamgc.cu:144-444
+ *   local_update_kernel launch ..... A/src/streamgc.cu:847-854
+ *   process_stream ................. A/src/streamgc.cu:809-1053
+ *   epoch step ..................... A/src/streamgc.cu:949-1036
+ *   drain_and_compact + repair ..... A/src/streamgc.cu:1243-1314, 1317-1393, 1057-1240
+ *   check_coloring_valid ........... A/src/streamgc.cu:448-556
+ *   point_query .................... A/src/host/query_server.cpp:23-28
+ *   load_graph_csr ................. A/src/host/graph_loader.cpp:29-169
+ *   generate_init_stream ........... A/src/host/stream_gen.cpp:30-61
+ *   generate_bench_stream_edges .... A/src/host/stream_gen.cpp:65-177
+ *   static_greedy_coloring ......... A/src/streamgc.cu:559-592
+ */
+#ifndef STREAMGC_AMD_H
+#define STREAMGC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGC_OK 0
+#define SGC_ERR_INVALID (-1)  /* bad argument */
+#define SGC_ERR_HIP (-2)      /* HIP runtime error (no device, launch failure, ...) */
+#define SGC_ERR_NOMEM (-3)    /* device or host allocation failed, or over budget */
+#define SGC_ERR_RANGE (-4)    /* a size exceeds what the layout supports */
+#define SGC_ERR_IO (-5)       /* file could not be read or written */
+#define SGC_ERR_STATE (-6)    /* call not valid in the engine's current state */
+
+/* 16-byte update record, byte-identical to the reference EdgeUpdate.
+ * type: 0 = ADD, 1 = DEL (2, 3 are accepted and ignored, as in the reference).
+ * pad bytes 0..2 are zero; bytes 3..6 (little-endian u32 at offset 12) carry
+ * the graph index in a pool and must be zero for a single-graph engine. */
+typedef struct sgc_edge_update {
+    uint32_t u;
+    uint32_t v;
+    uint8_t type;
+    uint8_t pad[7];
+} sgc_edge_update;
+
+/* Engine flags. */
+#define SGC_DETERMINISTIC (1u << 0)      /* one wavefront, updates in stream order, repair in vertex order */
+#define SGC_REPAIR_EVERY_EPOCH (1u << 1) /* epoch = compact, drain, Tier-1, Tier-2, publish (proper snapshots) */
+#define SGC_EXACT_HISTOGRAM (1u << 2)    /* hub histograms recounted wherever bitmaps are rebuilt */
+
+/* Engine configuration; the defaults (sgc_default_config) are the reference's. */
+typedef struct sgc_config {
+    uint32_t epoch_size;          /* E: epoch step after a batch once >= E updates (reference 32768 effective) */
+    uint32_t batch_size;          /* host batch (reference BATCH_SIZE 32768, A/src/streamgc.cu:785) */
+    uint32_t histogram_threshold; /* deg0 > threshold -> histogram colour selection (64) */
+    uint32_t slack_pct;           /* row capacity d + d*slack/100 (300) */
+    uint32_t min_row_capacity;    /* row capacity floor (8) */
+    uint32_t overflow_capacity;   /* overflow entries kept (1 << 18) */
+    uint32_t max_repair_iters;    /* Tier-1 iteration cap (20) */
+    uint32_t plateau_iters;       /* Tier-1 stops after this many equal counts (3) */
+    uint32_t tier2_passes;        /* Tier-2 pass cap (30) */
+    uint32_t flags;               /* SGC_* flags */
+} sgc_config;
+
+/* What one epoch step (or the end-of-phase repair) did. */
+typedef struct sgc_epoch_stats {
+    uint32_t epoch_index;
+    uint32_t overflow_count;     /* raw overflow counter at the drain (may exceed the capacity) */
+    uint32_t overflow_drained;   /* min(count, capacity) entries retried */
+    uint32_t overflow_failed;    /* retried entries that still found no slot */
+    uint32_t t1_iterations;
+    uint32_t t1_conflicts_last;  /* conflicts counted by the last Tier-1 sweep (0 = converged) */
+    uint32_t t2_fixed;           /* vertices recoloured by Tier-2 */
+    uint32_t t2_remaining;       /* row slots still conflicting or uncoloured after repair (0 = proper) */
+    uint32_t max_color;
+    uint32_t snapshot_published;
+    uint32_t t2_ran;             /* Tier-2 found work and ran */
+    uint32_t reserved;
+} sgc_epoch_stats;
+
+/* Running totals over the engine's life (algorithmic counters for the roofline). */
+typedef struct sgc_totals {
+    uint64_t updates;            /* updates applied (both endpoints in range) */
+    uint64_t scan_slots;         /* slots scanned by inserts and deletes (p_u + p_v) */
+    uint64_t hd_endpoints;       /* endpoints in histogram mode */
+    uint64_t recolors;           /* vertex recolours during updates */
+    uint64_t recolor_cap_slots;  /* row capacity scanned by those recolours */
+    uint64_t recolor_live;       /* live neighbours updated by those recolours */
+    uint64_t epochs;
+    uint64_t t1_sweeps;
+    uint64_t t2_runs;
+    uint64_t overflow_events;
+    uint64_t live_slots;         /* live row slots summed over compactions */
+    uint64_t grouped;            /* parallel-batch updates applied in stream order by the wave owning their edge */
+    uint64_t reserved[4];
+} sgc_totals;
+
+typedef struct sgc_engine sgc_engine;
+
+void sgc_default_config(sgc_config* cfg);
+const char* sgc_last_error(const sgc_engine* e); /* NULL engine: last global error */
+const char* sgc_version(void);
+
+/* Device probe: number of HIP devices (0 when none); never fails. */
+int sgc_device_count(void);
+
+/* = allocate_gpu_partition (A/src/streamgc.cu:144-444) + the priority fill
+ * (:755-767).  Input: symmetric CSR with sorted rows (as the loader makes),
+ * deg0 = row lengths (fixes the priority pi(v) = (deg0 << 32) | v). */
+int sgc_create(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+               const uint32_t* deg0, sgc_engine** out);
+int sgc_destroy(sgc_engine* e);
+
+/* = process_stream (A/src/streamgc.cu:809-1053): batches of batch_size, the
+ * per-update kernel, an epoch step after each batch once >= epoch_size.
+ * updates on_device = 0: host memory; 1: device memory (already in HBM).
+ * epochs_out (optional) receives up to max_epochs per-epoch records. */
+int sgc_submit(sgc_engine* e, const sgc_edge_update* updates, uint64_t count, int on_device,
+               sgc_epoch_stats* epochs_out, uint32_t max_epochs, uint32_t* epochs_done);
+
+/* = one local_update_kernel launch over count updates (no epoch step). */
+int sgc_apply(sgc_engine* e, const sgc_edge_update* updates, uint64_t count, int on_device);
+
+/* = one epoch step (A/src/streamgc.cu:949-1036, or the repair-every-epoch order). */
+int sgc_epoch(sgc_engine* e, sgc_epoch_stats* out);
+
+/* = drain_and_compact + run_conflict_repair + cpu_serial_repair at the end of
+ * a phase (A/src/streamgc.cu:1405-1408, 1420-1423); publishes a snapshot. */
+int sgc_finish(sgc_engine* e, sgc_epoch_stats* out);
+
+/* Change the engine flags between phases (e.g. the reference's epoch order
+ * during the init stream, repair every epoch during the measured stream).
+ * Any SGC_* flag may change; the layout does not. */
+int sgc_set_flags(sgc_engine* e, uint32_t flags);
+
+/* = point_query (A/src/host/query_server.cpp:23-28) over the published snapshot. */
+int sgc_query(sgc_engine* e, const uint32_t* ids, uint32_t count, uint16_t* out_colors);
+
+/* Downloads for validation and output. */
+int sgc_download_words(sgc_engine* e, uint32_t* words);          /* n packed (version << 16 | color) */
+int sgc_download_colors(sgc_engine* e, uint16_t* colors);        /* n live colours */
+int sgc_download_snapshot(sgc_engine* e, uint16_t* colors);      /* n published colours */
+int sgc_num_slots(sgc_engine* e, uint64_t* slots);
+int sgc_download_csr(sgc_engine* e, uint32_t* row_ptr, uint32_t* col_idx); /* n+1, slots */
+int sgc_download_bitmaps(sgc_engine* e, uint64_t* bitmaps);
+int sgc_num_hd(sgc_engine* e, uint32_t* num_hd);
+int sgc_download_hist(sgc_engine* e, uint32_t* hist);            /* num_hd * 1024 */
+int sgc_upload_words(sgc_engine* e, const uint32_t* words);
+int sgc_upload_bitmaps(sgc_engine* e, const uint64_t* bitmaps);
+int sgc_upload_hist(sgc_engine* e, const uint32_t* hist);
+int sgc_totals_get(sgc_engine* e, sgc_totals* out);
+int sgc_overflow_count(sgc_engine* e, uint32_t* count);
+
+/* One colour selection for vertex v on the device (FindAvailableColor,
+ * A/include/color_select.cuh:93-115), for known-answer tests. */
+int sgc_find_color(sgc_engine* e, uint32_t v, uint16_t* out);
+
+/* = check_coloring_valid (A/src/streamgc.cu:448-556) on the engine's own rows:
+ * conflicting live edges (counted once per stored slot) and uncoloured
+ * vertices with a live slot, from the live colours (snapshot = 0) or the
+ * published snapshot (snapshot = 1).  max_color as the reference counts it. */
+int sgc_validate(sgc_engine* e, int snapshot, uint64_t* violations, uint64_t* uncolored_with_edges,
+                 uint32_t* max_color);
+
+/* Keep a copy of every published snapshot in a device-side history of up to
+ * `capacity` snapshots (0 disables).  Copies are device-to-device. */
+int sgc_history_reserve(sgc_engine* e, uint32_t capacity);
+int sgc_history_count(sgc_engine* e, uint32_t* count);
+
+/* Device-resident update buffers (stream staging in HBM). */
+int sgc_device_upload(const void* host, uint64_t bytes, void** dev_out);
+int sgc_device_free(void* dev);
+
+/* Independent validator: an edge list with life intervals [born, died) in
+ * stream positions (died = UINT64_MAX: alive at the end), checked against each
+ * snapshot in the history; snapshot k is checked at stream position pos[k].
+ * bad_out[k] = conflicting live edges + uncoloured endpoints of live edges. */
+int sgc_validate_history(sgc_engine* e, const uint32_t* eu, const uint32_t* ev, const uint64_t* born,
+                         const uint64_t* died, uint64_t m, const uint64_t* pos, uint32_t count,
+                         uint64_t* bad_out);
+
+/* Timing of the last sgc_submit / sgc_epoch calls in milliseconds, measured
+ * with HIP events on the engine's stream: [0] update kernels, [1] epoch
+ * kernels, [2] host-side Tier-2, [3] wall.  Kernel-level per-name totals:
+ * sgc_kernel_times fills up to cap (name, total_ms, launches) triples. */
+int sgc_last_timing(sgc_engine* e, double* ms4);
+int sgc_kernel_timing_enable(sgc_engine* e, int enable);
+int sgc_kernel_times(sgc_engine* e, char* names, uint32_t name_stride, double* total_ms, uint64_t* launches,
+                     uint32_t cap, uint32_t* count);
+
+/* ---- host pipeline (C++17) ---- */
+
+/* Graph container (host CSR, symmetric, sorted rows, no self-loops). */
+typedef struct sgc_csr sgc_csr;
+
+/* = load_graph_csr (A/src/host/graph_loader.cpp:29-169). */
+int sgc_csr_load_mtx(const char* path, sgc_csr** out);
+/* Dedup, drop self-loops, symmetrise, sort rows (the loader's rules). */
+int sgc_csr_from_edges(uint32_t n, const uint32_t* u, const uint32_t* v, uint64_t m, sgc_csr** out);
+int sgc_csr_info(const sgc_csr* g, uint32_t* n, uint64_t* nnz, uint32_t* max_deg);
+int sgc_csr_export(const sgc_csr* g, uint32_t* row_ptr, uint32_t* col_idx, uint32_t* deg);
+int sgc_csr_free(sgc_csr* g);
+
+/* Synthetic inputs (no reference counterpart; seeded, thread-count independent).
+ * Chung-Lu: power-law expected degrees (exponent gamma, given max degree),
+ * exactly m distinct edges, optional random vertex permutation.
+ * R-MAT: edge_factor * 2^scale draws with quadrant probabilities a, b, c. */
+int sgc_gen_chung_lu(uint32_t n, uint64_t m, double gamma, double max_degree, uint64_t seed, int permute,
+                     sgc_csr** out);
+int sgc_gen_rmat(uint32_t scale, double edge_factor, double a, double b, double c, uint64_t seed, sgc_csr** out);
+/* Core numbers (Batagelj-Zaversnik). */
+int sgc_core_numbers(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, uint32_t* core);
+/* Dense-growth stream inside {v : core(v) >= core_min}: ins_per_del inserts
+ * (new pairs) per delete (live edges inside the region), on average. */
+int sgc_dense_growth_stream(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const uint32_t* core,
+                            uint32_t core_min, uint64_t size, uint32_t ins_per_del, uint64_t seed,
+                            sgc_edge_update* out);
+
+/* = generate_init_stream (A/src/host/stream_gen.cpp:30-61): row_ptr[n]/2 records. */
+int sgc_init_stream(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const uint32_t* deg,
+                    sgc_edge_update* out, uint64_t* count);
+
+/* = generate_bench_stream_edges (A/src/host/stream_gen.cpp:65-177); byte-exact. */
+int sgc_churn_stream(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, uint64_t size,
+                     uint32_t insert_pct, uint64_t seed, sgc_edge_update* out);
+
+/* = static_greedy_coloring (A/src/streamgc.cu:559-592): max colour of first fit in id order. */
+int sgc_static_greedy(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, uint32_t* max_color);
+
+/* Edge life intervals of a stream over a base graph, for sgc_validate_history.
+ * Two calls: with null outputs it reports m; then fills eu, ev, born, died. */
+int sgc_edge_intervals(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+                       const sgc_edge_update* stream, uint64_t count, uint64_t* m, uint32_t* eu, uint32_t* ev,
+                       uint64_t* born, uint64_t* died);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

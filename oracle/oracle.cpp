@@ -116,6 +116,7 @@ struct Config {              // same field order as sgc_config in include/stream
     uint32_t flags;
 };
 constexpr uint32_t FLAG_REPAIR_EVERY_EPOCH = 1u << 1;
+constexpr uint32_t FLAG_EXACT_HISTOGRAM = 1u << 2;
 
 struct Stats {               // same layout as sgc_epoch_stats in include/streamgc.h
     uint32_t epoch_index;
@@ -299,12 +300,27 @@ struct Engine {
             if (!done) st.overflow_failed++;
         }
     }
-    // A/include/color_select.cuh:213-251 + A/src/kernels/epoch_maintenance.cu:113-128
+    // A/include/color_select.cuh:213-251 + A/src/kernels/epoch_maintenance.cu:113-128.
+    // With FLAG_EXACT_HISTOGRAM (no reference counterpart: the reference never
+    // rebuilds histograms, A/src/kernels/epoch_maintenance.cu:120-123) the
+    // histogram rows are recounted from the live rows at the same point.
     void rebuild_bitmaps() {
+        const bool exact = (cfg.flags & FLAG_EXACT_HISTOGRAM) != 0;
         #pragma omp parallel for num_threads(threads) schedule(dynamic, 1024)
         for (int64_t vi = 0; vi < (int64_t)n; vi++) {
             uint32_t v = (uint32_t)vi;
-            if (hd_map[v] != INVALID) continue;
+            if (hd_map[v] != INVALID) {
+                if (!exact) continue;
+                uint32_t* h = &hist[(size_t)hd_map[v] * HIST_BINS];
+                std::fill(h, h + HIST_BINS, 0u);
+                for (uint32_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) {
+                    uint32_t nb = col[j];
+                    if (nb == INVALID || nb >= n) continue;
+                    uint16_t c = color(nb);
+                    if (c != 0 && c <= HIST_BINS) h[c - 1]++;
+                }
+                continue;
+            }
             uint64_t bm = 0;
             for (uint32_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) {
                 uint32_t nb = col[j];
@@ -366,7 +382,10 @@ struct Engine {
     }
     // A/src/streamgc.cu:1057-1240 (Tier-2), adjacency from both row directions
     void tier2(Stats& st) {
-        st.t2_ran = 1;
+        // Tier-2 runs unconditionally here, as in the reference; t2_ran
+        // reports whether it had anything to do (an uncoloured endpoint or a
+        // conflict on some row slot).
+        st.t2_ran = row_conflicts() > 0 ? 1 : 0;
         std::vector<uint64_t> off(n + 1, 0);
         for (uint32_t u = 0; u < n; u++)
             for (uint32_t j = row_ptr[u]; j < row_ptr[u + 1]; j++) {
@@ -688,6 +707,7 @@ void* orc_engine_create(uint32_t n, const uint32_t* row_ptr, const uint32_t* col
     return e;
 }
 void orc_engine_destroy(void* h) { delete (Engine*)h; }
+void orc_set_flags(void* h, uint32_t flags) { ((Engine*)h)->cfg.flags = flags; }
 void orc_set_threads(void* h, int t) { ((Engine*)h)->threads = t > 0 ? t : 1; }
 
 uint32_t orc_submit(void* h, const uint32_t* u, const uint32_t* v, const uint8_t* t, uint64_t count, Stats* per_epoch,

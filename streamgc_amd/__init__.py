@@ -1,0 +1,430 @@
+"""MI355X-native StreamGC engine: streaming graph colouring on gfx950.
+
+Python mirror of the reference's host interface for the hot path, over the
+C ABI of libstreamgc_amd.so (include/streamgc.h).  Names follow the
+reference: StreamGCConfig (A/include/streamgc.cuh:113-135), EdgeUpdate
+(A/include/types.h:8-21), GPUPartition (A/include/streamgc.cuh:26-110) with
+the phases of main() (A/src/streamgc.cu:809-1053, 1243-1393, 1057-1240),
+load_graph_csr (A/src/host/graph_loader.cpp:29), generate_init_stream and
+generate_bench_stream_edges (A/src/host/stream_gen.cpp:30,65).
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import EDGE_UPDATE, SgcError, check, lib
+
+__all__ = ["StreamGCConfig", "default_config", "CSRGraph", "GPUPartition", "DeviceStream", "EDGE_UPDATE",
+           "load_graph_csr", "csr_from_edges", "csr_from_edge_arrays", "gen_chung_lu", "gen_rmat",
+           "gen_delaunay", "core_numbers", "dense_growth_stream", "generate_init_stream", "generate_bench_stream_edges",
+           "static_greedy_coloring", "check_coloring_valid", "edge_intervals", "make_updates", "SgcError",
+           "device_count"]
+
+UPDATE_ADD = 0
+UPDATE_DEL = 1
+
+
+@dataclass
+class StreamGCConfig:
+    """Engine configuration (the reference's defaults)."""
+    epoch_size: int = 32768
+    batch_size: int = 32768
+    histogram_threshold: int = 64
+    slack_pct: int = 300
+    min_row_capacity: int = 8
+    overflow_capacity: int = 1 << 18
+    max_repair_iters: int = 20
+    plateau_iters: int = 3
+    tier2_passes: int = 30
+    deterministic: bool = False
+    repair_every_epoch: bool = False
+    exact_histogram: bool = False
+
+    def to_c(self):
+        flags = (_lib.SGC_DETERMINISTIC if self.deterministic else 0) | \
+                (_lib.SGC_REPAIR_EVERY_EPOCH if self.repair_every_epoch else 0) | \
+                (_lib.SGC_EXACT_HISTOGRAM if self.exact_histogram else 0)
+        return _lib.Config(self.epoch_size, self.batch_size, self.histogram_threshold, self.slack_pct,
+                           self.min_row_capacity, self.overflow_capacity, self.max_repair_iters,
+                           self.plateau_iters, self.tier2_passes, flags)
+
+
+def default_config(**kw):
+    return StreamGCConfig(**kw)
+
+
+def device_count():
+    return lib().sgc_device_count()
+
+
+class CSRGraph:
+    """Symmetric CSR with sorted rows (A/src/host/graph_loader.cpp:20-26)."""
+
+    def __init__(self, num_vertices, row_ptr, col_idx, degree=None):
+        self.num_vertices = int(num_vertices)
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        self.col_idx = np.ascontiguousarray(col_idx, dtype=np.uint32)
+        if degree is None:
+            degree = np.diff(self.row_ptr.astype(np.int64))
+        self.degree = np.ascontiguousarray(degree, dtype=np.uint32)
+
+    @property
+    def n(self):
+        return self.num_vertices
+
+    @property
+    def num_edges(self):
+        """Directed entries (2 x undirected), as the reference counts them."""
+        return int(self.row_ptr[-1])
+
+    def edges(self):
+        u = np.repeat(np.arange(self.n, dtype=np.uint32), np.diff(self.row_ptr.astype(np.int64)))
+        m = u < self.col_idx
+        return u[m], self.col_idx[m]
+
+
+def csr_from_edges(n, edges):
+    """CSR from an edge list with the loader's rules (dedup, no self-loops, symmetric, sorted rows)."""
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    if len(e):
+        e = e[e[:, 0] != e[:, 1]]
+        e = np.unique(np.sort(e, axis=1), axis=0)
+    a = np.concatenate([e[:, 0], e[:, 1]])
+    b = np.concatenate([e[:, 1], e[:, 0]])
+    order = np.lexsort((b, a))
+    a, b = a[order], b[order]
+    deg = np.bincount(a, minlength=n).astype(np.uint32)
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    np.cumsum(deg, out=rp[1:])
+    return CSRGraph(n, rp, b.astype(np.uint32), deg)
+
+
+def _from_handle(h):
+    L = lib()
+    n, nnz, md = C.c_uint32(), C.c_uint64(), C.c_uint32()
+    try:
+        check(L.sgc_csr_info(h, C.byref(n), C.byref(nnz), C.byref(md)))
+        rp = np.zeros(n.value + 1, np.uint32)
+        col = np.zeros(max(nnz.value, 1), np.uint32)
+        deg = np.zeros(max(n.value, 1), np.uint32)
+        check(L.sgc_csr_export(h, rp.ctypes.data, col.ctypes.data, deg.ctypes.data))
+    finally:
+        L.sgc_csr_free(h)
+    return CSRGraph(n.value, rp, col[:nnz.value], deg[:n.value])
+
+
+def load_graph_csr(path):
+    """= load_graph_csr (A/src/host/graph_loader.cpp:29-169)."""
+    h = C.c_void_p()
+    check(lib().sgc_csr_load_mtx(path.encode(), C.byref(h)))
+    return _from_handle(h)
+
+
+def csr_from_edge_arrays(n, u, v):
+    """Same rules as csr_from_edges, in C++ (large edge lists)."""
+    h = C.c_void_p()
+    u = np.ascontiguousarray(u, np.uint32)
+    v = np.ascontiguousarray(v, np.uint32)
+    check(lib().sgc_csr_from_edges(n, u, v, len(u), C.byref(h)))
+    return _from_handle(h)
+
+
+def gen_chung_lu(n, m, gamma=2.3, max_degree=15000.0, seed=42, permute=True):
+    """Power-law Chung-Lu graph with exactly m distinct edges."""
+    h = C.c_void_p()
+    check(lib().sgc_gen_chung_lu(n, m, gamma, max_degree, seed, 1 if permute else 0, C.byref(h)))
+    return _from_handle(h)
+
+
+def gen_rmat(scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=42):
+    """R-MAT graph: edge_factor * 2^scale draws, deduplicated and symmetrised."""
+    h = C.c_void_p()
+    check(lib().sgc_gen_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)))
+    return _from_handle(h)
+
+
+def gen_delaunay(num_points, seed=42):
+    """Delaunay triangulation of num_points uniform points in the unit square."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    pts = rng.random((num_points, 2))
+    tri = Delaunay(pts).simplices.astype(np.int64)
+    u = np.concatenate([tri[:, 0], tri[:, 1], tri[:, 2]])
+    v = np.concatenate([tri[:, 1], tri[:, 2], tri[:, 0]])
+    return csr_from_edge_arrays(num_points, u.astype(np.uint32), v.astype(np.uint32))
+
+
+def core_numbers(g):
+    core = np.zeros(max(g.n, 1), np.uint32)
+    check(lib().sgc_core_numbers(g.n, g.row_ptr, g.col_idx, core))
+    return core[:g.n]
+
+
+def dense_growth_stream(g, size, core_pct=90, ins_per_del=2, seed=42, core=None):
+    """Dense-growth stream inside the vertices whose core number is at least the core_pct percentile."""
+    core = core_numbers(g) if core is None else core
+    core_min = int(np.percentile(core, core_pct, method="lower"))
+    out = np.zeros(max(size, 1), dtype=EDGE_UPDATE)
+    check(lib().sgc_dense_growth_stream(g.n, g.row_ptr, g.col_idx, np.ascontiguousarray(core, np.uint32),
+                                        core_min, size, ins_per_del, seed, out.ctypes.data))
+    return out[:size]
+
+
+def make_updates(u, v, t=None, graph=0):
+    r = np.zeros(len(u), dtype=EDGE_UPDATE)
+    r["u"] = u
+    r["v"] = v
+    if t is not None:
+        r["type"] = t
+    r["graph"] = graph
+    return r
+
+
+def generate_init_stream(g):
+    """= generate_init_stream (A/src/host/stream_gen.cpp:30-61): all-ADD records."""
+    out = np.zeros(max(g.num_edges // 2, 1), dtype=EDGE_UPDATE)
+    cnt = C.c_uint64()
+    check(lib().sgc_init_stream(g.n, g.row_ptr, g.col_idx, g.degree, out.ctypes.data, C.byref(cnt)))
+    return out[:cnt.value]
+
+
+def generate_bench_stream_edges(g, stream_size, insert_pct, seed):
+    """= generate_bench_stream_edges (A/src/host/stream_gen.cpp:65-177): churn records."""
+    out = np.zeros(max(stream_size, 1), dtype=EDGE_UPDATE)
+    check(lib().sgc_churn_stream(g.n, g.row_ptr, g.col_idx, stream_size, insert_pct, seed, out.ctypes.data))
+    return out[:stream_size]
+
+
+def static_greedy_coloring(g):
+    """= static_greedy_coloring (A/src/streamgc.cu:559-592): max colour of id-order first fit."""
+    mx = C.c_uint32()
+    check(lib().sgc_static_greedy(g.n, g.row_ptr, g.col_idx, C.byref(mx)))
+    return mx.value
+
+
+def edge_intervals(g, stream):
+    """Life intervals [born, died) of every edge over a stream (for the independent validator)."""
+    L = lib()
+    m = C.c_uint64()
+    stream = np.ascontiguousarray(stream, dtype=EDGE_UPDATE)
+    check(L.sgc_edge_intervals(g.n, g.row_ptr, g.col_idx, stream.ctypes.data, len(stream), C.byref(m),
+                               None, None, None, None))
+    eu = np.zeros(max(m.value, 1), np.uint32)
+    ev = np.zeros(max(m.value, 1), np.uint32)
+    born = np.zeros(max(m.value, 1), np.uint64)
+    died = np.zeros(max(m.value, 1), np.uint64)
+    check(L.sgc_edge_intervals(g.n, g.row_ptr, g.col_idx, stream.ctypes.data, len(stream), C.byref(m),
+                               eu.ctypes.data, ev.ctypes.data, born.ctypes.data, died.ctypes.data))
+    k = m.value
+    return eu[:k], ev[:k], born[:k], died[:k]
+
+
+class DeviceStream:
+    """An update stream resident in HBM (uploaded once)."""
+
+    def __init__(self, records):
+        records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
+        self.count = len(records)
+        p = C.c_void_p()
+        check(lib().sgc_device_upload(records.ctypes.data, records.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def at(self, offset):
+        return self.ptr + 16 * int(offset)
+
+    def close(self):
+        if self.ptr:
+            lib().sgc_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+
+class GPUPartition:
+    """One graph resident on the GPU (A/include/streamgc.cuh:26-110 + allocate_gpu_partition)."""
+
+    def __init__(self, g, config=None):
+        self.graph = g
+        self.config = config or StreamGCConfig()
+        h = C.c_void_p()
+        cfg = self.config.to_c()
+        check(lib().sgc_create(C.byref(cfg), g.n, g.row_ptr, g.col_idx if g.num_edges else
+                               np.zeros(1, np.uint32), g.degree if g.n else np.zeros(1, np.uint32),
+                               C.byref(h)))
+        self.h = h.value
+        self.n = g.n
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sgc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ck(self, rc):
+        return check(rc, self.h)
+
+    # -- the hot path
+    def process_stream(self, records, max_epochs=4096):
+        """= process_stream (A/src/streamgc.cu:809-1053).  records: host array or DeviceStream."""
+        st = (_lib.EpochStats * max(max_epochs, 1))()
+        ne = C.c_uint32()
+        if isinstance(records, DeviceStream):
+            self._ck(lib().sgc_submit(self.h, records.ptr, records.count, 1, st, max_epochs, C.byref(ne)))
+        else:
+            records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
+            self._ck(lib().sgc_submit(self.h, records.ctypes.data, len(records), 0, st, max_epochs, C.byref(ne)))
+        return [st[i].as_dict() for i in range(min(ne.value, max_epochs))]
+
+    def submit_device(self, ptr, count, max_epochs=0):
+        st = (_lib.EpochStats * max(max_epochs, 1))()
+        ne = C.c_uint32()
+        self._ck(lib().sgc_submit(self.h, ptr, count, 1, st if max_epochs else None, max_epochs, C.byref(ne)))
+        return [st[i].as_dict() for i in range(min(ne.value, max_epochs))]
+
+    def apply(self, records):
+        """= one local_update_kernel launch per batch, no epoch step."""
+        if isinstance(records, DeviceStream):
+            self._ck(lib().sgc_apply(self.h, records.ptr, records.count, 1))
+        else:
+            records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
+            self._ck(lib().sgc_apply(self.h, records.ctypes.data, len(records), 0))
+
+    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None):
+        """Switch the epoch procedure (and determinism) between phases."""
+        if deterministic is not None:
+            self.config.deterministic = bool(deterministic)
+        if repair_every_epoch is not None:
+            self.config.repair_every_epoch = bool(repair_every_epoch)
+        if exact_histogram is not None:
+            self.config.exact_histogram = bool(exact_histogram)
+        self._ck(lib().sgc_set_flags(self.h, self.config.to_c().flags))
+
+    def epoch(self):
+        st = _lib.EpochStats()
+        self._ck(lib().sgc_epoch(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def finish(self):
+        """= drain_and_compact + run_conflict_repair + cpu_serial_repair."""
+        st = _lib.EpochStats()
+        self._ck(lib().sgc_finish(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def query(self, ids):
+        """= point_query over the published snapshot."""
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        out = np.zeros(len(ids), np.uint16)
+        if len(ids):
+            self._ck(lib().sgc_query(self.h, ids, len(ids), out))
+        return out
+
+    # -- state access
+    def words(self):
+        w = np.zeros(self.n, np.uint32)
+        self._ck(lib().sgc_download_words(self.h, w))
+        return w
+
+    def colors(self):
+        return (self.words() & 0xFFFF).astype(np.uint16)
+
+    def snapshot(self):
+        s = np.zeros(self.n, np.uint16)
+        self._ck(lib().sgc_download_snapshot(self.h, s))
+        return s
+
+    def csr(self):
+        s = C.c_uint64()
+        self._ck(lib().sgc_num_slots(self.h, C.byref(s)))
+        rp = np.zeros(self.n + 1, np.uint32)
+        col = np.zeros(max(s.value, 1), np.uint32)
+        self._ck(lib().sgc_download_csr(self.h, rp, col))
+        return rp, col[:s.value]
+
+    def bitmaps(self):
+        b = np.zeros(self.n, np.uint64)
+        self._ck(lib().sgc_download_bitmaps(self.h, b))
+        return b
+
+    def hist(self):
+        k = C.c_uint32()
+        self._ck(lib().sgc_num_hd(self.h, C.byref(k)))
+        h = np.zeros(max(k.value * 1024, 1), np.uint32)
+        self._ck(lib().sgc_download_hist(self.h, h))
+        return h[:k.value * 1024].reshape(k.value, 1024)
+
+    def set_words(self, w):
+        self._ck(lib().sgc_upload_words(self.h, np.ascontiguousarray(w, np.uint32)))
+
+    def set_bitmaps(self, b):
+        self._ck(lib().sgc_upload_bitmaps(self.h, np.ascontiguousarray(b, np.uint64)))
+
+    def set_hist(self, h):
+        self._ck(lib().sgc_upload_hist(self.h, np.ascontiguousarray(h, np.uint32).ravel()))
+
+    def find_color(self, v):
+        out = C.c_uint16()
+        self._ck(lib().sgc_find_color(self.h, v, C.byref(out)))
+        return out.value
+
+    def overflow_count(self):
+        c = C.c_uint32()
+        self._ck(lib().sgc_overflow_count(self.h, C.byref(c)))
+        return c.value
+
+    def totals(self):
+        t = _lib.Totals()
+        self._ck(lib().sgc_totals_get(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def validate(self, snapshot=False):
+        """= check_coloring_valid on the engine's rows: (violations, uncoloured-with-edges, max colour)."""
+        a, b, m = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        self._ck(lib().sgc_validate(self.h, 1 if snapshot else 0, C.byref(a), C.byref(b), C.byref(m)))
+        return a.value, b.value, m.value
+
+    def history_reserve(self, capacity):
+        self._ck(lib().sgc_history_reserve(self.h, capacity))
+
+    def history_count(self):
+        c = C.c_uint32()
+        self._ck(lib().sgc_history_count(self.h, C.byref(c)))
+        return c.value
+
+    def validate_history(self, intervals, positions):
+        eu, ev, born, died = intervals
+        pos = np.ascontiguousarray(positions, np.uint64)
+        bad = np.zeros(max(len(pos), 1), np.uint64)
+        self._ck(lib().sgc_validate_history(self.h, eu, ev, born, died, len(eu), pos, len(pos), bad))
+        return bad[:len(pos)]
+
+    def kernel_timing(self, enable=True):
+        self._ck(lib().sgc_kernel_timing_enable(self.h, 1 if enable else 0))
+
+    def kernel_times(self):
+        cap = 64
+        stride = 32
+        names = C.create_string_buffer(cap * stride)
+        ms = (C.c_double * cap)()
+        launches = (C.c_uint64 * cap)()
+        cnt = C.c_uint32()
+        self._ck(lib().sgc_kernel_times(self.h, names, stride, ms, launches, cap, C.byref(cnt)))
+        out = {}
+        for i in range(min(cnt.value, cap)):
+            nm = names.raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode()
+            out[nm] = (ms[i], launches[i])
+        return out
+
+    def last_timing(self):
+        a = (C.c_double * 4)()
+        self._ck(lib().sgc_last_timing(self.h, a))
+        return {"update_ms": a[0], "epoch_ms": a[1], "tier2_host_ms": a[2], "wall_ms": a[3]}
+
+
+def check_coloring_valid(partition, snapshot=False):
+    """= check_coloring_valid (A/src/streamgc.cu:448-556)."""
+    return partition.validate(snapshot)

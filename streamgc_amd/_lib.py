@@ -1,0 +1,138 @@
+"""ctypes binding of libstreamgc_amd.so (the C ABI declared in include/streamgc.h).
+
+The library is loaded from this package directory only; there is no fallback:
+a missing or unloadable library raises, so no caller can silently run
+anything but the HIP engine.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libstreamgc_amd.so")
+
+SGC_OK = 0
+SGC_DETERMINISTIC = 1 << 0
+SGC_REPAIR_EVERY_EPOCH = 1 << 1
+SGC_EXACT_HISTOGRAM = 1 << 2
+
+# numpy view of the 16-byte update record (A/include/types.h:16-21)
+EDGE_UPDATE = np.dtype([("u", "<u4"), ("v", "<u4"), ("type", "u1"), ("pad", "u1", 3), ("graph", "<u4")])
+assert EDGE_UPDATE.itemsize == 16
+
+
+class Config(C.Structure):
+    _fields_ = [(k, C.c_uint32) for k in (
+        "epoch_size", "batch_size", "histogram_threshold", "slack_pct", "min_row_capacity",
+        "overflow_capacity", "max_repair_iters", "plateau_iters", "tier2_passes", "flags")]
+
+
+class EpochStats(C.Structure):
+    _fields_ = [(k, C.c_uint32) for k in (
+        "epoch_index", "overflow_count", "overflow_drained", "overflow_failed", "t1_iterations",
+        "t1_conflicts_last", "t2_fixed", "t2_remaining", "max_color", "snapshot_published",
+        "t2_ran", "reserved")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Totals(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in (
+        "updates", "scan_slots", "hd_endpoints", "recolors", "recolor_cap_slots", "recolor_live",
+        "epochs", "t1_sweeps", "t2_runs", "overflow_events", "live_slots", "grouped")] + \
+        [("reserved", C.c_uint64 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+_recp = np.ctypeslib.ndpointer(dtype=EDGE_UPDATE, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+# name -> (argtypes); every function returns int unless listed in _RESTYPE
+SIGNATURES = {
+    "sgc_default_config": [C.POINTER(Config)],
+    "sgc_last_error": [_vp],
+    "sgc_version": [],
+    "sgc_device_count": [],
+    "sgc_create": [C.POINTER(Config), C.c_uint32, _u32p, _u32p, _u32p, C.POINTER(_vp)],
+    "sgc_destroy": [_vp],
+    "sgc_submit": [_vp, _vp, C.c_uint64, C.c_int, C.POINTER(EpochStats), C.c_uint32, C.POINTER(C.c_uint32)],
+    "sgc_apply": [_vp, _vp, C.c_uint64, C.c_int],
+    "sgc_epoch": [_vp, C.POINTER(EpochStats)],
+    "sgc_finish": [_vp, C.POINTER(EpochStats)],
+    "sgc_query": [_vp, _u32p, C.c_uint32, _u16p],
+    "sgc_set_flags": [_vp, C.c_uint32],
+    "sgc_download_words": [_vp, _u32p],
+    "sgc_download_colors": [_vp, _u16p],
+    "sgc_download_snapshot": [_vp, _u16p],
+    "sgc_num_slots": [_vp, C.POINTER(C.c_uint64)],
+    "sgc_download_csr": [_vp, _u32p, _u32p],
+    "sgc_download_bitmaps": [_vp, _u64p],
+    "sgc_num_hd": [_vp, C.POINTER(C.c_uint32)],
+    "sgc_download_hist": [_vp, _u32p],
+    "sgc_upload_words": [_vp, _u32p],
+    "sgc_upload_bitmaps": [_vp, _u64p],
+    "sgc_upload_hist": [_vp, _u32p],
+    "sgc_totals_get": [_vp, C.POINTER(Totals)],
+    "sgc_overflow_count": [_vp, C.POINTER(C.c_uint32)],
+    "sgc_find_color": [_vp, C.c_uint32, C.POINTER(C.c_uint16)],
+    "sgc_validate": [_vp, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
+    "sgc_history_reserve": [_vp, C.c_uint32],
+    "sgc_history_count": [_vp, C.POINTER(C.c_uint32)],
+    "sgc_validate_history": [_vp, _u32p, _u32p, _u64p, _u64p, C.c_uint64, _u64p, C.c_uint32, _u64p],
+    "sgc_device_upload": [_vp, C.c_uint64, C.POINTER(_vp)],
+    "sgc_device_free": [_vp],
+    "sgc_last_timing": [_vp, C.POINTER(C.c_double)],
+    "sgc_kernel_timing_enable": [_vp, C.c_int],
+    "sgc_kernel_times": [_vp, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32,
+                         C.POINTER(C.c_uint32)],
+    "sgc_csr_load_mtx": [C.c_char_p, C.POINTER(_vp)],
+    "sgc_csr_from_edges": [C.c_uint32, _u32p, _u32p, C.c_uint64, C.POINTER(_vp)],
+    "sgc_csr_info": [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
+    "sgc_csr_export": [_vp, _vp, _vp, _vp],
+    "sgc_csr_free": [_vp],
+    "sgc_gen_chung_lu": [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_uint64, C.c_int, C.POINTER(_vp)],
+    "sgc_gen_rmat": [C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(_vp)],
+    "sgc_core_numbers": [C.c_uint32, _u32p, _u32p, _u32p],
+    "sgc_dense_growth_stream": [C.c_uint32, _u32p, _u32p, _u32p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64,
+                                _vp],
+    "sgc_init_stream": [C.c_uint32, _u32p, _u32p, _u32p, _vp, C.POINTER(C.c_uint64)],
+    "sgc_churn_stream": [C.c_uint32, _u32p, _u32p, C.c_uint64, C.c_uint32, C.c_uint64, _vp],
+    "sgc_static_greedy": [C.c_uint32, _u32p, _u32p, C.POINTER(C.c_uint32)],
+    "sgc_edge_intervals": [C.c_uint32, _u32p, _u32p, _vp, C.c_uint64, C.POINTER(C.c_uint64), _vp, _vp, _vp, _vp],
+}
+_RESTYPE = {"sgc_default_config": None, "sgc_last_error": C.c_char_p, "sgc_version": C.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m streamgc_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+class SgcError(RuntimeError):
+    pass
+
+
+def check(rc, engine=None):
+    if rc != SGC_OK:
+        msg = lib().sgc_last_error(engine)
+        raise SgcError(f"streamgc error {rc}: {msg.decode() if msg else ''}")
+    return rc

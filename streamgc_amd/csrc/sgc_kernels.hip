@@ -1,0 +1,605 @@
+// sgc_kernels.hip -- HIP kernels of the StreamGC hot path for gfx950.
+//
+// Work decomposition: updates are one wavefront each (grid-stride), or one
+// wavefront per segment replayed in order (deterministic mode).  Epoch sweeps
+// run one wavefront per vertex row over a virtual vertex space that spans all
+// graphs of a pool (vbase = prefix of vertex counts); rows are streamed 64
+// slots per load.  The deterministic repair sweep runs one wavefront per
+// graph in vertex order.
+#include <hip/hip_runtime.h>
+
+#include "sgc_device.hpp"
+#include "sgc_launch.hpp"
+
+namespace sgc {
+
+__device__ __forceinline__ uint32_t wave_id() { return (blockIdx.x * blockDim.x + threadIdx.x) / WAVE; }
+__device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / WAVE; }
+
+__device__ __forceinline__ void locate(const uint32_t* __restrict__ vbase, uint32_t ng, uint32_t i, uint32_t& gid,
+                                       uint32_t& v) {
+    if (ng == 1) {
+        gid = 0;
+        v = i;
+        return;
+    }
+    uint32_t lo = 0, hi = ng;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (vbase[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    gid = lo;
+    v = i - vbase[lo];
+}
+
+// ---------------------------------------------------------------- per-edge ordering
+// Parallel waves apply a batch in no particular order, so two updates of one
+// edge in one batch (a delete and its re-insert, frequent in churn streams)
+// could land in either order and leave the rows disagreeing with the stream.
+// Before a parallel batch, the updates are grouped by edge (global vertex ids
+// of the pool, lo << 32 | hi) in a hash table: each slot heads a linked list
+// of the batch positions of that edge.  The wave that owns the list head
+// applies the whole group in stream order; different edges stay parallel.
+constexpr uint64_t KEY_EMPTY = ~0ull;
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint64_t edge_key(const GraphDev* __restrict__ graphs, const uint32_t* __restrict__ gbase,
+                                             const uint4& r, int multi) {
+    const uint32_t gid = multi ? r.w : 0u;
+    const uint32_t n = graphs[gid].n;
+    if (r.x >= n || r.y >= n || (r.z & 0xFFu) > 1u) return KEY_EMPTY;
+    const uint32_t base = multi ? gbase[gid] : 0u;
+    const uint32_t a = base + (r.x < r.y ? r.x : r.y), b = base + (r.x < r.y ? r.y : r.x);
+    return ((uint64_t)a << 32) | b;
+}
+__device__ __forceinline__ uint32_t key_hash(uint64_t k, uint32_t mask) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    return (uint32_t)k & mask;
+}
+
+__global__ __launch_bounds__(256) void k_group_clear(uint64_t* __restrict__ keys, uint32_t* __restrict__ head,
+                                                     uint32_t size) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += gridDim.x * blockDim.x) {
+        gp(keys)[i] = KEY_EMPTY;
+        gp(head)[i] = NONE;
+    }
+}
+
+// Slot of `key` (inserted when absent).  The table holds at least twice the
+// batch, so a free slot always exists and the probe ends.
+__device__ __forceinline__ uint32_t group_slot(uint64_t* __restrict__ keys, uint64_t key, uint32_t mask) {
+    uint32_t s = key_hash(key, mask);
+    for (uint32_t probes = 0; probes <= mask; s = (s + 1) & mask, probes++) {
+        uint64_t found = KEY_EMPTY;
+        __hip_atomic_compare_exchange_strong(gp(&keys[s]), &found, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        if (found == KEY_EMPTY || found == key) return s;
+    }
+    return NONE;
+}
+__device__ __forceinline__ uint32_t group_find(const uint64_t* __restrict__ keys, uint64_t key, uint32_t mask) {
+    uint32_t s = key_hash(key, mask);
+    for (uint32_t probes = 0; probes <= mask; s = (s + 1) & mask, probes++) {
+        const uint64_t k = gp(keys)[s];
+        if (k == key) return s;
+        if (k == KEY_EMPTY) return NONE;
+    }
+    return NONE;
+}
+
+__global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__ graphs,
+                                                    const uint32_t* __restrict__ gbase,
+                                                    const uint4* __restrict__ upd, uint32_t count, int multi,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ head,
+                                                    uint32_t* __restrict__ next, uint32_t mask) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
+        uint32_t prev = NONE;
+        if (key != KEY_EMPTY) {
+            const uint32_t s = group_slot(keys, key, mask);
+            if (s != NONE)
+                prev = __hip_atomic_exchange(gp(&head[s]), i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        gp(next)[i] = prev;
+    }
+}
+
+// ---------------------------------------------------------------- updates
+__device__ __forceinline__ void apply_record(const GraphDev* __restrict__ graphs, const uint4* __restrict__ upd,
+                                             uint32_t i, int multi, WaveAcc& acc) {
+    const uint4 r = upd[i];
+    const uint32_t gid = multi ? r.w : 0u;
+    const GraphDev g = graphs[gid];
+    apply_update(g, r.x, r.y, r.z & 0xFFu, acc);
+}
+
+__global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__ graphs,
+                                                    const uint4* __restrict__ upd, uint32_t count, int multi,
+                                                    const uint32_t* __restrict__ gbase,
+                                                    const uint64_t* __restrict__ gkeys,
+                                                    const uint32_t* __restrict__ ghead,
+                                                    const uint32_t* __restrict__ gnext, uint32_t gmask,
+                                                    uint64_t* totals) {
+    WaveAcc acc;
+    uint64_t grouped = 0;
+    for (uint32_t i = wave_id(); i < count; i += wave_count()) {
+        if (!gkeys) {
+            apply_record(graphs, upd, i, multi, acc);
+            continue;
+        }
+        const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
+        const uint32_t s = key == KEY_EMPTY ? NONE : group_find(gkeys, key, gmask);
+        if (s == NONE) {                       // not an edge update of a live graph: apply alone
+            apply_record(graphs, upd, i, multi, acc);
+            continue;
+        }
+        const uint32_t h = gp(ghead)[s];
+        if (h != i) continue;                  // the list head's wave applies this group
+        // Apply the group in ascending stream position: repeatedly take the
+        // smallest position above the last one applied (groups are short).
+        uint32_t last = 0;
+        bool first = true;
+        for (;;) {
+            uint32_t best = NONE;
+            for (uint32_t k = h; k != NONE; k = gp(gnext)[k])
+                if ((first || k > last) && (best == NONE || k < best)) best = k;
+            if (best == NONE) break;
+            apply_record(graphs, upd, best, multi, acc);
+            grouped += best != i;
+            last = best;
+            first = false;
+        }
+    }
+    acc.flush(totals);
+    if (grouped && totals && lane() == 0) add64(&totals[T_GROUPED], grouped);
+}
+
+__global__ __launch_bounds__(256) void k_apply_seg(const GraphDev* __restrict__ graphs,
+                                                   const uint4* __restrict__ upd, const uint32_t* __restrict__ seg,
+                                                   uint32_t nseg, int multi, uint64_t* totals) {
+    WaveAcc acc;
+    for (uint32_t k = wave_id(); k < nseg; k += wave_count())
+        for (uint32_t i = seg[2 * k]; i < seg[2 * k + 1]; i++) apply_record(graphs, upd, i, multi, acc);
+    acc.flush(totals);
+}
+
+// ---------------------------------------------------------------- snapshot
+// A/src/kernels/epoch_maintenance.cu:5-16, into the back buffer.
+__global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                  const uint32_t* __restrict__ vbase, uint32_t V,
+                                                  const uint32_t* __restrict__ active) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        uint32_t gid, v;
+        locate(vbase, ng, i, gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        gp(g.snap_write)[v] = color_of(gp(g.word)[v]);
+    }
+}
+
+// ---------------------------------------------------------------- compaction
+// A/src/kernels/epoch_maintenance.cu:20-61: stable left-pack of the live
+// slots, free slots after them.  One pass, writing only slots whose content
+// changes: a live value moves to w + (live lanes before it); a slot whose
+// value moved and that lies past the chunk's new write front is freed (a
+// later chunk may refill it, in program order).
+__global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                 const uint32_t* __restrict__ vbase, uint32_t V,
+                                                 const uint32_t* __restrict__ active, uint64_t* totals) {
+    uint64_t live_total = 0;
+    const uint64_t lt_mask = (1ull << lane()) - 1ull;
+    for (uint32_t i = wave_id(); i < V; i += wave_count()) {
+        uint32_t gid, v;
+        locate(vbase, ng, i, gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+        uint32_t w = s;
+        for (uint32_t base = s; base < e; base += WAVE) {
+            const uint32_t j = base + lane();
+            const bool in = j < e;
+            const uint32_t val = in ? gp(g.col)[j] : INVALID;
+            const bool live = val != INVALID;
+            const uint64_t m = ballot(live);
+            const uint32_t pos = w + (uint32_t)__builtin_popcountll(m & lt_mask);
+            const uint32_t w_after = w + (uint32_t)__builtin_popcountll(m);
+            if (live && pos != j) gp(g.col)[pos] = val;
+            if (live && pos != j && j >= w_after) gp(g.col)[j] = INVALID;
+            w = w_after;
+        }
+        live_total += w - s;
+    }
+    if (totals && lane() == 0 && live_total) add64(&totals[T_LIVE_SLOTS], live_total);
+}
+
+// ---------------------------------------------------------------- overflow drain
+// A/src/streamgc.cu:972-983: read the counter, reset it, clamp to capacity.
+__global__ void k_drain_prep(const GraphDev* __restrict__ graphs, uint32_t ng, const uint32_t* __restrict__ active) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= ng || (active && !active[gid])) return;
+    SGC_G uint32_t* c = gp(graphs[gid].ctr);
+    uint32_t raw = c[C_OVF];
+    c[C_OVF_RAW] = raw;
+    c[C_OVF_SNAP] = raw < graphs[gid].ovf_cap ? raw : graphs[gid].ovf_cap;
+    c[C_OVF] = 0;
+    c[C_DRAIN_FAILED] = 0;
+}
+
+// A/src/kernels/epoch_maintenance.cu:66-87: one wavefront per entry, graph = blockIdx.y.
+__global__ __launch_bounds__(256) void k_drain_elem(const GraphDev* __restrict__ graphs,
+                                                    const uint32_t* __restrict__ active) {
+    const uint32_t gid = blockIdx.y;
+    if (active && !active[gid]) return;
+    const GraphDev g = graphs[gid];
+    const uint32_t count = gp(g.ctr)[C_OVF_SNAP];
+    uint32_t failed = 0;
+    for (uint32_t k = wave_id(); k < count; k += wave_count()) {
+        bool f = false;
+        row_insert(g, gp(g.ovf)[2 * (size_t)k], gp(g.ovf)[2 * (size_t)k + 1], false, &f, nullptr);
+        failed += f;
+    }
+    if (failed && lane() == 0) add32(&g.ctr[C_DRAIN_FAILED], failed);
+}
+
+// The same in buffer order, one wavefront per graph (deterministic mode).
+__global__ __launch_bounds__(64) void k_drain_serial(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                     const uint32_t* __restrict__ active) {
+    for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
+        if (active && !active[gid]) continue;
+        const GraphDev g = graphs[gid];
+        const uint32_t count = gp(g.ctr)[C_OVF_SNAP];
+        uint32_t failed = 0;
+        for (uint32_t k = 0; k < count; k++) {
+            bool f = false;
+            row_insert(g, gp(g.ovf)[2 * (size_t)k], gp(g.ovf)[2 * (size_t)k + 1], false, &f, nullptr);
+            failed += f;
+        }
+        if (lane() == 0) gp(g.ctr)[C_DRAIN_FAILED] = failed;
+    }
+}
+
+// ---------------------------------------------------------------- bitmap rebuild
+// A/include/color_select.cuh:213-251, A/src/kernels/epoch_maintenance.cu:113-128:
+// bitmap-mode rows get the exact OR of their live neighbours' colours <= 64.
+__global__ __launch_bounds__(256) void k_rebuild(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                 const uint32_t* __restrict__ vbase, uint32_t V,
+                                                 const uint32_t* __restrict__ active) {
+    for (uint32_t i = wave_id(); i < V; i += wave_count()) {
+        uint32_t gid, v;
+        locate(vbase, ng, i, gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        if (gp(g.hd_map)[v] != INVALID) continue;
+        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+        uint64_t bm = 0;
+        for (uint32_t base = s; base < e; base += WAVE) {
+            const uint32_t j = base + lane();
+            const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
+            if (nb != INVALID && nb < g.n) {
+                const uint32_t c = color_of(gp(g.word)[nb]);
+                if (c != 0 && c <= 64) bm |= 1ull << (c - 1);
+            }
+        }
+        bm = wave_or64(bm);
+        if (lane() == 0) gp(g.bitmap)[v] = bm;
+    }
+}
+
+// ---------------------------------------------------------------- histogram rebuild in LDS
+// Exact-histogram mode: every hub's 1024 counters are recounted from its live
+// row (colours 1..1024), one wavefront per hub, the bins in that wavefront's
+// own 4 KiB of LDS, then written out as 16 coalesced dwords per lane.  The
+// reference never rebuilds histograms (A/src/kernels/epoch_maintenance.cu:120-123),
+// so counts drift; this mode keeps them exact at every rebuild.
+__global__ __launch_bounds__(256) void k_rebuild_hist(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                      const uint32_t* __restrict__ hbase, uint32_t H,
+                                                      const uint32_t* __restrict__ active) {
+    __shared__ uint32_t bins[4][HIST_BINS];
+    uint32_t* b = bins[threadIdx.x / WAVE];
+    for (uint32_t i = wave_id(); i < H; i += wave_count()) {
+        uint32_t gid, hd;
+        locate(hbase, ng, i, gid, hd);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t v = gp(g.hubs)[hd];
+        for (uint32_t k = lane(); k < HIST_BINS; k += WAVE) b[k] = 0;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+        for (uint32_t base = s; base < e; base += WAVE) {
+            const uint32_t j = base + lane();
+            const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
+            if (nb != INVALID && nb < g.n) {
+                const uint32_t c = color_of(gp(g.word)[nb]);
+                if (c != 0 && c <= HIST_BINS) atomicAdd(&b[c - 1], 1u);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        SGC_G uint32_t* dst = gp(g.hist) + (size_t)hd * HIST_BINS;
+        for (uint32_t k = lane(); k < HIST_BINS; k += WAVE) dst[k] = b[k];
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---------------------------------------------------------------- tier-1 repair
+// A/src/kernels/conflict_repair.cu:10-114 for one vertex; returns 1 if counted.
+__device__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) {
+    const uint32_t w = bcast0(ld(&g.word[v]));
+    const uint16_t mine = color_of(w);
+    const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+    if (mine == 0) {
+        bool has_nb = false;
+        for (uint32_t base = s; base < e && !has_nb; base += WAVE) {
+            const uint32_t j = base + lane();
+            has_nb = ballot(j < e && ld(&g.col[j]) != INVALID) != 0;
+        }
+        if (!has_nb) return 0;
+        const uint16_t c = find_color(g, v);
+        if (lane() == 0) st(&g.word[v], bump(w, c));
+        return 1;
+    }
+    const uint32_t my_deg = gp(g.deg0)[v];
+    bool need = false;
+    for (uint32_t base = s; base < e && !need; base += WAVE) {
+        const uint32_t j = base + lane();
+        const uint32_t nb = j < e ? ld(&g.col[j]) : INVALID;
+        bool hit = false;
+        if (nb != INVALID && nb < g.n && color_of(ld(&g.word[nb])) == mine) {
+            const uint32_t d = gp(g.deg0)[nb];
+            hit = (my_deg < d) || (my_deg == d && v < nb);  // pi(v) < pi(nb)
+        }
+        need = ballot(hit) != 0;
+    }
+    if (!need) return 0;
+    if (mine <= 64 && lane() == 0) or64(&g.bitmap[v], 1ull << (mine - 1));
+    const uint16_t c = find_color(g, v);
+    if (lane() == 0) st(&g.word[v], bump(w, c));
+    return 1;
+}
+
+__global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                    const uint32_t* __restrict__ vbase, uint32_t V,
+                                                    const uint32_t* __restrict__ active) {
+    uint32_t cnt = 0, cur_gid = 0xFFFFFFFFu;
+    for (uint32_t i = wave_id(); i < V; i += wave_count()) {
+        uint32_t gid, v;
+        locate(vbase, ng, i, gid, v);
+        if (active && !active[gid]) continue;
+        if (gid != cur_gid) {
+            if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
+            cnt = 0;
+            cur_gid = gid;
+        }
+        cnt += tier1_vertex(graphs[gid], v);
+    }
+    if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
+}
+
+// Vertex order on one wavefront per graph (Gauss-Seidel, deterministic).
+__global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                     const uint32_t* __restrict__ active) {
+    for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
+        if (active && !active[gid]) continue;
+        const GraphDev g = graphs[gid];
+        uint32_t cnt = 0;
+        for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex(g, v);
+        if (lane() == 0) add32(&g.ctr[C_CONFLICTS], cnt);
+    }
+}
+
+// ---------------------------------------------------------------- checks
+// Tier-2 has work iff some row slot joins an uncoloured endpoint or two
+// endpoints of one colour (A/src/streamgc.cu:1190-1218 then change nothing).
+__global__ __launch_bounds__(256) void k_t2_check(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                  const uint32_t* __restrict__ vbase, uint32_t V,
+                                                  const uint32_t* __restrict__ active) {
+    for (uint32_t i = wave_id(); i < V; i += wave_count()) {
+        uint32_t gid, v;
+        locate(vbase, ng, i, gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+        const uint16_t cv = color_of(gp(g.word)[v]);
+        uint32_t bad = 0;
+        for (uint32_t base = s; base < e; base += WAVE) {
+            const uint32_t j = base + lane();
+            const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
+            if (nb != INVALID && nb < g.n && nb != v) {
+                const uint16_t cn = color_of(gp(g.word)[nb]);
+                bad += (cv == 0 || cn == 0 || cv == cn);
+            }
+        }
+        bad = wave_sum32(bad);
+        if (bad && lane() == 0) add32(&g.ctr[C_T2BAD], bad);
+    }
+}
+
+// check_coloring_valid on the engine's rows (A/src/streamgc.cu:448-556);
+// colours from `colors` when given (a snapshot), else the live words.
+__global__ __launch_bounds__(256) void k_validate(const GraphDev* __restrict__ graphs, uint32_t gid,
+                                                  const uint16_t* __restrict__ colors) {
+    const GraphDev& g = graphs[gid];
+    uint32_t viol = 0, unc = 0, mx = 0;
+    for (uint32_t v = wave_id(); v < g.n; v += wave_count()) {
+        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+        const uint16_t cv = colors ? colors[v] : color_of(gp(g.word)[v]);
+        uint32_t bad = 0, has = 0;
+        for (uint32_t base = s; base < e; base += WAVE) {
+            const uint32_t j = base + lane();
+            const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
+            if (nb != INVALID) {
+                has = 1;
+                if (nb < g.n) {
+                    const uint16_t cn = colors ? colors[nb] : color_of(gp(g.word)[nb]);
+                    bad += (cv != 0 && cv == cn);
+                }
+            }
+        }
+        viol += wave_sum32(bad);
+        unc += (cv == 0 && ballot(has != 0)) ? 1 : 0;
+        mx = cv > mx ? cv : mx;
+    }
+    if (lane() == 0) {
+        if (viol) add32(&g.ctr[C_VIOL], viol);
+        if (unc) add32(&g.ctr[C_UNCOL], unc);
+        max32(&g.ctr[C_MAXC], mx);
+    }
+}
+
+__global__ void k_find_color(const GraphDev* __restrict__ graphs, uint32_t gid, uint32_t v, uint16_t* out) {
+    const GraphDev g = graphs[gid];
+    const uint16_t c = find_color(g, v);
+    if (lane() == 0) *out = c;
+}
+
+// Independent validator: edges with life intervals checked against snapshot
+// k at stream position pos[k] (blockIdx.y = k).
+__global__ __launch_bounds__(256) void k_validate_history(const uint16_t* __restrict__ snaps, uint32_t n,
+                                                          const uint32_t* __restrict__ eu,
+                                                          const uint32_t* __restrict__ ev,
+                                                          const uint64_t* __restrict__ born,
+                                                          const uint64_t* __restrict__ died, uint64_t m,
+                                                          const uint64_t* __restrict__ pos,
+                                                          unsigned long long* bad_out) {
+    const uint32_t k = blockIdx.y;
+    const uint16_t* snap = snaps + (size_t)k * n;
+    const uint64_t p = pos[k];
+    uint32_t bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (born[i] <= p && p < died[i]) {
+            const uint16_t a = snap[eu[i]], b = snap[ev[i]];
+            bad += (a == 0) + (b == 0) + (a != 0 && a == b);
+        }
+    }
+    bad = wave_sum32(bad);
+    if (bad && lane() == 0) atomicAdd(&bad_out[k], (unsigned long long)bad);
+}
+
+// Slack-CSR construction (A/src/streamgc.cu:350-433): copy each loaded row to
+// the front of its slack row, free slots after it.
+__global__ __launch_bounds__(256) void k_fill_rows(uint32_t n, const uint32_t* __restrict__ rp0,
+                                                   const uint32_t* __restrict__ col0,
+                                                   const uint32_t* __restrict__ rp, uint32_t* __restrict__ col) {
+    for (uint32_t v = wave_id(); v < n; v += wave_count()) {
+        const uint32_t s0 = rp0[v], d = rp0[v + 1] - s0;
+        const uint32_t s = rp[v], e = rp[v + 1];
+        for (uint32_t j = s + lane(); j < e; j += WAVE) {
+            const uint32_t k = j - s;
+            col[j] = k < d ? col0[s0 + k] : INVALID;
+        }
+    }
+}
+
+}  // namespace sgc
+
+// ---------------------------------------------------------------- launchers
+namespace sgc {
+
+static inline uint32_t grid_for_waves(uint64_t waves, uint32_t waves_per_block, uint32_t cap) {
+    uint64_t b = (waves + waves_per_block - 1) / waves_per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (uint32_t)b;
+}
+
+hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg) {
+    if (count == 0) return hipSuccess;
+    const uint64_t* keys = nullptr;
+    const uint32_t *head = nullptr, *next = nullptr;
+    uint32_t mask = 0;
+    if (eg && eg->keys && count > 1) {
+        if ((uint64_t)eg->size < 2ull * count || eg->next_cap < count) return hipErrorInvalidValue;
+        keys = eg->keys;
+        head = eg->head;
+        next = eg->next;
+        mask = eg->size - 1;
+        uint32_t cg = grid_for_waves(eg->size, 4 * WAVE, 1024);
+        k_group_clear<<<cg, 256, 0, L.stream>>>(eg->keys, eg->head, eg->size);
+        uint32_t mg = grid_for_waves(count, 4 * WAVE, 1024);
+        k_group_link<<<mg, 256, 0, L.stream>>>(L.graphs, L.gbase, (const uint4*)upd, count, multi, eg->keys,
+                                               eg->head, eg->next, mask);
+    }
+    uint32_t grid = grid_for_waves(count, 4, L.max_blocks);
+    k_apply_elem<<<grid, 256, 0, L.stream>>>(L.graphs, (const uint4*)upd, count, multi, L.gbase, keys, head, next,
+                                              mask, L.totals);
+    return hipGetLastError();
+}
+hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg, int multi) {
+    if (nseg == 0) return hipSuccess;
+    uint32_t grid = grid_for_waves(nseg, 1, L.max_blocks);
+    k_apply_seg<<<grid, 64, 0, L.stream>>>(L.graphs, (const uint4*)upd, seg, nseg, multi, L.totals);
+    return hipGetLastError();
+}
+hipError_t launch_snapshot(const LaunchCtx& L) {
+    uint32_t grid = grid_for_waves(L.V, 4 * WAVE, L.max_blocks);
+    k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
+    return hipGetLastError();
+}
+hipError_t launch_compact(const LaunchCtx& L) {
+    uint32_t grid = grid_for_waves(L.V, 4, L.max_blocks);
+    k_compact<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.totals);
+    return hipGetLastError();
+}
+hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
+    k_drain_prep<<<(L.ng + 63) / 64, 64, 0, L.stream>>>(L.graphs, L.ng, L.active);
+    if (serial) {
+        k_drain_serial<<<L.ng < 1024 ? L.ng : 1024, 64, 0, L.stream>>>(L.graphs, L.ng, L.active);
+    } else {
+        dim3 grid(grid_for_waves(max_count, 4, 4096), L.ng);
+        k_drain_elem<<<grid, 256, 0, L.stream>>>(L.graphs, L.active);
+    }
+    return hipGetLastError();
+}
+hipError_t launch_rebuild(const LaunchCtx& L) {
+    uint32_t grid = grid_for_waves(L.V, 4, L.max_blocks);
+    k_rebuild<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
+    return hipGetLastError();
+}
+hipError_t launch_rebuild_hist(const LaunchCtx& L) {
+    if (L.H == 0) return hipSuccess;
+    uint32_t grid = grid_for_waves(L.H, 4, L.max_blocks);
+    k_rebuild_hist<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.hbase, L.H, L.active);
+    return hipGetLastError();
+}
+hipError_t launch_tier1(const LaunchCtx& L, bool serial) {
+    if (serial) {
+        k_tier1_serial<<<L.ng < 1024 ? L.ng : 1024, 64, 0, L.stream>>>(L.graphs, L.ng, L.active);
+    } else {
+        uint32_t grid = grid_for_waves(L.V, 4, L.max_blocks);
+        k_tier1_elem<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
+    }
+    return hipGetLastError();
+}
+hipError_t launch_t2_check(const LaunchCtx& L) {
+    uint32_t grid = grid_for_waves(L.V, 4, L.max_blocks);
+    k_t2_check<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
+    return hipGetLastError();
+}
+hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors) {
+    uint32_t grid = grid_for_waves(n, 4, L.max_blocks);
+    k_validate<<<grid, 256, 0, L.stream>>>(L.graphs, gid, colors);
+    return hipGetLastError();
+}
+hipError_t launch_find_color(const LaunchCtx& L, uint32_t gid, uint32_t v, uint16_t* out) {
+    k_find_color<<<1, 64, 0, L.stream>>>(L.graphs, gid, v, out);
+    return hipGetLastError();
+}
+hipError_t launch_validate_history(hipStream_t s, const uint16_t* snaps, uint32_t n, const uint32_t* eu,
+                                   const uint32_t* ev, const uint64_t* born, const uint64_t* died, uint64_t m,
+                                   const uint64_t* pos, uint32_t count, unsigned long long* bad_out) {
+    if (count == 0 || m == 0) return hipSuccess;
+    uint64_t blocks = (m + 255) / 256;
+    dim3 grid((uint32_t)(blocks < 2048 ? blocks : 2048), count);
+    k_validate_history<<<grid, 256, 0, s>>>(snaps, n, eu, ev, born, died, m, pos, bad_out);
+    return hipGetLastError();
+}
+hipError_t launch_fill_rows(hipStream_t s, uint32_t n, const uint32_t* rp0, const uint32_t* col0, const uint32_t* rp,
+                            uint32_t* col) {
+    uint32_t grid = grid_for_waves(n, 4, 16384);
+    k_fill_rows<<<grid, 256, 0, s>>>(n, rp0, col0, rp, col);
+    return hipGetLastError();
+}
+
+}  // namespace sgc

@@ -1,0 +1,53 @@
+// sgc_launch.hpp -- host-side launch interface of the StreamGC kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgc {
+
+struct GraphDev;
+
+// One launch context: the graph descriptors (device array), the virtual
+// vertex space of the pool, optional per-graph active flags and totals.
+struct LaunchCtx {
+    hipStream_t stream = nullptr;
+    const GraphDev* graphs = nullptr;  // device array of ng descriptors
+    uint32_t ng = 1;
+    const uint32_t* vbase = nullptr;   // device, ng + 1 prefix of vertex counts (null when ng == 1)
+    uint32_t V = 0;                    // total vertices
+    const uint32_t* active = nullptr;  // device, ng flags (null = all)
+    const uint32_t* gbase = nullptr;   // device, ng global-vertex offsets (pool edge keys)
+    const uint32_t* hbase = nullptr;   // device, ng + 1 prefix of hub counts (null when ng == 1)
+    uint32_t H = 0;                    // total hubs
+    uint64_t* totals = nullptr;        // device u64 counters
+    uint32_t max_blocks = 16384;
+};
+
+// Per-edge grouping of a parallel batch: hash table (size: power of two >=
+// 2 x batch) of edge keys with list heads, and a next link per update.
+struct EdgeGroups {
+    uint64_t* keys = nullptr;
+    uint32_t* head = nullptr;
+    uint32_t* next = nullptr;
+    uint32_t size = 0;
+    uint32_t next_cap = 0;
+};
+
+hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg);
+hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg, int multi);
+hipError_t launch_snapshot(const LaunchCtx& L);
+hipError_t launch_compact(const LaunchCtx& L);
+hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
+hipError_t launch_rebuild(const LaunchCtx& L);
+hipError_t launch_rebuild_hist(const LaunchCtx& L);
+hipError_t launch_tier1(const LaunchCtx& L, bool serial);
+hipError_t launch_t2_check(const LaunchCtx& L);
+hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors);
+hipError_t launch_find_color(const LaunchCtx& L, uint32_t gid, uint32_t v, uint16_t* out);
+hipError_t launch_validate_history(hipStream_t s, const uint16_t* snaps, uint32_t n, const uint32_t* eu,
+                                   const uint32_t* ev, const uint64_t* born, const uint64_t* died, uint64_t m,
+                                   const uint64_t* pos, uint32_t count, unsigned long long* bad_out);
+hipError_t launch_fill_rows(hipStream_t s, uint32_t n, const uint32_t* rp0, const uint32_t* col0, const uint32_t* rp,
+                            uint32_t* col);
+
+}  // namespace sgc

@@ -36,6 +36,7 @@ class Stats(C.Structure):
 
 FLAG_DETERMINISTIC = 1 << 0
 FLAG_REPAIR_EVERY_EPOCH = 1 << 1
+FLAG_EXACT_HISTOGRAM = 1 << 2
 
 
 def default_config(**kw):
@@ -72,6 +73,7 @@ def lib():
         L.orc_engine_create.argtypes = [C.c_uint32, _u32p, _u32p, _u32p, C.POINTER(Config)]
         L.orc_engine_destroy.argtypes = [C.c_void_p]
         L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_flags.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_submit.restype = C.c_uint32
         L.orc_submit.argtypes = [C.c_void_p, _u32p, _u32p, _u8p, C.c_uint64, C.POINTER(Stats), C.c_uint32]
         L.orc_apply.argtypes = [C.c_void_p, _u32p, _u32p, _u8p, C.c_uint64]
@@ -199,6 +201,10 @@ class Engine:
         if getattr(self, "h", None):
             lib().orc_engine_destroy(self.h)
             self.h = None
+
+    def set_flags(self, flags):
+        self.cfg.flags = flags
+        lib().orc_set_flags(self.h, flags)
 
     def submit(self, u, v, t=None, max_epochs=4096):
         u = np.ascontiguousarray(u, np.uint32)
