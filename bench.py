@@ -115,7 +115,7 @@ def main():
     bench = S.generate_bench_stream_edges(g, (W + K) * E, 50, 42)
     log(f"streams: init {len(init)}, churn {len(bench)} ({time.perf_counter() - t:.1f}s)")
 
-    eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E))
+    eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True))
     t = time.perf_counter()
     ds_init = S.DeviceStream(init)
     eng.process_stream(ds_init, max_epochs=0)

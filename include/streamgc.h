@@ -196,6 +196,60 @@ int sgc_kernel_timing_enable(sgc_engine* e, int enable);
 int sgc_kernel_times(sgc_engine* e, char* names, uint32_t name_stride, double* total_ms, uint64_t* launches,
                      uint32_t cap, uint32_t* count);
 
+/* ---- pools: many graphs resident on one GPU (batched and served modes) ----
+ * No reference counterpart: the reference runs one graph per process.  Every
+ * phase is one launch over all graphs taking part.  Records carry the graph
+ * id in bytes 12..15 (sgc_edge_update.pad[3..6]).  Flags are pool-wide (the
+ * sgc_config flags of sgc_pool_add are ignored).  Deterministic pools replay
+ * each graph's updates in stream order on one wavefront per graph, so every
+ * graph's result is the same alone and in the mix. */
+typedef struct sgc_pool sgc_pool;
+typedef struct sgc_pool_config {
+    uint32_t max_graphs;       /* graph slots */
+    uint32_t batch_capacity;   /* largest batch one launch takes (all graphs together) */
+    uint32_t flags;            /* SGC_* flags for every graph */
+    uint32_t reserved;
+    uint64_t hbm_budget;       /* bytes the pool's graphs may occupy (0 = unlimited) */
+} sgc_pool_config;
+
+const char* sgc_pool_last_error(const sgc_pool* p);
+int sgc_pool_create(const sgc_pool_config* cfg, sgc_pool** out);
+int sgc_pool_destroy(sgc_pool* p);
+/* Device bytes a graph would occupy (what admission charges against the budget). */
+int sgc_pool_graph_bytes(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* deg0,
+                         uint64_t* bytes);
+/* Admit a graph: SGC_ERR_NOMEM when the pool is full or the budget would be exceeded. */
+int sgc_pool_add(sgc_pool* p, const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+                 const uint32_t* deg0, uint32_t* gid);
+int sgc_pool_remove(sgc_pool* p, uint32_t gid);
+int sgc_pool_usage(sgc_pool* p, uint64_t* used_bytes, uint64_t* budget, uint32_t* graphs);
+int sgc_pool_set_flags(sgc_pool* p, uint32_t flags);
+/* One launch per batch of up to batch_capacity records, no epoch step. */
+int sgc_pool_apply(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, int on_device);
+/* Graphs whose updates since their last epoch reached their epoch_size. */
+int sgc_pool_due(sgc_pool* p, uint32_t* gids, uint32_t cap, uint32_t* count);
+/* One epoch step for a set of graphs, in shared launches. */
+int sgc_pool_epoch(sgc_pool* p, const uint32_t* gids, uint32_t count, sgc_epoch_stats* out);
+/* End-of-phase drain, repair and publish for a set of graphs. */
+int sgc_pool_finish(sgc_pool* p, const uint32_t* gids, uint32_t count, sgc_epoch_stats* out);
+/* process_stream over a mixed stream: batches, then one shared epoch for the due graphs. */
+int sgc_pool_submit(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, int on_device,
+                    sgc_epoch_stats* stats_out, uint32_t max_stats, uint32_t* nstats);
+int sgc_pool_query(sgc_pool* p, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors);
+int sgc_pool_download_words(sgc_pool* p, uint32_t gid, uint32_t* words);
+int sgc_pool_download_snapshot(sgc_pool* p, uint32_t gid, uint16_t* colors);
+int sgc_pool_validate(sgc_pool* p, uint32_t gid, int snapshot, uint64_t* violations, uint64_t* uncolored_with_edges,
+                      uint32_t* max_color);
+int sgc_pool_history_reserve(sgc_pool* p, uint32_t gid, uint32_t capacity);
+int sgc_pool_history_count(sgc_pool* p, uint32_t gid, uint32_t* count);
+int sgc_pool_validate_history(sgc_pool* p, uint32_t gid, const uint32_t* eu, const uint32_t* ev, const uint64_t* born,
+                              const uint64_t* died, uint64_t m, const uint64_t* pos, uint32_t count,
+                              uint64_t* bad_out);
+int sgc_pool_totals(sgc_pool* p, sgc_totals* out);
+int sgc_pool_kernel_timing_enable(sgc_pool* p, int enable);
+int sgc_pool_kernel_times(sgc_pool* p, char* names, uint32_t name_stride, double* total_ms, uint64_t* launches,
+                          uint32_t cap, uint32_t* count);
+
 /* ---- host pipeline (C++17) ---- */
 
 /* Graph container (host CSR, symmetric, sorted rows, no self-loops). */

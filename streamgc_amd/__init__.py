@@ -20,7 +20,9 @@ __all__ = ["StreamGCConfig", "default_config", "CSRGraph", "GPUPartition", "Devi
            "load_graph_csr", "csr_from_edges", "csr_from_edge_arrays", "gen_chung_lu", "gen_rmat",
            "gen_delaunay", "core_numbers", "dense_growth_stream", "generate_init_stream", "generate_bench_stream_edges",
            "static_greedy_coloring", "check_coloring_valid", "edge_intervals", "make_updates", "SgcError",
-           "device_count"]
+           "SgcNoMemory", "device_count", "GraphPool", "tag_graph"]
+
+from ._lib import SgcNoMemory  # noqa: E402
 
 UPDATE_ADD = 0
 UPDATE_DEL = 1
@@ -423,6 +425,171 @@ class GPUPartition:
         a = (C.c_double * 4)()
         self._ck(lib().sgc_last_timing(self.h, a))
         return {"update_ms": a[0], "epoch_ms": a[1], "tier2_host_ms": a[2], "wall_ms": a[3]}
+
+
+class GraphPool:
+    """Many graphs resident on one GPU; every phase is one launch over all graphs taking part.
+
+    Records carry the graph id in their `graph` field.  Flags are pool-wide.
+    """
+
+    def __init__(self, max_graphs=256, batch_capacity=1 << 18, hbm_budget=0, deterministic=False,
+                 repair_every_epoch=False, exact_histogram=False):
+        flags = StreamGCConfig(deterministic=deterministic, repair_every_epoch=repair_every_epoch,
+                               exact_histogram=exact_histogram).to_c().flags
+        cfg = _lib.PoolConfig(max_graphs, batch_capacity, flags, 0, hbm_budget)
+        h = C.c_void_p()
+        check(lib().sgc_pool_create(C.byref(cfg), C.byref(h)))
+        self.h = h.value
+        self.flags = flags
+        self.batch_capacity = batch_capacity
+        self.graphs = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sgc_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ck(self, rc):
+        return check(rc, pool=self.h)
+
+    @staticmethod
+    def graph_bytes(g, config=None):
+        cfg = (config or StreamGCConfig()).to_c()
+        b = C.c_uint64()
+        check(lib().sgc_pool_graph_bytes(C.byref(cfg), g.n, g.row_ptr, g.degree, C.byref(b)))
+        return b.value
+
+    def add(self, g, config=None):
+        """Admit a graph; raises SgcNoMemory when it does not fit."""
+        cfg = (config or StreamGCConfig()).to_c()
+        gid = C.c_uint32()
+        col = g.col_idx if g.num_edges else np.zeros(1, np.uint32)
+        self._ck(lib().sgc_pool_add(self.h, C.byref(cfg), g.n, g.row_ptr, col, g.degree, C.byref(gid)))
+        self.graphs[gid.value] = g
+        return gid.value
+
+    def remove(self, gid):
+        self._ck(lib().sgc_pool_remove(self.h, gid))
+        self.graphs.pop(gid, None)
+
+    def usage(self):
+        u, b, k = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        self._ck(lib().sgc_pool_usage(self.h, C.byref(u), C.byref(b), C.byref(k)))
+        return u.value, b.value, k.value
+
+    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None):
+        c = StreamGCConfig(deterministic=bool(self.flags & _lib.SGC_DETERMINISTIC),
+                           repair_every_epoch=bool(self.flags & _lib.SGC_REPAIR_EVERY_EPOCH),
+                           exact_histogram=bool(self.flags & _lib.SGC_EXACT_HISTOGRAM))
+        if deterministic is not None:
+            c.deterministic = deterministic
+        if repair_every_epoch is not None:
+            c.repair_every_epoch = repair_every_epoch
+        if exact_histogram is not None:
+            c.exact_histogram = exact_histogram
+        self.flags = c.to_c().flags
+        self._ck(lib().sgc_pool_set_flags(self.h, self.flags))
+
+    def apply(self, records):
+        records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
+        self._ck(lib().sgc_pool_apply(self.h, records.ctypes.data, len(records), 0))
+
+    def submit(self, records, max_stats=65536):
+        """process_stream over a mixed stream: batches, then one shared epoch for the due graphs."""
+        records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
+        st = (_lib.EpochStats * max(max_stats, 1))()
+        k = C.c_uint32()
+        self._ck(lib().sgc_pool_submit(self.h, records.ctypes.data, len(records), 0, st if max_stats else None,
+                                       max_stats, C.byref(k)))
+        return [st[i].as_dict() for i in range(min(k.value, max_stats))]
+
+    def due(self):
+        cap = max(len(self.graphs), 1)
+        out = np.zeros(cap, np.uint32)
+        k = C.c_uint32()
+        self._ck(lib().sgc_pool_due(self.h, out, cap, C.byref(k)))
+        return out[:min(k.value, cap)].tolist()
+
+    def _epoch(self, fn, gids):
+        gids = np.ascontiguousarray(gids, np.uint32)
+        st = (_lib.EpochStats * max(len(gids), 1))()
+        self._ck(fn(self.h, gids, len(gids), st))
+        return [st[i].as_dict() for i in range(len(gids))]
+
+    def epoch(self, gids):
+        return self._epoch(lib().sgc_pool_epoch, gids)
+
+    def finish(self, gids=None):
+        return self._epoch(lib().sgc_pool_finish, sorted(self.graphs) if gids is None else gids)
+
+    def query(self, gid, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        out = np.zeros(len(ids), np.uint16)
+        if len(ids):
+            self._ck(lib().sgc_pool_query(self.h, gid, ids, len(ids), out))
+        return out
+
+    def words(self, gid):
+        w = np.zeros(self.graphs[gid].n, np.uint32)
+        self._ck(lib().sgc_pool_download_words(self.h, gid, w))
+        return w
+
+    def colors(self, gid):
+        return (self.words(gid) & 0xFFFF).astype(np.uint16)
+
+    def snapshot(self, gid):
+        s = np.zeros(self.graphs[gid].n, np.uint16)
+        self._ck(lib().sgc_pool_download_snapshot(self.h, gid, s))
+        return s
+
+    def validate(self, gid, snapshot=False):
+        a, b, m = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        self._ck(lib().sgc_pool_validate(self.h, gid, 1 if snapshot else 0, C.byref(a), C.byref(b), C.byref(m)))
+        return a.value, b.value, m.value
+
+    def history_reserve(self, gid, capacity):
+        self._ck(lib().sgc_pool_history_reserve(self.h, gid, capacity))
+
+    def history_count(self, gid):
+        c = C.c_uint32()
+        self._ck(lib().sgc_pool_history_count(self.h, gid, C.byref(c)))
+        return c.value
+
+    def validate_history(self, gid, intervals, positions):
+        eu, ev, born, died = intervals
+        pos = np.ascontiguousarray(positions, np.uint64)
+        bad = np.zeros(max(len(pos), 1), np.uint64)
+        self._ck(lib().sgc_pool_validate_history(self.h, gid, eu, ev, born, died, len(eu), pos, len(pos), bad))
+        return bad[:len(pos)]
+
+    def totals(self):
+        t = _lib.Totals()
+        self._ck(lib().sgc_pool_totals(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def kernel_timing(self, enable=True):
+        self._ck(lib().sgc_pool_kernel_timing_enable(self.h, 1 if enable else 0))
+
+    def kernel_times(self):
+        cap, stride = 64, 32
+        names = C.create_string_buffer(cap * stride)
+        ms = (C.c_double * cap)()
+        launches = (C.c_uint64 * cap)()
+        cnt = C.c_uint32()
+        self._ck(lib().sgc_pool_kernel_times(self.h, names, stride, ms, launches, cap, C.byref(cnt)))
+        return {names.raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode(): (ms[i], launches[i])
+                for i in range(min(cnt.value, cap))}
+
+
+def tag_graph(records, gid):
+    """Copy of records addressed to graph gid of a pool."""
+    r = np.array(records, dtype=EDGE_UPDATE, copy=True)
+    r["graph"] = gid
+    return r
 
 
 def check_coloring_valid(partition, snapshot=False):

@@ -48,6 +48,11 @@ class Totals(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class PoolConfig(C.Structure):
+    _fields_ = [("max_graphs", C.c_uint32), ("batch_capacity", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64)]
+
+
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
@@ -92,6 +97,32 @@ SIGNATURES = {
     "sgc_kernel_timing_enable": [_vp, C.c_int],
     "sgc_kernel_times": [_vp, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_uint32,
                          C.POINTER(C.c_uint32)],
+    "sgc_pool_last_error": [_vp],
+    "sgc_pool_create": [C.POINTER(PoolConfig), C.POINTER(_vp)],
+    "sgc_pool_destroy": [_vp],
+    "sgc_pool_graph_bytes": [C.POINTER(Config), C.c_uint32, _u32p, _u32p, C.POINTER(C.c_uint64)],
+    "sgc_pool_add": [_vp, C.POINTER(Config), C.c_uint32, _u32p, _u32p, _u32p, C.POINTER(C.c_uint32)],
+    "sgc_pool_remove": [_vp, C.c_uint32],
+    "sgc_pool_usage": [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
+    "sgc_pool_set_flags": [_vp, C.c_uint32],
+    "sgc_pool_apply": [_vp, _vp, C.c_uint64, C.c_int],
+    "sgc_pool_due": [_vp, _u32p, C.c_uint32, C.POINTER(C.c_uint32)],
+    "sgc_pool_epoch": [_vp, _u32p, C.c_uint32, C.POINTER(EpochStats)],
+    "sgc_pool_finish": [_vp, _u32p, C.c_uint32, C.POINTER(EpochStats)],
+    "sgc_pool_submit": [_vp, _vp, C.c_uint64, C.c_int, C.POINTER(EpochStats), C.c_uint32, C.POINTER(C.c_uint32)],
+    "sgc_pool_query": [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p],
+    "sgc_pool_download_words": [_vp, C.c_uint32, _u32p],
+    "sgc_pool_download_snapshot": [_vp, C.c_uint32, _u16p],
+    "sgc_pool_validate": [_vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                          C.POINTER(C.c_uint32)],
+    "sgc_pool_history_reserve": [_vp, C.c_uint32, C.c_uint32],
+    "sgc_pool_history_count": [_vp, C.c_uint32, C.POINTER(C.c_uint32)],
+    "sgc_pool_validate_history": [_vp, C.c_uint32, _u32p, _u32p, _u64p, _u64p, C.c_uint64, _u64p, C.c_uint32,
+                                  _u64p],
+    "sgc_pool_totals": [_vp, C.POINTER(Totals)],
+    "sgc_pool_kernel_timing_enable": [_vp, C.c_int],
+    "sgc_pool_kernel_times": [_vp, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                              C.c_uint32, C.POINTER(C.c_uint32)],
     "sgc_csr_load_mtx": [C.c_char_p, C.POINTER(_vp)],
     "sgc_csr_from_edges": [C.c_uint32, _u32p, _u32p, C.c_uint64, C.POINTER(_vp)],
     "sgc_csr_info": [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
@@ -107,7 +138,8 @@ SIGNATURES = {
     "sgc_static_greedy": [C.c_uint32, _u32p, _u32p, C.POINTER(C.c_uint32)],
     "sgc_edge_intervals": [C.c_uint32, _u32p, _u32p, _vp, C.c_uint64, C.POINTER(C.c_uint64), _vp, _vp, _vp, _vp],
 }
-_RESTYPE = {"sgc_default_config": None, "sgc_last_error": C.c_char_p, "sgc_version": C.c_char_p}
+_RESTYPE = {"sgc_default_config": None, "sgc_last_error": C.c_char_p, "sgc_version": C.c_char_p,
+            "sgc_pool_last_error": C.c_char_p}
 
 _lib = None
 
@@ -131,8 +163,16 @@ class SgcError(RuntimeError):
     pass
 
 
-def check(rc, engine=None):
+SGC_ERR_NOMEM = -3
+
+
+class SgcNoMemory(SgcError):
+    """Admission refused: the pool is full or its HBM budget would be exceeded."""
+
+
+def check(rc, engine=None, pool=None):
     if rc != SGC_OK:
-        msg = lib().sgc_last_error(engine)
-        raise SgcError(f"streamgc error {rc}: {msg.decode() if msg else ''}")
+        msg = lib().sgc_pool_last_error(pool) if pool is not None else lib().sgc_last_error(engine)
+        text = f"streamgc error {rc}: {msg.decode() if msg else ''}"
+        raise (SgcNoMemory if rc == SGC_ERR_NOMEM else SgcError)(text)
     return rc

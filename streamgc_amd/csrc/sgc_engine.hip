@@ -1,10 +1,20 @@
-// sgc_engine.cpp -- the single-graph StreamGC engine behind the C ABI.
+// sgc_engine.hip -- the StreamGC engine: a pool of graphs resident in HBM.
 //
-// Layout in HBM (one graph): word[n] u32, deg0[n] u32, row_ptr[n+1] u32,
-// col[S] u32 (slack rows), bitmap[n] u64, hd_map[n] u32, hist[nhd*1024] u32,
-// overflow 2*cap u32, counters, snapshot 2 x n u16 (double buffer).
-// All work is enqueued on one HIP stream; the host synchronises only where a
-// data-dependent decision is needed (Tier-1 convergence, the Tier-2 check).
+// Every graph has its own slack CSR, packed vertex words, bitmaps, hub
+// histograms, overflow buffer and double-buffered snapshot (layout below).
+// Every phase of the hot path is ONE launch over all graphs taking part: the
+// kernels walk a virtual vertex space (vbase = prefix of vertex counts) with
+// per-graph active flags, so 256 small graphs cost the same number of
+// launches as one large graph.  The single-graph C ABI (sgc_create ...) is a
+// pool of one.
+//
+// Per-graph layout: word[n] u32, deg0[n] u32, row_ptr[n+1] u32, col[S] u32,
+// bitmap[n] u64, hd_map[n] u32, hubs[nhd] u32, hist[nhd*1024] u32,
+// overflow 2*cap u32, snapshot 2 x n u16; counters of all graphs in one
+// array (one copy reads them all).
+//
+// All work of a pool is enqueued on one HIP stream; the host synchronises only
+// for data-dependent decisions (Tier-1 convergence, the Tier-2 check).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,98 +38,175 @@ namespace {
 std::string g_last_error;
 std::mutex g_err_mu;
 
+int global_fail(int code, const std::string& m) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_error = m;
+    return code;
+}
+
 struct KTime {
     double ms = 0;
     uint64_t launches = 0;
 };
-
 struct PendingTiming {
     const char* name;
     hipEvent_t a, b;
 };
 
+// Bytes one graph occupies on the device (admission control).
+uint64_t graph_bytes(uint32_t n, uint64_t S, uint32_t nhd, uint32_t ovf_cap) {
+    return 28ull * n + 4 + 4 * S + 4ull * nhd + 4096ull * nhd + 8ull * (ovf_cap ? ovf_cap : 1);
+}
+
+// Slack capacities (A/src/streamgc.cu:373-378) and histogram rows
+// (A/src/streamgc.cu:226-250) of a graph.
+int plan_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* deg0,
+               std::vector<uint32_t>& rp, std::vector<uint32_t>& hd, std::vector<uint32_t>& hubs, uint64_t& S) {
+    rp.assign((size_t)n + 1, 0);
+    S = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const uint64_t d = row_ptr[v + 1] - row_ptr[v];
+        uint64_t cap = d + d * cfg->slack_pct / 100;
+        if (cap < cfg->min_row_capacity) cap = cfg->min_row_capacity;
+        S += cap;
+        if (S >= 0xFFFFFFFFull) return SGC_ERR_RANGE;
+        rp[v + 1] = (uint32_t)S;
+    }
+    hd.assign(n, INVALID);
+    hubs.clear();
+    for (uint32_t v = 0; v < n; v++)
+        if (deg0[v] > cfg->histogram_threshold) {
+            hd[v] = (uint32_t)hubs.size();
+            hubs.push_back(v);
+        }
+    return SGC_OK;
+}
+
 }  // namespace
 
-struct sgc_engine {
+struct GraphState {
+    bool used = false;
     sgc_config cfg{};
-    uint32_t n = 0;
+    uint32_t n = 0, nhd = 0;
     uint64_t S = 0;
-    uint32_t nhd = 0;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t qstream = nullptr;
-
+    uint64_t bytes = 0;
     uint32_t *word = nullptr, *deg0 = nullptr, *row_ptr = nullptr, *col = nullptr, *hd_map = nullptr;
-    uint32_t* hubs = nullptr;
-    uint32_t *hist = nullptr, *ovf = nullptr, *ctr = nullptr;
+    uint32_t *hubs = nullptr, *hist = nullptr, *ovf = nullptr;
     uint64_t* bitmap = nullptr;
-    uint16_t* snap = nullptr;            // 2 * n
-    uint16_t** snap_dst = nullptr;       // device: {snap, snap + n}
-    GraphDev hg{};
-    GraphDev* dg = nullptr;
-    uint64_t* totals = nullptr;          // device T_COUNT
-    sgc_edge_update* batch = nullptr;    // device staging, batch_size records
-    uint32_t* seg = nullptr;             // device: one segment [0, count)
-    EdgeGroups eg;                       // per-edge grouping of parallel batches
-    uint32_t* h_seg = nullptr;           // pinned
-    uint32_t* h_ctr = nullptr;           // pinned C_COUNT
-    uint64_t* h_totals = nullptr;        // pinned T_COUNT
-    uint16_t* d_tmp16 = nullptr;
-    uint32_t* d_qids = nullptr;          // query staging
-    uint16_t* d_qout = nullptr;
-    uint32_t qcap = 0;
-
-    int published = -1;                  // half holding the published snapshot
-    hipEvent_t pub_ev[2] = {nullptr, nullptr};
-    hipEvent_t read_ev[2] = {nullptr, nullptr};
-    std::mutex qmu;
-
+    uint16_t* snap = nullptr;  // 2 * n
+    int published = -1;
+    uint64_t since_epoch = 0;
+    uint32_t epochs = 0;
     uint16_t* history = nullptr;
     uint32_t hist_cap = 0, hist_count = 0;
+    hipEvent_t pub_ev[2] = {nullptr, nullptr};
+    hipEvent_t read_ev[2] = {nullptr, nullptr};
+    // Tier-1 loop state of the current repair
+    uint32_t prev = 0, plateau = 0;
+    bool iterating = false;
+    sgc_epoch_stats st{};
 
-    uint32_t epochs = 0;
+    void release() {
+        for (void* p : {(void*)word, (void*)deg0, (void*)row_ptr, (void*)col, (void*)hd_map, (void*)hubs, (void*)hist,
+                        (void*)ovf, (void*)bitmap, (void*)snap, (void*)history})
+            if (p) (void)hipFree(p);
+        for (int i = 0; i < 2; i++) {
+            if (pub_ev[i]) (void)hipEventDestroy(pub_ev[i]);
+            if (read_ev[i]) (void)hipEventDestroy(read_ev[i]);
+        }
+        *this = GraphState();
+    }
+};
+
+// A pinned host mirror of a small device array, uploaded in stream order; the
+// mirror is rewritten only after its last upload has completed.
+template <class T>
+struct Staged {
+    T* h = nullptr;
+    T* d = nullptr;
+    size_t count = 0;
+    hipEvent_t done = nullptr;
+    hipError_t alloc(size_t c) {
+        count = c;
+        hipError_t e = hipHostMalloc((void**)&h, c * sizeof(T));
+        if (e == hipSuccess) e = hipMalloc((void**)&d, c * sizeof(T));
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+        if (e == hipSuccess) memset(h, 0, c * sizeof(T));
+        return e;
+    }
+    hipError_t writable() { return hipEventSynchronize(done); }
+    hipError_t upload(hipStream_t s, size_t first = 0, size_t n = (size_t)-1) {
+        if (n == (size_t)-1) n = count - first;
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMemcpyAsync(d + first, h + first, n * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(done, s);
+        return e;
+    }
+    void release() {
+        if (done) (void)hipEventSynchronize(done);
+        if (h) (void)hipHostFree(h);
+        if (d) (void)hipFree(d);
+        if (done) (void)hipEventDestroy(done);
+        h = nullptr;
+        d = nullptr;
+        done = nullptr;
+    }
+};
+
+struct sgc_pool {
+    uint32_t max_graphs = 0, batch_cap = 0, flags = 0;
+    uint64_t budget = 0, used_bytes = 0;
+    hipStream_t stream = nullptr, qstream = nullptr;
+    std::vector<GraphState> gs;
+    Staged<GraphDev> graphs;              // descriptors
+    Staged<uint32_t> vbase, hbase, gbase, active;
+    Staged<uint32_t> perm, seg;           // deterministic per-graph replay
+    uint32_t* d_ctr = nullptr;            // max_graphs * C_COUNT
+    uint32_t* h_ctr = nullptr;            // pinned copy
+    uint64_t* d_totals = nullptr;
+    uint64_t* h_totals = nullptr;
+    EdgeGroups eg;
+    sgc_edge_update* d_batch = nullptr;   // staging of host batches
+    uint32_t ng = 0, V = 0, H = 0;        // slots in use, vertices, hubs
+    uint16_t* d_tmp16 = nullptr;
+    uint32_t* d_qids = nullptr;
+    uint16_t* d_qout = nullptr;
+    uint32_t qcap = 0;
+    std::mutex qmu;
     uint64_t t1_sweeps = 0, t2_runs = 0;
     std::string err;
-
     bool ktiming = false;
     std::vector<PendingTiming> pending;
     std::vector<hipEvent_t> free_events;
     std::map<std::string, KTime> ktimes;
     double last_ms[4] = {0, 0, 0, 0};
 
-    bool deterministic() const { return (cfg.flags & SGC_DETERMINISTIC) != 0; }
-    bool exact_hist() const { return (cfg.flags & SGC_EXACT_HISTOGRAM) != 0; }
-
-    // Bitmap rebuild, plus the hub histograms in exact-histogram mode.
-    hipError_t rebuild(const LaunchCtx& L) {
-        hipError_t e = timed("rebuild", [&] { return launch_rebuild(L); });
-        if (e == hipSuccess && exact_hist() && nhd) e = timed("rebuild_hist", [&] { return launch_rebuild_hist(L); });
-        return e;
-    }
+    bool deterministic() const { return (flags & SGC_DETERMINISTIC) != 0; }
+    bool exact_hist() const { return (flags & SGC_EXACT_HISTOGRAM) != 0; }
+    bool repair_every_epoch() const { return (flags & SGC_REPAIR_EVERY_EPOCH) != 0; }
 
     int fail(int code, const std::string& msg) {
         err = msg;
-        std::lock_guard<std::mutex> lk(g_err_mu);
-        g_last_error = msg;
-        return code;
+        return global_fail(code, msg);
     }
-    int hip_fail(hipError_t e, const char* what) {
-        return fail(SGC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-    }
+    int hip_fail(hipError_t e, const char* what) { return fail(SGC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)); }
 
-    LaunchCtx ctx() const {
+    LaunchCtx ctx(bool use_active) const {
         LaunchCtx L;
         L.stream = stream;
-        L.graphs = dg;
-        L.ng = 1;
-        L.vbase = nullptr;
-        L.V = n;
-        L.active = nullptr;
-        L.totals = totals;
-        L.H = nhd;
+        L.graphs = graphs.d;
+        L.ng = ng ? ng : 1;
+        L.vbase = ng > 1 ? vbase.d : nullptr;
+        L.V = V;
+        L.active = use_active ? active.d : nullptr;
+        L.gbase = gbase.d;
+        L.hbase = ng > 1 ? hbase.d : nullptr;
+        L.H = H;
+        L.totals = d_totals;
         return L;
     }
 
+    // ------------------------------------------------------------ timing
     hipEvent_t get_event() {
         if (!free_events.empty()) {
             hipEvent_t e = free_events.back();
@@ -154,217 +241,510 @@ struct sgc_engine {
         pending.clear();
     }
 
+    // ------------------------------------------------------------ host mirrors
     hipError_t read_ctr() {
-        hipError_t e = hipMemcpyAsync(h_ctr, ctr, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+        hipError_t e = hipMemcpyAsync(h_ctr, d_ctr, (size_t)ng * C_COUNT * 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        return e;
+    }
+    uint32_t ctr(uint32_t gid, uint32_t idx) const { return h_ctr[(size_t)gid * C_COUNT + idx]; }
+
+    // Active flags for a set of graphs (uploaded in stream order).
+    hipError_t set_active(const std::vector<uint32_t>& gids) {
+        hipError_t e = active.writable();
         if (e != hipSuccess) return e;
-        return hipStreamSynchronize(stream);
+        memset(active.h, 0, ng * 4);
+        for (uint32_t g : gids) active.h[g] = 1;
+        return active.upload(stream, 0, ng);
     }
-    hipError_t zero_ctr(uint32_t idx) {
-        return hipMemsetAsync(ctr + idx, 0, sizeof(uint32_t), stream);
-    }
-
-    int apply_device(const sgc_edge_update* d_upd, uint32_t count) {
-        if (count == 0) return SGC_OK;
-        LaunchCtx L = ctx();
-        hipError_t e;
-        if (deterministic()) {
-            h_seg[0] = 0;
-            h_seg[1] = count;
-            e = hipMemcpyAsync(seg, h_seg, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, stream);
-            if (e != hipSuccess) return hip_fail(e, "segment upload");
-            // the pinned segment is reused: wait until the copy has consumed it
-            e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) return hip_fail(e, "segment upload");
-            e = timed("apply", [&] { return launch_apply_seg(L, d_upd, seg, 1, 0); });
-        } else {
-            e = timed("apply", [&] { return launch_apply(L, d_upd, count, 0, &eg); });
-        }
-        if (e != hipSuccess) return hip_fail(e, "apply launch");
-        return SGC_OK;
+    hipError_t set_active_iterating(const std::vector<uint32_t>& gids) {
+        hipError_t e = active.writable();
+        if (e != hipSuccess) return e;
+        memset(active.h, 0, ng * 4);
+        for (uint32_t g : gids) active.h[g] = gs[g].iterating ? 1 : 0;
+        return active.upload(stream, 0, ng);
     }
 
-    int snapshot_publish() {
-        const int half = published == 0 ? 1 : 0;
-        LaunchCtx L = ctx();
-        // queries still reading this half must finish first
-        hipError_t e = hipStreamWaitEvent(stream, read_ev[half], 0);
-        if (e != hipSuccess) return hip_fail(e, "snapshot wait");
-        e = timed("snapshot", [&] { return launch_snapshot_to(L, half); });
-        if (e != hipSuccess) return hip_fail(e, "snapshot launch");
-        e = hipEventRecord(pub_ev[half], stream);
-        if (e != hipSuccess) return hip_fail(e, "snapshot event");
-        if (history && hist_count < hist_cap) {
-            e = hipMemcpyAsync(history + (size_t)hist_count * n, snap + (size_t)half * n, (size_t)n * 2,
-                               hipMemcpyDeviceToDevice, stream);
-            if (e != hipSuccess) return hip_fail(e, "history copy");
-            hist_count++;
-        }
-        std::lock_guard<std::mutex> lk(qmu);
-        published = half;
-        return SGC_OK;
-    }
-    hipError_t launch_snapshot_to(const LaunchCtx& L, int half);
-
-    int drain(sgc_epoch_stats* st) {
-        LaunchCtx L = ctx();
-        hipError_t e = timed("drain", [&] { return launch_drain(L, deterministic(), cfg.overflow_capacity); });
-        if (e != hipSuccess) return hip_fail(e, "drain launch");
-        (void)st;
-        return SGC_OK;
-    }
-
-    // run_conflict_repair (A/src/streamgc.cu:1317-1393)
-    int tier1(sgc_epoch_stats* st) {
-        LaunchCtx L = ctx();
-        uint32_t prev = 0xFFFFFFFFu, plateau = 0;
-        for (uint32_t it = 0; it < cfg.max_repair_iters; it++) {
-            hipError_t e = rebuild(L);
-            if (e == hipSuccess) e = zero_ctr(C_CONFLICTS);
-            if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L, deterministic()); });
-            if (e == hipSuccess) e = read_ctr();
-            if (e != hipSuccess) return hip_fail(e, "tier-1");
-            t1_sweeps++;
-            const uint32_t c = h_ctr[C_CONFLICTS];
-            st->t1_iterations = it + 1;
-            st->t1_conflicts_last = c;
-            if (c == 0) break;
-            if (c == prev) {
-                if (++plateau >= cfg.plateau_iters) break;
-            } else {
-                plateau = 0;
+    hipError_t refresh_layout() {
+        hipError_t e = vbase.writable();
+        if (e == hipSuccess) e = hbase.writable();
+        if (e == hipSuccess) e = gbase.writable();
+        if (e != hipSuccess) return e;
+        ng = 0;
+        for (uint32_t g = 0; g < max_graphs; g++)
+            if (gs[g].used) ng = g + 1;
+        uint64_t v = 0, h = 0;
+        for (uint32_t g = 0; g < ng; g++) {
+            vbase.h[g] = (uint32_t)v;
+            hbase.h[g] = (uint32_t)h;
+            gbase.h[g] = (uint32_t)v;
+            if (gs[g].used) {
+                v += gs[g].n;
+                h += gs[g].nhd;
             }
-            prev = c;
         }
-        return SGC_OK;
+        vbase.h[ng] = (uint32_t)v;
+        hbase.h[ng] = (uint32_t)h;
+        V = (uint32_t)v;
+        H = (uint32_t)h;
+        e = vbase.upload(stream, 0, ng + 1);
+        if (e == hipSuccess) e = hbase.upload(stream, 0, ng + 1);
+        if (e == hipSuccess) e = gbase.upload(stream, 0, ng + 1);
+        return e;
     }
 
-    int t2_check(uint32_t* bad) {
-        LaunchCtx L = ctx();
-        hipError_t e = zero_ctr(C_T2BAD);
-        if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(L); });
-        if (e == hipSuccess) e = read_ctr();
-        if (e != hipSuccess) return hip_fail(e, "tier-2 check");
-        *bad = h_ctr[C_T2BAD];
-        return SGC_OK;
-    }
+    hipError_t upload_descriptor(uint32_t gid) { return graphs.upload(stream, gid, 1); }
 
-    // cpu_serial_repair (A/src/streamgc.cu:1057-1240), run only when the
-    // check finds work (otherwise it would change nothing).
-    int tier2(sgc_epoch_stats* st) {
-        uint32_t bad = 0;
-        int r = t2_check(&bad);
-        if (r) return r;
-        if (bad == 0) {
-            st->t2_remaining = 0;
-            return SGC_OK;
-        }
-        auto t0 = std::chrono::steady_clock::now();
-        std::vector<uint32_t> w(n), rp(n + 1), c(S);
-        hipError_t e = hipMemcpyAsync(w.data(), word, (size_t)n * 4, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(rp.data(), row_ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c.data(), col, S * 4, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) return hip_fail(e, "tier-2 download");
-        st->t2_fixed = tier2_host(n, rp.data(), c.data(), w.data(), cfg.tier2_passes);
-        st->t2_ran = 1;
-        t2_runs++;
-        e = hipMemcpyAsync(word, w.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) return hip_fail(e, "tier-2 upload");
-        last_ms[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        r = t2_check(&bad);
-        if (r) return r;
-        st->t2_remaining = bad;
-        return SGC_OK;
-    }
+    // ------------------------------------------------------------ graphs
+    int add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+                  const uint32_t* deg0, uint32_t* gid_out);
+    int remove_graph(uint32_t gid);
 
-    int max_color(uint32_t* out);
-
-    int read_drain_stats(sgc_epoch_stats* st) {
-        hipError_t e = read_ctr();
-        if (e != hipSuccess) return hip_fail(e, "counters");
-        st->overflow_count = h_ctr[C_OVF_RAW];
-        st->overflow_drained = h_ctr[C_OVF_RAW] ? h_ctr[C_OVF_SNAP] : 0;
-        st->overflow_failed = h_ctr[C_DRAIN_FAILED];
-        return SGC_OK;
+    // ------------------------------------------------------------ hot path
+    int apply(const sgc_edge_update* upd, uint32_t count, bool on_device, std::vector<uint64_t>& per_graph);
+    int epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<sgc_epoch_stats>* out);
+    hipError_t rebuild(const LaunchCtx& L) {
+        hipError_t e = timed("rebuild", [&] { return launch_rebuild(L); });
+        if (e == hipSuccess && exact_hist() && H) e = timed("rebuild_hist", [&] { return launch_rebuild_hist(L); });
+        return e;
     }
-
-    // One epoch step: reference order (A/src/streamgc.cu:949-1036) or the
-    // repair-every-epoch order (README.md:170-176).
-    int epoch(sgc_epoch_stats* out) {
-        sgc_epoch_stats st{};
-        st.epoch_index = epochs++;
-        LaunchCtx L = ctx();
-        hipError_t e;
-        int r;
-        if (cfg.flags & SGC_REPAIR_EVERY_EPOCH) {
-            e = timed("compact", [&] { return launch_compact(L); });
-            if (e != hipSuccess) return hip_fail(e, "compact");
-            if ((r = drain(&st))) return r;
-            if ((r = read_drain_stats(&st))) return r;
-            if ((r = tier1(&st))) return r;
-            if ((r = tier2(&st))) return r;
-            if ((r = snapshot_publish())) return r;
-            st.snapshot_published = 1;
-        } else {
-            if ((r = snapshot_publish())) return r;
-            st.snapshot_published = 1;
-            e = timed("compact", [&] { return launch_compact(L); });
-            if (e != hipSuccess) return hip_fail(e, "compact");
-            if ((r = drain(&st))) return r;
-            e = rebuild(L);
-            if (e != hipSuccess) return hip_fail(e, "rebuild");
-            if (out && (r = read_drain_stats(&st))) return r;
-        }
-        if (out) {
-            if ((r = max_color(&st.max_color))) return r;
-            *out = st;
-        }
-        return SGC_OK;
-    }
-
-    // drain_and_compact + run_conflict_repair + cpu_serial_repair
-    int finish(sgc_epoch_stats* out) {
-        sgc_epoch_stats st{};
-        st.epoch_index = epochs;
-        LaunchCtx L = ctx();
-        int r;
-        hipError_t e = timed("compact", [&] { return launch_compact(L); });
-        if (e != hipSuccess) return hip_fail(e, "compact");
-        if ((r = drain(&st))) return r;
-        if ((r = read_drain_stats(&st))) return r;
-        e = rebuild(L);
-        if (e != hipSuccess) return hip_fail(e, "rebuild");
-        if ((r = tier1(&st))) return r;
-        if ((r = tier2(&st))) return r;
-        if ((r = snapshot_publish())) return r;
-        st.snapshot_published = 1;
-        if ((r = max_color(&st.max_color))) return r;
-        if (out) *out = st;
-        return SGC_OK;
-    }
+    int tier1(const std::vector<uint32_t>& gids);
+    int tier2(const std::vector<uint32_t>& gids);
+    int publish(const std::vector<uint32_t>& gids);
+    int read_drain(const std::vector<uint32_t>& gids);
+    int max_colors(const std::vector<uint32_t>& gids);
 
     void release();
 };
 
-// Device helpers that live in this translation unit.
-namespace {
-__global__ void k_snapshot_one(const GraphDev* __restrict__ g, uint16_t* __restrict__ dst) {
-    const uint32_t n = g->n;
-    const SGC_G uint32_t* w = gp(g->word);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        gp(dst)[i] = (uint16_t)(w[i] & 0xFFFFu);
-}
-__global__ void k_max_color(const GraphDev* __restrict__ g, uint32_t* out) {
-    const uint32_t n = g->n;
-    uint32_t mx = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t c = gp(g->word)[i] & 0xFFFFu;
-        mx = c > mx ? c : mx;
+// ============================================================== graph admission
+int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+                        const uint32_t* deg0, uint32_t* gid_out) {
+    if (!cfg || !row_ptr || !deg0 || n == 0) return fail(SGC_ERR_INVALID, "add graph: bad argument");
+    if (cfg->batch_size == 0 || cfg->epoch_size == 0) return fail(SGC_ERR_INVALID, "add graph: zero batch/epoch");
+    if (cfg->batch_size > batch_cap) return fail(SGC_ERR_INVALID, "add graph: batch larger than the pool's");
+    const uint64_t nnz = row_ptr[n];
+    if (nnz && !col_idx) return fail(SGC_ERR_INVALID, "add graph: col_idx missing");
+    for (uint32_t v = 0; v < n; v++)
+        if (row_ptr[v + 1] < row_ptr[v]) return fail(SGC_ERR_INVALID, "add graph: row_ptr not monotone");
+    for (uint64_t j = 0; j < nnz; j++)
+        if (col_idx[j] >= n) return fail(SGC_ERR_INVALID, "add graph: column id out of range");
+    uint32_t gid = max_graphs;
+    for (uint32_t g = 0; g < max_graphs; g++)
+        if (!gs[g].used) {
+            gid = g;
+            break;
+        }
+    if (gid == max_graphs) return fail(SGC_ERR_NOMEM, "add graph: pool is full");
+    if ((uint64_t)V + n >= 0xFFFFFFFFull) return fail(SGC_ERR_RANGE, "add graph: pool vertex space exhausted");
+    std::vector<uint32_t> rp, hd, hub_list;
+    uint64_t S = 0;
+    if (plan_graph(cfg, n, row_ptr, deg0, rp, hd, hub_list, S) != SGC_OK)
+        return fail(SGC_ERR_RANGE, "add graph: slot count exceeds 32-bit offsets");
+    const uint32_t nhd = (uint32_t)hub_list.size();
+    const uint64_t bytes = graph_bytes(n, S, nhd, cfg->overflow_capacity);
+    if (budget && used_bytes + bytes > budget)
+        return fail(SGC_ERR_NOMEM, "add graph: over the pool's HBM budget");
+
+    GraphState& G = gs[gid];
+    G = GraphState();
+    G.cfg = *cfg;
+    G.n = n;
+    G.S = S;
+    G.nhd = nhd;
+    G.bytes = bytes;
+    hipStream_t s = stream;
+    hipError_t e = hipSuccess;
+    auto A = [&](void** p, size_t b) {
+        if (e == hipSuccess) e = hipMalloc(p, b ? b : 4);
+    };
+    A((void**)&G.word, (size_t)n * 4);
+    A((void**)&G.deg0, (size_t)n * 4);
+    A((void**)&G.row_ptr, ((size_t)n + 1) * 4);
+    A((void**)&G.col, S * 4);
+    A((void**)&G.bitmap, (size_t)n * 8);
+    A((void**)&G.hd_map, (size_t)n * 4);
+    A((void**)&G.hubs, (size_t)nhd * 4);
+    A((void**)&G.hist, (size_t)nhd * HIST_BINS * 4);
+    A((void**)&G.ovf, (size_t)cfg->overflow_capacity * 8);
+    A((void**)&G.snap, (size_t)n * 4);
+    for (int i = 0; i < 2 && e == hipSuccess; i++) {
+        e = hipEventCreateWithFlags(&G.pub_ev[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&G.read_ev[i], hipEventDisableTiming);
     }
-    mx = wave_max32(mx);
-    if (lane() == 0 && mx) max32(out, mx);
+    if (e == hipSuccess) e = hipMemsetAsync(G.word, 0, (size_t)n * 4, s);
+    if (e == hipSuccess) e = hipMemsetAsync(G.bitmap, 0, (size_t)n * 8, s);
+    if (e == hipSuccess && nhd) e = hipMemsetAsync(G.hist, 0, (size_t)nhd * HIST_BINS * 4, s);
+    if (e == hipSuccess) e = hipMemsetAsync(G.snap, 0, (size_t)n * 4, s);
+    if (e == hipSuccess) e = hipMemsetAsync(d_ctr + (size_t)gid * C_COUNT, 0, C_COUNT * 4, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(G.deg0, deg0, (size_t)n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(G.row_ptr, rp.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(G.hd_map, hd.data(), (size_t)n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && nhd) e = hipMemcpyAsync(G.hubs, hub_list.data(), (size_t)nhd * 4, hipMemcpyHostToDevice, s);
+    // prefill the slack rows on the device from the loaded CSR (A/src/streamgc.cu:400-432)
+    uint32_t *rp0 = nullptr, *col0 = nullptr;
+    A((void**)&rp0, ((size_t)n + 1) * 4);
+    A((void**)&col0, nnz * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(rp0, row_ptr, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && nnz) e = hipMemcpyAsync(col0, col_idx, nnz * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_fill_rows(s, n, rp0, col0, G.row_ptr, G.col);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (rp0) (void)hipFree(rp0);
+    if (col0) (void)hipFree(col0);
+    if (e == hipSuccess) e = graphs.writable();
+    if (e != hipSuccess) {
+        G.release();
+        return hip_fail(e, "add graph");
+    }
+    GraphDev& d = graphs.h[gid];
+    d = GraphDev();
+    d.word = G.word;
+    d.deg0 = G.deg0;
+    d.row_ptr = G.row_ptr;
+    d.col = G.col;
+    d.bitmap = G.bitmap;
+    d.hd_map = G.hd_map;
+    d.hubs = G.hubs;
+    d.hist = G.hist;
+    d.ovf = G.ovf;
+    d.ctr = d_ctr + (size_t)gid * C_COUNT;
+    d.snap_write = G.snap;
+    d.n = n;
+    d.ovf_cap = cfg->overflow_capacity;
+    d.nhd = nhd;
+    G.used = true;
+    used_bytes += bytes;
+    e = upload_descriptor(gid);
+    if (e == hipSuccess) e = refresh_layout();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "add graph");
+    *gid_out = gid;
+    return SGC_OK;
 }
+
+int sgc_pool::remove_graph(uint32_t gid) {
+    if (gid >= max_graphs || !gs[gid].used) return fail(SGC_ERR_INVALID, "remove graph: no such graph");
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(qstream);
+    if (e != hipSuccess) return hip_fail(e, "remove graph");
+    used_bytes -= gs[gid].bytes;
+    gs[gid].release();
+    e = graphs.writable();
+    if (e == hipSuccess) {
+        graphs.h[gid] = GraphDev();
+        e = upload_descriptor(gid);
+    }
+    if (e == hipSuccess) e = refresh_layout();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "remove graph");
+    return SGC_OK;
+}
+
+// ============================================================== updates
+// One launch over a batch (local_update_kernel, A/src/streamgc.cu:847-854).
+// Parallel: one wavefront per update, same-edge updates grouped in stream
+// order.  Deterministic: one wavefront per graph, that graph's updates in
+// stream order.  per_graph receives the updates per graph.
+int sgc_pool::apply(const sgc_edge_update* upd, uint32_t count, bool on_device, std::vector<uint64_t>& per_graph) {
+    per_graph.assign(ng, 0);
+    if (count == 0) return SGC_OK;
+    if (count > batch_cap) return fail(SGC_ERR_INVALID, "apply: batch larger than the pool's batch capacity");
+    const bool multi = ng > 1;
+    const sgc_edge_update* src = upd;
+    hipError_t e = hipSuccess;
+    if (!on_device) {
+        e = hipMemcpyAsync(d_batch, upd, (size_t)count * sizeof(sgc_edge_update), hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return hip_fail(e, "batch upload");
+        src = d_batch;
+    }
+    // per-graph counts (and the deterministic order) need the graph ids
+    std::vector<uint32_t> host_gid;
+    if (multi) {
+        if (on_device) {
+            std::vector<sgc_edge_update> tmp(count);
+            e = hipMemcpyAsync(tmp.data(), upd, (size_t)count * sizeof(sgc_edge_update), hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return hip_fail(e, "batch read");
+            host_gid.resize(count);
+            for (uint32_t i = 0; i < count; i++) memcpy(&host_gid[i], tmp[i].pad + 3, 4);
+        } else {
+            host_gid.resize(count);
+            for (uint32_t i = 0; i < count; i++) memcpy(&host_gid[i], upd[i].pad + 3, 4);
+        }
+        for (uint32_t i = 0; i < count; i++) {
+            if (host_gid[i] >= ng || !gs[host_gid[i]].used) return fail(SGC_ERR_INVALID, "apply: unknown graph id");
+            per_graph[host_gid[i]]++;
+        }
+    } else {
+        if (!on_device)
+            for (uint32_t i = 0; i < count; i++) {
+                uint32_t gid;
+                memcpy(&gid, upd[i].pad + 3, 4);
+                if (gid != 0) return fail(SGC_ERR_INVALID, "apply: graph id set on a single-graph batch");
+            }
+        per_graph[0] = count;
+    }
+    LaunchCtx L = ctx(false);
+    if (deterministic()) {
+        e = perm.writable();
+        if (e == hipSuccess) e = seg.writable();
+        if (e != hipSuccess) return hip_fail(e, "segments");
+        uint32_t nseg = 0;
+        if (!multi) {
+            seg.h[0] = 0;
+            seg.h[1] = count;
+            nseg = 1;
+        } else {
+            std::vector<uint32_t> start(ng + 1, 0);
+            for (uint32_t g = 0; g < ng; g++) start[g + 1] = start[g] + (uint32_t)per_graph[g];
+            std::vector<uint32_t> at(start.begin(), start.end() - 1);
+            for (uint32_t i = 0; i < count; i++) perm.h[at[host_gid[i]]++] = i;
+            for (uint32_t g = 0; g < ng; g++)
+                if (start[g + 1] > start[g]) {
+                    seg.h[2 * nseg] = start[g];
+                    seg.h[2 * nseg + 1] = start[g + 1];
+                    nseg++;
+                }
+            e = perm.upload(stream, 0, count);
+        }
+        if (e == hipSuccess) e = seg.upload(stream, 0, 2 * nseg);
+        if (e == hipSuccess)
+            e = timed("apply", [&] { return launch_apply_seg(L, src, seg.d, nseg, multi ? perm.d : nullptr, multi); });
+    } else {
+        e = timed("apply", [&] { return launch_apply(L, src, count, multi, &eg); });
+    }
+    if (e != hipSuccess) return hip_fail(e, "apply launch");
+    if (!on_device) {
+        // the staging buffer is reused by the next batch
+        e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "apply");
+    }
+    for (uint32_t g = 0; g < ng; g++) gs[g].since_epoch += per_graph[g];
+    return SGC_OK;
+}
+
+// ============================================================== epoch pieces
+int sgc_pool::read_drain(const std::vector<uint32_t>& gids) {
+    hipError_t e = read_ctr();
+    if (e != hipSuccess) return hip_fail(e, "counters");
+    for (uint32_t g : gids) {
+        gs[g].st.overflow_count = ctr(g, C_OVF_RAW);
+        gs[g].st.overflow_drained = ctr(g, C_OVF_RAW) ? ctr(g, C_OVF_SNAP) : 0;
+        gs[g].st.overflow_failed = ctr(g, C_DRAIN_FAILED);
+    }
+    return SGC_OK;
+}
+
+// run_conflict_repair (A/src/streamgc.cu:1317-1393) for every graph of the set
+// at once: each iteration is one rebuild and one sweep over the graphs still
+// iterating; each graph stops on its own (converged, plateau, iteration cap).
+int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
+    uint32_t max_iters = 0;
+    for (uint32_t g : gids) {
+        gs[g].prev = 0xFFFFFFFFu;
+        gs[g].plateau = 0;
+        gs[g].iterating = gs[g].cfg.max_repair_iters > 0;
+        gs[g].st.t1_iterations = 0;
+        max_iters = std::max(max_iters, gs[g].cfg.max_repair_iters);
+    }
+    for (uint32_t it = 0; it < max_iters; it++) {
+        bool any = false;
+        for (uint32_t g : gids) any |= gs[g].iterating;
+        if (!any) break;
+        hipError_t e = set_active_iterating(gids);
+        LaunchCtx L = ctx(true);
+        if (e == hipSuccess) e = rebuild(L);
+        if (e == hipSuccess) e = launch_zero_ctr(L, C_CONFLICTS);
+        if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L, deterministic()); });
+        if (e == hipSuccess) e = read_ctr();
+        if (e != hipSuccess) return hip_fail(e, "tier-1");
+        t1_sweeps++;
+        for (uint32_t g : gids) {
+            GraphState& G = gs[g];
+            if (!G.iterating) continue;
+            const uint32_t c = ctr(g, C_CONFLICTS);
+            G.st.t1_iterations = it + 1;
+            G.st.t1_conflicts_last = c;
+            if (c == 0 || it + 1 >= G.cfg.max_repair_iters) {
+                G.iterating = false;
+            } else if (c == G.prev) {
+                if (++G.plateau >= G.cfg.plateau_iters) G.iterating = false;
+            } else {
+                G.plateau = 0;
+            }
+            G.prev = c;
+        }
+    }
+    return SGC_OK;
+}
+
+// cpu_serial_repair (A/src/streamgc.cu:1057-1240), run on the host only for
+// the graphs whose row check finds an uncoloured endpoint or a conflict
+// (elsewhere it would change nothing).
+int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
+    hipError_t e = set_active(gids);
+    LaunchCtx L = ctx(true);
+    if (e == hipSuccess) e = launch_zero_ctr(L, C_T2BAD);
+    if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(L); });
+    if (e == hipSuccess) e = read_ctr();
+    if (e != hipSuccess) return hip_fail(e, "tier-2 check");
+    std::vector<uint32_t> ran;
+    for (uint32_t g : gids) {
+        GraphState& G = gs[g];
+        G.st.t2_remaining = ctr(g, C_T2BAD);
+        if (G.st.t2_remaining == 0) continue;
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<uint32_t> w(G.n), rp(G.n + 1), c(G.S);
+        e = hipMemcpyAsync(w.data(), G.word, (size_t)G.n * 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(rp.data(), G.row_ptr, ((size_t)G.n + 1) * 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c.data(), G.col, G.S * 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "tier-2 download");
+        G.st.t2_fixed = tier2_host(G.n, rp.data(), c.data(), w.data(), G.cfg.tier2_passes);
+        G.st.t2_ran = 1;
+        t2_runs++;
+        e = hipMemcpyAsync(G.word, w.data(), (size_t)G.n * 4, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "tier-2 upload");
+        last_ms[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        ran.push_back(g);
+    }
+    if (ran.empty()) return SGC_OK;
+    e = set_active(ran);
+    if (e == hipSuccess) e = launch_zero_ctr(L, C_T2BAD);
+    if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(L); });
+    if (e == hipSuccess) e = read_ctr();
+    if (e != hipSuccess) return hip_fail(e, "tier-2 check");
+    for (uint32_t g : ran) gs[g].st.t2_remaining = ctr(g, C_T2BAD);
+    return SGC_OK;
+}
+
+// epoch_snapshot_kernel (A/src/streamgc.cu:957-958) into each graph's back
+// buffer, then the buffers flip: queries move to the new snapshot.
+int sgc_pool::publish(const std::vector<uint32_t>& gids) {
+    hipError_t e = graphs.writable();
+    if (e != hipSuccess) return hip_fail(e, "descriptors");
+    std::vector<int> half(gids.size());
+    for (size_t k = 0; k < gids.size(); k++) {
+        GraphState& G = gs[gids[k]];
+        half[k] = G.published == 0 ? 1 : 0;
+        graphs.h[gids[k]].snap_write = G.snap + (size_t)half[k] * G.n;
+        // queries still reading this half must finish first
+        e = hipStreamWaitEvent(stream, G.read_ev[half[k]], 0);
+        if (e != hipSuccess) return hip_fail(e, "snapshot wait");
+    }
+    e = graphs.upload(stream, 0, ng);
+    if (e == hipSuccess) e = set_active(gids);
+    LaunchCtx L = ctx(true);
+    if (e == hipSuccess) e = timed("snapshot", [&] { return launch_snapshot(L); });
+    if (e != hipSuccess) return hip_fail(e, "snapshot");
+    std::lock_guard<std::mutex> lk(qmu);
+    for (size_t k = 0; k < gids.size(); k++) {
+        GraphState& G = gs[gids[k]];
+        e = hipEventRecord(G.pub_ev[half[k]], stream);
+        if (e == hipSuccess && G.history && G.hist_count < G.hist_cap) {
+            e = hipMemcpyAsync(G.history + (size_t)G.hist_count * G.n, G.snap + (size_t)half[k] * G.n,
+                               (size_t)G.n * 2, hipMemcpyDeviceToDevice, stream);
+            G.hist_count++;
+        }
+        if (e != hipSuccess) return hip_fail(e, "publish");
+        G.published = half[k];
+        G.st.snapshot_published = 1;
+    }
+    return SGC_OK;
+}
+
+int sgc_pool::max_colors(const std::vector<uint32_t>& gids) {
+    hipError_t e = set_active(gids);
+    LaunchCtx L = ctx(true);
+    if (e == hipSuccess) e = launch_zero_ctr(L, C_MAXC);
+    if (e == hipSuccess) e = launch_max_color(L);
+    if (e == hipSuccess) e = read_ctr();
+    if (e != hipSuccess) return hip_fail(e, "max colour");
+    for (uint32_t g : gids) gs[g].st.max_color = ctr(g, C_MAXC);
+    return SGC_OK;
+}
+
+// One epoch step for a set of graphs, all in the same launches.  Reference
+// order (A/src/streamgc.cu:949-1036): publish, compact, drain, rebuild.
+// Repair-every-epoch order (README.md:170-176): compact, drain, Tier-1 (whose
+// first step is the rebuild), Tier-2, publish.  finish = drain_and_compact +
+// run_conflict_repair + cpu_serial_repair (A/src/streamgc.cu:1243-1423) and a
+// publish.  Statistics (out != null) cost a few extra synchronisations.
+int sgc_pool::epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<sgc_epoch_stats>* out) {
+    if (gids.empty()) return SGC_OK;
+    for (uint32_t g : gids) {
+        gs[g].st = sgc_epoch_stats();
+        gs[g].st.epoch_index = gs[g].epochs;
+        if (!finish) gs[g].epochs++;
+        gs[g].since_epoch = 0;
+    }
+    int r;
+    hipError_t e = set_active(gids);
+    LaunchCtx L = ctx(true);
+    const bool repair = finish || repair_every_epoch();
+    if (!repair) {
+        if ((r = publish(gids))) return r;
+        e = set_active(gids);
+    }
+    if (e == hipSuccess) e = timed("compact", [&] { return launch_compact(L); });
+    uint32_t ovf_cap = 0;
+    for (uint32_t g : gids) ovf_cap = std::max(ovf_cap, gs[g].cfg.overflow_capacity);
+    if (e == hipSuccess) e = timed("drain", [&] { return launch_drain(L, deterministic(), ovf_cap); });
+    if (e != hipSuccess) return hip_fail(e, "compact/drain");
+    if (repair || out)
+        if ((r = read_drain(gids))) return r;
+    if (repair) {
+        if (finish) {
+            e = rebuild(L);
+            if (e != hipSuccess) return hip_fail(e, "rebuild");
+        }
+        if ((r = tier1(gids))) return r;
+        if ((r = tier2(gids))) return r;
+        if ((r = publish(gids))) return r;
+    } else {
+        e = rebuild(L);
+        if (e != hipSuccess) return hip_fail(e, "rebuild");
+    }
+    if (out) {
+        if ((r = max_colors(gids))) return r;
+        out->clear();
+        for (uint32_t g : gids) out->push_back(gs[g].st);
+    }
+    return SGC_OK;
+}
+
+void sgc_pool::release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (qstream) (void)hipStreamSynchronize(qstream);
+    for (auto& G : gs) G.release();
+    graphs.release();
+    vbase.release();
+    hbase.release();
+    gbase.release();
+    active.release();
+    perm.release();
+    seg.release();
+    for (void* p : {(void*)d_ctr, (void*)d_totals, (void*)d_batch, (void*)d_tmp16, (void*)d_qids, (void*)d_qout,
+                    (void*)eg.keys, (void*)eg.head, (void*)eg.next})
+        if (p) (void)hipFree(p);
+    for (void* p : {(void*)h_ctr, (void*)h_totals})
+        if (p) (void)hipHostFree(p);
+    for (auto& p : pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (hipEvent_t ev : free_events) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (qstream) (void)hipStreamDestroy(qstream);
+    stream = qstream = nullptr;
+}
+
+// The single-graph engine is a pool of one.
+struct sgc_engine : sgc_pool {};
+
+namespace {
+
 __global__ void k_gather16(const uint16_t* __restrict__ src, uint32_t n, const uint32_t* __restrict__ ids,
                            uint32_t count, uint16_t* __restrict__ out) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
@@ -372,56 +752,197 @@ __global__ void k_gather16(const uint16_t* __restrict__ src, uint32_t n, const u
         out[i] = v < n ? src[v] : 0;
     }
 }
-}  // namespace
 
-hipError_t sgc_engine::launch_snapshot_to(const LaunchCtx& L, int half) {
-    uint32_t grid = (n + 1023) / 1024;
-    grid = grid < 1 ? 1 : (grid > 8192 ? 8192 : grid);
-    k_snapshot_one<<<grid, 256, 0, L.stream>>>(dg, snap + (size_t)half * n);
-    return hipGetLastError();
-}
-
-int sgc_engine::max_color(uint32_t* out) {
-    hipError_t e = zero_ctr(C_MAXC);
-    uint32_t grid = (n + 1023) / 1024;
-    grid = grid < 1 ? 1 : (grid > 4096 ? 4096 : grid);
-    if (e == hipSuccess) {
-        k_max_color<<<grid, 256, 0, stream>>>(dg, ctr + C_MAXC);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = read_ctr();
-    if (e != hipSuccess) return hip_fail(e, "max colour");
-    *out = h_ctr[C_MAXC];
+int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t flags, uint64_t budget) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return P->fail(SGC_ERR_HIP, "no HIP device");
+    P->max_graphs = max_graphs;
+    P->batch_cap = batch_cap;
+    P->flags = flags;
+    P->budget = budget;
+    P->gs.resize(max_graphs);
+    hipError_t e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&P->qstream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = P->graphs.alloc(max_graphs);
+    if (e == hipSuccess) e = P->vbase.alloc(max_graphs + 1);
+    if (e == hipSuccess) e = P->hbase.alloc(max_graphs + 1);
+    if (e == hipSuccess) e = P->gbase.alloc(max_graphs + 1);
+    if (e == hipSuccess) e = P->active.alloc(max_graphs);
+    if (e == hipSuccess) e = P->perm.alloc(batch_cap);
+    if (e == hipSuccess) e = P->seg.alloc(2 * (size_t)max_graphs);
+    if (e == hipSuccess) e = hipMalloc(&P->d_ctr, (size_t)max_graphs * C_COUNT * 4);
+    if (e == hipSuccess) e = hipMemsetAsync(P->d_ctr, 0, (size_t)max_graphs * C_COUNT * 4, P->stream);
+    if (e == hipSuccess) e = hipHostMalloc(&P->h_ctr, (size_t)max_graphs * C_COUNT * 4);
+    if (e == hipSuccess) e = hipMalloc(&P->d_totals, T_COUNT * 8);
+    if (e == hipSuccess) e = hipMemsetAsync(P->d_totals, 0, T_COUNT * 8, P->stream);
+    if (e == hipSuccess) e = hipHostMalloc(&P->h_totals, T_COUNT * 8);
+    if (e == hipSuccess) e = hipMalloc(&P->d_batch, (size_t)batch_cap * sizeof(sgc_edge_update));
+    if (e == hipSuccess) e = hipMalloc(&P->d_tmp16, 64);
+    P->eg.size = 16;
+    while (P->eg.size < 2u * batch_cap) P->eg.size <<= 1;
+    P->eg.next_cap = batch_cap;
+    if (e == hipSuccess) e = hipMalloc(&P->eg.keys, (size_t)P->eg.size * 8);
+    if (e == hipSuccess) e = hipMalloc(&P->eg.head, (size_t)P->eg.size * 4);
+    if (e == hipSuccess) e = hipMalloc(&P->eg.next, (size_t)batch_cap * 4);
+    if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+    if (e != hipSuccess) return P->hip_fail(e, "pool create");
     return SGC_OK;
 }
 
-void sgc_engine::release() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    if (qstream) (void)hipStreamSynchronize(qstream);
-    for (void* p : {(void*)word, (void*)hubs, (void*)deg0, (void*)row_ptr, (void*)col, (void*)hd_map, (void*)hist, (void*)ovf,
-                    (void*)ctr, (void*)bitmap, (void*)snap, (void*)snap_dst, (void*)dg, (void*)totals, (void*)batch,
-                    (void*)seg, (void*)d_tmp16, (void*)d_qids, (void*)d_qout, (void*)history, (void*)eg.keys,
-                    (void*)eg.head, (void*)eg.next})
-        if (p) (void)hipFree(p);
-    for (void* p : {(void*)h_seg, (void*)h_ctr, (void*)h_totals})
-        if (p) (void)hipHostFree(p);
-    for (auto& p : pending) {
-        (void)hipEventDestroy(p.a);
-        (void)hipEventDestroy(p.b);
+int submit_impl(sgc_pool* P, const sgc_edge_update* updates, uint64_t count, int on_device,
+                sgc_epoch_stats* stats_out, uint32_t max_stats, uint32_t* nstats) {
+    auto t0 = std::chrono::steady_clock::now();
+    P->last_ms[2] = 0;
+    uint32_t produced = 0;
+    uint64_t done = 0;
+    // batch size of a pool = the smallest batch_size among its graphs (one batch for a graph of one)
+    uint32_t bsz = P->batch_cap;
+    for (uint32_t g = 0; g < P->ng; g++)
+        if (P->gs[g].used) bsz = std::min(bsz, P->gs[g].cfg.batch_size);
+    std::vector<uint64_t> per_graph;
+    std::vector<sgc_epoch_stats> st;
+    while (done < count) {
+        const uint32_t b = (uint32_t)std::min<uint64_t>(bsz, count - done);
+        int r = P->apply(updates + done, b, on_device != 0, per_graph);
+        if (r) return r;
+        done += b;
+        std::vector<uint32_t> due;
+        for (uint32_t g = 0; g < P->ng; g++)
+            if (P->gs[g].used && P->gs[g].since_epoch >= P->gs[g].cfg.epoch_size) due.push_back(g);
+        if (!due.empty()) {
+            const bool want = stats_out && produced < max_stats;
+            r = P->epoch(due, false, want ? &st : nullptr);
+            if (r) return r;
+            if (want)
+                for (auto& s : st)
+                    if (produced < max_stats) stats_out[produced++] = s;
+        }
     }
-    for (hipEvent_t e : free_events) (void)hipEventDestroy(e);
-    for (int i = 0; i < 2; i++) {
-        if (pub_ev[i]) (void)hipEventDestroy(pub_ev[i]);
-        if (read_ev[i]) (void)hipEventDestroy(read_ev[i]);
-    }
-    if (stream) (void)hipStreamDestroy(stream);
-    if (qstream) (void)hipStreamDestroy(qstream);
+    hipError_t e = hipStreamSynchronize(P->stream);
+    if (e != hipSuccess) return P->hip_fail(e, "submit");
+    P->resolve_timing();
+    P->last_ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (nstats) *nstats = produced;
+    return SGC_OK;
 }
+
+// process_stream for a pool: the mixed stream is split per graph (stream order
+// kept inside each graph) and applied in rounds.  A round takes, from every
+// graph with pending updates, at most what is left until that graph's next
+// epoch (E_g updates per epoch), within the pool's batch capacity; the graphs
+// that reach their boundary then share one epoch.  Each graph therefore sees
+// exactly the batches and epochs it would see alone with batch = epoch = E_g.
+int pool_submit_impl(sgc_pool* P, const sgc_edge_update* updates, uint64_t count, sgc_epoch_stats* stats_out,
+                     uint32_t max_stats, uint32_t* nstats) {
+    auto t0 = std::chrono::steady_clock::now();
+    P->last_ms[2] = 0;
+    std::vector<std::vector<uint32_t>> queue(P->ng);
+    for (uint64_t i = 0; i < count; i++) {
+        uint32_t gid;
+        memcpy(&gid, updates[i].pad + 3, 4);
+        if (gid >= P->ng || !P->gs[gid].used) return P->fail(SGC_ERR_INVALID, "submit: unknown graph id");
+        if (i > 0xFFFFFFFFull) return P->fail(SGC_ERR_RANGE, "submit: stream longer than 2^32 records");
+        queue[gid].push_back((uint32_t)i);
+    }
+    std::vector<size_t> head(P->ng, 0);
+    std::vector<sgc_edge_update> batch;
+    batch.reserve(std::min<uint64_t>(count, P->batch_cap));
+    std::vector<uint64_t> per_graph;
+    std::vector<sgc_epoch_stats> st;
+    uint32_t produced = 0;
+    for (;;) {
+        batch.clear();
+        for (uint32_t g = 0; g < P->ng && batch.size() < P->batch_cap; g++) {
+            const GraphState& G = P->gs[g];
+            if (!G.used || head[g] == queue[g].size()) continue;
+            const uint64_t room = G.cfg.epoch_size - std::min<uint64_t>(G.since_epoch, G.cfg.epoch_size - 1);
+            uint64_t take = std::min<uint64_t>(queue[g].size() - head[g], room);
+            take = std::min<uint64_t>(take, P->batch_cap - batch.size());
+            for (uint64_t k = 0; k < take; k++) batch.push_back(updates[queue[g][head[g] + k]]);
+            head[g] += take;
+        }
+        if (batch.empty()) break;
+        int r = P->apply(batch.data(), (uint32_t)batch.size(), false, per_graph);
+        if (r) return r;
+        std::vector<uint32_t> due;
+        for (uint32_t g = 0; g < P->ng; g++)
+            if (P->gs[g].used && P->gs[g].since_epoch >= P->gs[g].cfg.epoch_size) due.push_back(g);
+        if (!due.empty()) {
+            const bool want = stats_out && produced < max_stats;
+            r = P->epoch(due, false, want ? &st : nullptr);
+            if (r) return r;
+            if (want)
+                for (auto& s : st)
+                    if (produced < max_stats) stats_out[produced++] = s;
+        }
+    }
+    hipError_t e = hipStreamSynchronize(P->stream);
+    if (e != hipSuccess) return P->hip_fail(e, "submit");
+    P->resolve_timing();
+    P->last_ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (nstats) *nstats = produced;
+    return SGC_OK;
+}
+
+int download(sgc_pool* P, void* dst, const void* src, size_t bytes) {
+    hipError_t r = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, P->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(P->stream);
+    if (r != hipSuccess) return P->hip_fail(r, "download");
+    return SGC_OK;
+}
+int upload(sgc_pool* P, void* dst, const void* src, size_t bytes) {
+    hipError_t r = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, P->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(P->stream);
+    if (r != hipSuccess) return P->hip_fail(r, "upload");
+    return SGC_OK;
+}
+
+GraphState* graph_of(sgc_pool* P, uint32_t gid) {
+    if (!P || gid >= P->max_graphs || !P->gs[gid].used) return nullptr;
+    return &P->gs[gid];
+}
+
+int query_impl(sgc_pool* P, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out) {
+    GraphState* G = graph_of(P, gid);
+    if (!G || (!ids && count) || (!out && count)) return SGC_ERR_INVALID;
+    if (count == 0) return SGC_OK;
+    std::lock_guard<std::mutex> lk(P->qmu);
+    if (G->published < 0) {
+        memset(out, 0, (size_t)count * 2);
+        return SGC_OK;
+    }
+    if (count > P->qcap) {
+        if (P->d_qids) (void)hipFree(P->d_qids);
+        if (P->d_qout) (void)hipFree(P->d_qout);
+        P->d_qids = nullptr;
+        P->d_qout = nullptr;
+        P->qcap = 0;
+        hipError_t r = hipMalloc(&P->d_qids, (size_t)count * 4);
+        if (r == hipSuccess) r = hipMalloc(&P->d_qout, (size_t)count * 2);
+        if (r != hipSuccess) return P->hip_fail(r, "query buffers");
+        P->qcap = count;
+    }
+    const int half = G->published;
+    hipError_t r = hipStreamWaitEvent(P->qstream, G->pub_ev[half], 0);
+    if (r == hipSuccess) r = hipMemcpyAsync(P->d_qids, ids, (size_t)count * 4, hipMemcpyHostToDevice, P->qstream);
+    if (r == hipSuccess) {
+        const uint32_t grid = std::min<uint32_t>((count + 255) / 256, 1024);
+        k_gather16<<<grid, 256, 0, P->qstream>>>(G->snap + (size_t)half * G->n, G->n, P->d_qids, count, P->d_qout);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipEventRecord(G->read_ev[half], P->qstream);
+    if (r == hipSuccess) r = hipMemcpyAsync(out, P->d_qout, (size_t)count * 2, hipMemcpyDeviceToHost, P->qstream);
+    if (r == hipSuccess) r = hipStreamSynchronize(P->qstream);
+    if (r != hipSuccess) return P->hip_fail(r, "query");
+    return SGC_OK;
+}
+
+}  // namespace
 
 // ======================================================================== C ABI
 extern "C" {
 
-const char* sgc_version(void) { return "streamgc_amd 0.1 (gfx950)"; }
+const char* sgc_version(void) { return "streamgc_amd 0.2 (gfx950)"; }
 
 void sgc_default_config(sgc_config* c) {
     if (!c) return;
@@ -442,6 +963,11 @@ const char* sgc_last_error(const sgc_engine* e) {
     std::lock_guard<std::mutex> lk(g_err_mu);
     return g_last_error.c_str();
 }
+const char* sgc_pool_last_error(const sgc_pool* p) {
+    if (p) return p->err.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_last_error.c_str();
+}
 
 int sgc_device_count(void) {
     int c = 0;
@@ -449,141 +975,21 @@ int sgc_device_count(void) {
     return c;
 }
 
-static int global_fail(int code, const std::string& m) {
-    std::lock_guard<std::mutex> lk(g_err_mu);
-    g_last_error = m;
-    return code;
-}
-
-#define TRY(call, what)                                   \
-    do {                                                  \
-        hipError_t _e = (call);                           \
-        if (_e != hipSuccess) {                           \
-            int _r = E->hip_fail(_e, what);               \
-            std::string _m = E->err;                      \
-            E->release();                                 \
-            delete E;                                     \
-            return global_fail(_r, _m);                   \
-        }                                                 \
-    } while (0)
-
+// ---------------------------------------------------------------- single graph
 int sgc_create(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
                const uint32_t* deg0, sgc_engine** out) {
     if (!cfg || !row_ptr || !deg0 || !out || n == 0) return global_fail(SGC_ERR_INVALID, "sgc_create: bad argument");
     if (cfg->batch_size == 0 || cfg->epoch_size == 0) return global_fail(SGC_ERR_INVALID, "sgc_create: zero batch/epoch");
-    const uint64_t nnz = row_ptr[n];
-    if (nnz && !col_idx) return global_fail(SGC_ERR_INVALID, "sgc_create: col_idx missing");
-    for (uint32_t v = 0; v < n; v++)
-        if (row_ptr[v + 1] < row_ptr[v]) return global_fail(SGC_ERR_INVALID, "sgc_create: row_ptr not monotone");
-    for (uint64_t j = 0; j < nnz; j++)
-        if (col_idx[j] >= n) return global_fail(SGC_ERR_INVALID, "sgc_create: column id out of range");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return global_fail(SGC_ERR_HIP, "sgc_create: no HIP device");
-
-    // slack capacities (A/src/streamgc.cu:373-378)
-    std::vector<uint32_t> rp(n + 1, 0);
-    uint64_t S = 0;
-    for (uint32_t v = 0; v < n; v++) {
-        const uint64_t d = row_ptr[v + 1] - row_ptr[v];
-        uint64_t cap = d + d * cfg->slack_pct / 100;
-        if (cap < cfg->min_row_capacity) cap = cfg->min_row_capacity;
-        S += cap;
-        if (S >= 0xFFFFFFFFull) return global_fail(SGC_ERR_RANGE, "sgc_create: slot count exceeds 32-bit offsets");
-        rp[v + 1] = (uint32_t)S;
-    }
-    // histogram rows for deg0 > threshold, in vertex order (A/src/streamgc.cu:226-250)
-    std::vector<uint32_t> hd(n, INVALID), hub_list;
-    uint32_t nhd = 0;
-    for (uint32_t v = 0; v < n; v++)
-        if (deg0[v] > cfg->histogram_threshold) {
-            hd[v] = nhd++;
-            hub_list.push_back(v);
-        }
-
     sgc_engine* E = new sgc_engine;
-    E->cfg = *cfg;
-    E->n = n;
-    E->S = S;
-    E->nhd = nhd;
-    TRY(hipGetDevice(&E->device), "device");
-    TRY(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking), "stream");
-    TRY(hipStreamCreateWithFlags(&E->qstream, hipStreamNonBlocking), "query stream");
-    for (int i = 0; i < 2; i++) {
-        TRY(hipEventCreateWithFlags(&E->pub_ev[i], hipEventDisableTiming), "event");
-        TRY(hipEventCreateWithFlags(&E->read_ev[i], hipEventDisableTiming), "event");
+    int r = pool_init(E, 1, cfg->batch_size, cfg->flags, 0);
+    uint32_t gid = 0;
+    if (r == SGC_OK) r = E->add_graph(cfg, n, row_ptr, col_idx, deg0, &gid);
+    if (r != SGC_OK) {
+        std::string m = "sgc_create: " + E->err;
+        E->release();
+        delete E;
+        return global_fail(r, m);
     }
-    TRY(hipMalloc(&E->word, (size_t)n * 4), "alloc word");
-    TRY(hipMalloc(&E->deg0, (size_t)n * 4), "alloc deg0");
-    TRY(hipMalloc(&E->row_ptr, ((size_t)n + 1) * 4), "alloc row_ptr");
-    TRY(hipMalloc(&E->col, S * 4), "alloc col");
-    TRY(hipMalloc(&E->bitmap, (size_t)n * 8), "alloc bitmap");
-    TRY(hipMalloc(&E->hd_map, (size_t)n * 4), "alloc hd_map");
-    TRY(hipMalloc(&E->hubs, (size_t)(nhd ? nhd : 1) * 4), "alloc hubs");
-    TRY(hipMalloc(&E->hist, (size_t)(nhd ? nhd : 1) * HIST_BINS * 4), "alloc hist");
-    TRY(hipMalloc(&E->ovf, (size_t)(cfg->overflow_capacity ? cfg->overflow_capacity : 1) * 8), "alloc overflow");
-    TRY(hipMalloc(&E->ctr, C_COUNT * 4), "alloc counters");
-    TRY(hipMalloc(&E->snap, (size_t)n * 2 * 2), "alloc snapshot");
-    TRY(hipMalloc(&E->dg, sizeof(GraphDev)), "alloc descriptor");
-    TRY(hipMalloc(&E->totals, T_COUNT * 8), "alloc totals");
-    TRY(hipMalloc(&E->batch, (size_t)cfg->batch_size * sizeof(sgc_edge_update)), "alloc batch");
-    TRY(hipMalloc(&E->seg, 2 * 4), "alloc seg");
-    E->eg.size = 16;
-    while (E->eg.size < 2u * cfg->batch_size) E->eg.size <<= 1;
-    E->eg.next_cap = cfg->batch_size;
-    TRY(hipMalloc(&E->eg.keys, (size_t)E->eg.size * 8), "alloc edge groups");
-    TRY(hipMalloc(&E->eg.head, (size_t)E->eg.size * 4), "alloc edge groups");
-    TRY(hipMalloc(&E->eg.next, (size_t)cfg->batch_size * 4), "alloc edge groups");
-    TRY(hipMalloc(&E->d_tmp16, 64), "alloc tmp");
-    TRY(hipHostMalloc(&E->h_seg, 2 * 4), "pinned seg");
-    TRY(hipHostMalloc(&E->h_ctr, C_COUNT * 4), "pinned counters");
-    TRY(hipHostMalloc(&E->h_totals, T_COUNT * 8), "pinned totals");
-
-    hipStream_t s = E->stream;
-    TRY(hipMemsetAsync(E->word, 0, (size_t)n * 4, s), "init");
-    TRY(hipMemsetAsync(E->bitmap, 0, (size_t)n * 8, s), "init");
-    TRY(hipMemsetAsync(E->hist, 0, (size_t)(nhd ? nhd : 1) * HIST_BINS * 4, s), "init");
-    TRY(hipMemsetAsync(E->ctr, 0, C_COUNT * 4, s), "init");
-    TRY(hipMemsetAsync(E->snap, 0, (size_t)n * 4, s), "init");
-    TRY(hipMemsetAsync(E->totals, 0, T_COUNT * 8, s), "init");
-    TRY(hipMemcpyAsync(E->deg0, deg0, (size_t)n * 4, hipMemcpyHostToDevice, s), "upload deg0");
-    TRY(hipMemcpyAsync(E->row_ptr, rp.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, s), "upload row_ptr");
-    TRY(hipMemcpyAsync(E->hd_map, hd.data(), (size_t)n * 4, hipMemcpyHostToDevice, s), "upload hd_map");
-    if (nhd) TRY(hipMemcpyAsync(E->hubs, hub_list.data(), (size_t)nhd * 4, hipMemcpyHostToDevice, s), "upload hubs");
-    {
-        // prefill the slack rows on the device from the loaded CSR
-        uint32_t *rp0 = nullptr, *col0 = nullptr;
-        TRY(hipMalloc(&rp0, ((size_t)n + 1) * 4), "alloc staging");
-        hipError_t e2 = hipMalloc(&col0, (nnz ? nnz : 1) * 4);
-        if (e2 != hipSuccess) {
-            (void)hipFree(rp0);
-            TRY(e2, "alloc staging");
-        }
-        e2 = hipMemcpyAsync(rp0, row_ptr, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, s);
-        if (e2 == hipSuccess && nnz) e2 = hipMemcpyAsync(col0, col_idx, nnz * 4, hipMemcpyHostToDevice, s);
-        if (e2 == hipSuccess) e2 = launch_fill_rows(s, n, rp0, col0, E->row_ptr, E->col);
-        if (e2 == hipSuccess) e2 = hipStreamSynchronize(s);
-        (void)hipFree(rp0);
-        (void)hipFree(col0);
-        TRY(e2, "slack rows");
-    }
-    GraphDev& g = E->hg;
-    g.word = E->word;
-    g.deg0 = E->deg0;
-    g.row_ptr = E->row_ptr;
-    g.col = E->col;
-    g.bitmap = E->bitmap;
-    g.hd_map = E->hd_map;
-    g.hubs = E->hubs;
-    g.hist = E->hist;
-    g.ovf = E->ovf;
-    g.ctr = E->ctr;
-    g.snap_write = E->snap;
-    g.n = n;
-    g.ovf_cap = cfg->overflow_capacity;
-    g.nhd = nhd;
-    TRY(hipMemcpyAsync(E->dg, &g, sizeof(GraphDev), hipMemcpyHostToDevice, s), "upload descriptor");
-    TRY(hipStreamSynchronize(s), "create sync");
     *out = E;
     return SGC_OK;
 }
@@ -595,29 +1001,25 @@ int sgc_destroy(sgc_engine* e) {
     return SGC_OK;
 }
 
+int sgc_set_flags(sgc_engine* e, uint32_t flags) {
+    if (!e || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM))) return SGC_ERR_INVALID;
+    e->flags = flags;
+    e->gs[0].cfg.flags = flags;
+    return SGC_OK;
+}
+
 int sgc_apply(sgc_engine* e, const sgc_edge_update* updates, uint64_t count, int on_device) {
     if (!e || (!updates && count)) return SGC_ERR_INVALID;
+    std::vector<uint64_t> pg;
     uint64_t done = 0;
     while (done < count) {
-        const uint32_t b = (uint32_t)std::min<uint64_t>(e->cfg.batch_size, count - done);
-        const sgc_edge_update* src = updates + done;
-        if (!on_device) {
-            hipError_t r = hipMemcpyAsync(e->batch, src, (size_t)b * sizeof(sgc_edge_update), hipMemcpyHostToDevice,
-                                          e->stream);
-            if (r != hipSuccess) return e->hip_fail(r, "batch upload");
-            src = e->batch;
-        }
-        int rc = e->apply_device(src, b);
-        if (rc) return rc;
-        if (!on_device) {
-            // the staging buffer is reused by the next batch
-            hipError_t r = hipStreamSynchronize(e->stream);
-            if (r != hipSuccess) return e->hip_fail(r, "apply");
-        }
+        const uint32_t b = (uint32_t)std::min<uint64_t>(e->batch_cap, count - done);
+        int r = e->apply(updates + done, b, on_device != 0, pg);
+        if (r) return r;
         done += b;
     }
-    hipError_t r = hipStreamSynchronize(e->stream);
-    if (r != hipSuccess) return e->hip_fail(r, "apply");
+    hipError_t h = hipStreamSynchronize(e->stream);
+    if (h != hipSuccess) return e->hip_fail(h, "apply");
     e->resolve_timing();
     return SGC_OK;
 }
@@ -625,240 +1027,182 @@ int sgc_apply(sgc_engine* e, const sgc_edge_update* updates, uint64_t count, int
 int sgc_submit(sgc_engine* e, const sgc_edge_update* updates, uint64_t count, int on_device,
                sgc_epoch_stats* epochs_out, uint32_t max_epochs, uint32_t* epochs_done) {
     if (!e || (!updates && count)) return SGC_ERR_INVALID;
-    auto t0 = std::chrono::steady_clock::now();
-    e->last_ms[2] = 0;
-    uint64_t done = 0, since = 0;
-    uint32_t ne = 0;
-    while (done < count) {
-        const uint32_t b = (uint32_t)std::min<uint64_t>(e->cfg.batch_size, count - done);
-        const sgc_edge_update* src = updates + done;
-        if (!on_device) {
-            hipError_t r = hipMemcpyAsync(e->batch, src, (size_t)b * sizeof(sgc_edge_update), hipMemcpyHostToDevice,
-                                          e->stream);
-            if (r != hipSuccess) return e->hip_fail(r, "batch upload");
-            src = e->batch;
-        }
-        int rc = e->apply_device(src, b);
-        if (rc) return rc;
-        done += b;
-        since += b;
-        if (since >= e->cfg.epoch_size) {
-            sgc_epoch_stats st{};
-            rc = e->epoch(epochs_out && ne < max_epochs ? &st : nullptr);
-            if (rc) return rc;
-            if (epochs_out && ne < max_epochs) epochs_out[ne] = st;
-            ne++;
-            since = 0;
-        }
-        if (!on_device) {
-            hipError_t r = hipStreamSynchronize(e->stream);
-            if (r != hipSuccess) return e->hip_fail(r, "submit");
-        }
-    }
-    hipError_t r = hipStreamSynchronize(e->stream);
-    if (r != hipSuccess) return e->hip_fail(r, "submit");
-    e->resolve_timing();
-    e->last_ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (epochs_done) *epochs_done = ne;
-    return SGC_OK;
+    return submit_impl(e, updates, count, on_device, epochs_out, max_epochs, epochs_done);
 }
 
 int sgc_epoch(sgc_engine* e, sgc_epoch_stats* out) {
     if (!e) return SGC_ERR_INVALID;
-    sgc_epoch_stats st{};
-    int r = e->epoch(&st);
+    std::vector<sgc_epoch_stats> st;
+    int r = e->epoch({0}, false, &st);
     if (r) return r;
     hipError_t h = hipStreamSynchronize(e->stream);
     if (h != hipSuccess) return e->hip_fail(h, "epoch");
     e->resolve_timing();
-    if (out) *out = st;
+    if (out) *out = st[0];
     return SGC_OK;
 }
 
 int sgc_finish(sgc_engine* e, sgc_epoch_stats* out) {
     if (!e) return SGC_ERR_INVALID;
-    sgc_epoch_stats st{};
-    int r = e->finish(&st);
+    std::vector<sgc_epoch_stats> st;
+    int r = e->epoch({0}, true, &st);
     if (r) return r;
     hipError_t h = hipStreamSynchronize(e->stream);
     if (h != hipSuccess) return e->hip_fail(h, "finish");
     e->resolve_timing();
-    if (out) *out = st;
-    return SGC_OK;
-}
-
-int sgc_set_flags(sgc_engine* e, uint32_t flags) {
-    if (!e || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM))) return SGC_ERR_INVALID;
-    e->cfg.flags = flags;
+    if (out) *out = st[0];
     return SGC_OK;
 }
 
 int sgc_query(sgc_engine* e, const uint32_t* ids, uint32_t count, uint16_t* out_colors) {
-    if (!e || (!ids && count) || (!out_colors && count)) return SGC_ERR_INVALID;
-    if (count == 0) return SGC_OK;
-    std::lock_guard<std::mutex> lk(e->qmu);
-    if (e->published < 0) {
-        memset(out_colors, 0, (size_t)count * 2);
-        return SGC_OK;
-    }
-    if (count > e->qcap) {
-        if (e->d_qids) (void)hipFree(e->d_qids);
-        if (e->d_qout) (void)hipFree(e->d_qout);
-        e->d_qids = nullptr;
-        e->d_qout = nullptr;
-        hipError_t r = hipMalloc(&e->d_qids, (size_t)count * 4);
-        if (r == hipSuccess) r = hipMalloc(&e->d_qout, (size_t)count * 2);
-        if (r != hipSuccess) {
-            e->qcap = 0;
-            return e->hip_fail(r, "query buffers");
-        }
-        e->qcap = count;
-    }
-    const int half = e->published;
-    hipError_t r = hipStreamWaitEvent(e->qstream, e->pub_ev[half], 0);
-    if (r == hipSuccess) r = hipMemcpyAsync(e->d_qids, ids, (size_t)count * 4, hipMemcpyHostToDevice, e->qstream);
-    if (r == hipSuccess) {
-        uint32_t grid = (count + 255) / 256;
-        k_gather16<<<grid > 1024 ? 1024 : grid, 256, 0, e->qstream>>>(e->snap + (size_t)half * e->n, e->n, e->d_qids,
-                                                                       count, e->d_qout);
-        r = hipGetLastError();
-    }
-    if (r == hipSuccess) r = hipEventRecord(e->read_ev[half], e->qstream);
-    if (r == hipSuccess) r = hipMemcpyAsync(out_colors, e->d_qout, (size_t)count * 2, hipMemcpyDeviceToHost, e->qstream);
-    if (r == hipSuccess) r = hipStreamSynchronize(e->qstream);
-    if (r != hipSuccess) return e->hip_fail(r, "query");
-    return SGC_OK;
+    if (!e) return SGC_ERR_INVALID;
+    return query_impl(e, 0, ids, count, out_colors);
 }
 
-static int download(sgc_engine* e, void* dst, const void* src, size_t bytes) {
-    if (!e || !dst) return SGC_ERR_INVALID;
-    hipError_t r = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, e->stream);
-    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
-    if (r != hipSuccess) return e->hip_fail(r, "download");
-    return SGC_OK;
+int sgc_download_words(sgc_engine* e, uint32_t* w) {
+    if (!e || !w) return SGC_ERR_INVALID;
+    return download(e, w, e->gs[0].word, (size_t)e->gs[0].n * 4);
 }
-static int upload(sgc_engine* e, void* dst, const void* src, size_t bytes) {
-    if (!e || !src) return SGC_ERR_INVALID;
-    hipError_t r = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream);
-    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
-    if (r != hipSuccess) return e->hip_fail(r, "upload");
-    return SGC_OK;
-}
-
-int sgc_download_words(sgc_engine* e, uint32_t* w) { return download(e, w, e ? e->word : nullptr, e ? (size_t)e->n * 4 : 0); }
 int sgc_download_colors(sgc_engine* e, uint16_t* c) {
     if (!e || !c) return SGC_ERR_INVALID;
-    std::vector<uint32_t> w(e->n);
+    std::vector<uint32_t> w(e->gs[0].n);
     int r = sgc_download_words(e, w.data());
     if (r) return r;
-    for (uint32_t i = 0; i < e->n; i++) c[i] = (uint16_t)(w[i] & 0xFFFF);
+    for (uint32_t i = 0; i < e->gs[0].n; i++) c[i] = (uint16_t)(w[i] & 0xFFFF);
     return SGC_OK;
 }
 int sgc_download_snapshot(sgc_engine* e, uint16_t* c) {
     if (!e || !c) return SGC_ERR_INVALID;
-    if (e->published < 0) {
-        memset(c, 0, (size_t)e->n * 2);
+    GraphState& G = e->gs[0];
+    if (G.published < 0) {
+        memset(c, 0, (size_t)G.n * 2);
         return SGC_OK;
     }
-    return download(e, c, e->snap + (size_t)e->published * e->n, (size_t)e->n * 2);
+    return download(e, c, G.snap + (size_t)G.published * G.n, (size_t)G.n * 2);
 }
 int sgc_num_slots(sgc_engine* e, uint64_t* s) {
     if (!e || !s) return SGC_ERR_INVALID;
-    *s = e->S;
+    *s = e->gs[0].S;
     return SGC_OK;
 }
 int sgc_download_csr(sgc_engine* e, uint32_t* rp, uint32_t* col) {
     if (!e || !rp || !col) return SGC_ERR_INVALID;
-    int r = download(e, rp, e->row_ptr, ((size_t)e->n + 1) * 4);
+    int r = download(e, rp, e->gs[0].row_ptr, ((size_t)e->gs[0].n + 1) * 4);
     if (r) return r;
-    return download(e, col, e->col, e->S * 4);
+    return download(e, col, e->gs[0].col, e->gs[0].S * 4);
 }
-int sgc_download_bitmaps(sgc_engine* e, uint64_t* b) { return download(e, b, e ? e->bitmap : nullptr, e ? (size_t)e->n * 8 : 0); }
+int sgc_download_bitmaps(sgc_engine* e, uint64_t* b) {
+    if (!e || !b) return SGC_ERR_INVALID;
+    return download(e, b, e->gs[0].bitmap, (size_t)e->gs[0].n * 8);
+}
 int sgc_num_hd(sgc_engine* e, uint32_t* k) {
     if (!e || !k) return SGC_ERR_INVALID;
-    *k = e->nhd;
+    *k = e->gs[0].nhd;
     return SGC_OK;
 }
 int sgc_download_hist(sgc_engine* e, uint32_t* h) {
     if (!e) return SGC_ERR_INVALID;
-    if (e->nhd == 0) return SGC_OK;
-    return download(e, h, e->hist, (size_t)e->nhd * HIST_BINS * 4);
+    if (e->gs[0].nhd == 0) return SGC_OK;
+    if (!h) return SGC_ERR_INVALID;
+    return download(e, h, e->gs[0].hist, (size_t)e->gs[0].nhd * HIST_BINS * 4);
 }
-int sgc_upload_words(sgc_engine* e, const uint32_t* w) { return upload(e, e ? e->word : nullptr, w, e ? (size_t)e->n * 4 : 0); }
-int sgc_upload_bitmaps(sgc_engine* e, const uint64_t* b) { return upload(e, e ? e->bitmap : nullptr, b, e ? (size_t)e->n * 8 : 0); }
+int sgc_upload_words(sgc_engine* e, const uint32_t* w) {
+    if (!e || !w) return SGC_ERR_INVALID;
+    return upload(e, e->gs[0].word, w, (size_t)e->gs[0].n * 4);
+}
+int sgc_upload_bitmaps(sgc_engine* e, const uint64_t* b) {
+    if (!e || !b) return SGC_ERR_INVALID;
+    return upload(e, e->gs[0].bitmap, b, (size_t)e->gs[0].n * 8);
+}
 int sgc_upload_hist(sgc_engine* e, const uint32_t* h) {
     if (!e) return SGC_ERR_INVALID;
-    if (e->nhd == 0) return SGC_OK;
-    return upload(e, e->hist, h, (size_t)e->nhd * HIST_BINS * 4);
+    if (e->gs[0].nhd == 0) return SGC_OK;
+    if (!h) return SGC_ERR_INVALID;
+    return upload(e, e->gs[0].hist, h, (size_t)e->gs[0].nhd * HIST_BINS * 4);
 }
 int sgc_overflow_count(sgc_engine* e, uint32_t* c) {
     if (!e || !c) return SGC_ERR_INVALID;
-    return download(e, c, e->ctr + C_OVF, 4);
+    return download(e, c, e->d_ctr + C_OVF, 4);
+}
+
+static int totals_impl(sgc_pool* P, sgc_totals* out) {
+    int r = download(P, P->h_totals, P->d_totals, T_COUNT * 8);
+    if (r) return r;
+    memset(out, 0, sizeof *out);
+    out->updates = P->h_totals[T_UPDATES];
+    out->scan_slots = P->h_totals[T_SCAN];
+    out->hd_endpoints = P->h_totals[T_HD];
+    out->recolors = P->h_totals[T_RECOLOR];
+    out->recolor_cap_slots = P->h_totals[T_RC_CAP];
+    out->recolor_live = P->h_totals[T_RC_LIVE];
+    out->overflow_events = P->h_totals[T_OVF_EVENTS];
+    out->live_slots = P->h_totals[T_LIVE_SLOTS];
+    out->grouped = P->h_totals[T_GROUPED];
+    for (auto& G : P->gs) out->epochs += G.epochs;
+    out->t1_sweeps = P->t1_sweeps;
+    out->t2_runs = P->t2_runs;
+    return SGC_OK;
 }
 int sgc_totals_get(sgc_engine* e, sgc_totals* out) {
     if (!e || !out) return SGC_ERR_INVALID;
-    int r = download(e, e->h_totals, e->totals, T_COUNT * 8);
-    if (r) return r;
-    memset(out, 0, sizeof *out);
-    out->updates = e->h_totals[T_UPDATES];
-    out->scan_slots = e->h_totals[T_SCAN];
-    out->hd_endpoints = e->h_totals[T_HD];
-    out->recolors = e->h_totals[T_RECOLOR];
-    out->recolor_cap_slots = e->h_totals[T_RC_CAP];
-    out->recolor_live = e->h_totals[T_RC_LIVE];
-    out->overflow_events = e->h_totals[T_OVF_EVENTS];
-    out->live_slots = e->h_totals[T_LIVE_SLOTS];
-    out->grouped = e->h_totals[T_GROUPED];
-    out->epochs = e->epochs;
-    out->t1_sweeps = e->t1_sweeps;
-    out->t2_runs = e->t2_runs;
-    return SGC_OK;
+    return totals_impl(e, out);
 }
 
 int sgc_find_color(sgc_engine* e, uint32_t v, uint16_t* out) {
-    if (!e || !out || v >= e->n) return SGC_ERR_INVALID;
-    LaunchCtx L = e->ctx();
-    hipError_t r = launch_find_color(L, 0, v, e->d_tmp16);
+    if (!e || !out || v >= e->gs[0].n) return SGC_ERR_INVALID;
+    hipError_t r = launch_find_color(e->ctx(false), 0, v, e->d_tmp16);
     if (r != hipSuccess) return e->hip_fail(r, "find colour");
     return download(e, out, e->d_tmp16, 2);
 }
 
-int sgc_validate(sgc_engine* e, int use_snapshot, uint64_t* violations, uint64_t* uncolored, uint32_t* max_color) {
-    if (!e) return SGC_ERR_INVALID;
-    if (use_snapshot && e->published < 0) return e->fail(SGC_ERR_STATE, "no snapshot published");
-    hipError_t r = hipMemsetAsync(e->ctr + C_VIOL, 0, 3 * 4, e->stream);
-    LaunchCtx L = e->ctx();
-    const uint16_t* colors = use_snapshot ? e->snap + (size_t)e->published * e->n : nullptr;
-    if (r == hipSuccess) r = launch_validate(L, 0, e->n, colors);
-    if (r == hipSuccess) r = e->read_ctr();
-    if (r != hipSuccess) return e->hip_fail(r, "validate");
-    if (violations) *violations = e->h_ctr[C_VIOL];
-    if (uncolored) *uncolored = e->h_ctr[C_UNCOL];
-    if (max_color) *max_color = e->h_ctr[C_MAXC];
+static int validate_impl(sgc_pool* P, uint32_t gid, int use_snapshot, uint64_t* violations, uint64_t* uncolored,
+                         uint32_t* max_color) {
+    GraphState* G = graph_of(P, gid);
+    if (!G) return SGC_ERR_INVALID;
+    if (use_snapshot && G->published < 0) return P->fail(SGC_ERR_STATE, "no snapshot published");
+    hipError_t r = hipMemsetAsync(P->d_ctr + (size_t)gid * C_COUNT + C_VIOL, 0, 3 * 4, P->stream);
+    const uint16_t* colors = use_snapshot ? G->snap + (size_t)G->published * G->n : nullptr;
+    if (r == hipSuccess) r = launch_validate(P->ctx(false), gid, G->n, colors);
+    if (r == hipSuccess) r = P->read_ctr();
+    if (r != hipSuccess) return P->hip_fail(r, "validate");
+    if (violations) *violations = P->ctr(gid, C_VIOL);
+    if (uncolored) *uncolored = P->ctr(gid, C_UNCOL);
+    if (max_color) *max_color = P->ctr(gid, C_MAXC);
     return SGC_OK;
 }
+int sgc_validate(sgc_engine* e, int use_snapshot, uint64_t* violations, uint64_t* uncolored, uint32_t* max_color) {
+    if (!e) return SGC_ERR_INVALID;
+    return validate_impl(e, 0, use_snapshot, violations, uncolored, max_color);
+}
 
+static int history_reserve_impl(sgc_pool* P, uint32_t gid, uint32_t capacity) {
+    GraphState* G = graph_of(P, gid);
+    if (!G) return SGC_ERR_INVALID;
+    (void)hipStreamSynchronize(P->stream);
+    if (G->history) (void)hipFree(G->history);
+    G->history = nullptr;
+    G->hist_cap = G->hist_count = 0;
+    if (capacity == 0) return SGC_OK;
+    hipError_t r = hipMalloc(&G->history, (size_t)capacity * G->n * 2);
+    if (r != hipSuccess) return P->hip_fail(r, "history alloc");
+    G->hist_cap = capacity;
+    return SGC_OK;
+}
 int sgc_history_reserve(sgc_engine* e, uint32_t capacity) {
     if (!e) return SGC_ERR_INVALID;
-    (void)hipStreamSynchronize(e->stream);
-    if (e->history) (void)hipFree(e->history);
-    e->history = nullptr;
-    e->hist_cap = e->hist_count = 0;
-    if (capacity == 0) return SGC_OK;
-    hipError_t r = hipMalloc(&e->history, (size_t)capacity * e->n * 2);
-    if (r != hipSuccess) return e->hip_fail(r, "history alloc");
-    e->hist_cap = capacity;
-    return SGC_OK;
+    return history_reserve_impl(e, 0, capacity);
 }
 int sgc_history_count(sgc_engine* e, uint32_t* c) {
     if (!e || !c) return SGC_ERR_INVALID;
-    *c = e->hist_count;
+    *c = e->gs[0].hist_count;
     return SGC_OK;
 }
 
-int sgc_validate_history(sgc_engine* e, const uint32_t* eu, const uint32_t* ev, const uint64_t* born,
-                         const uint64_t* died, uint64_t m, const uint64_t* pos, uint32_t count, uint64_t* bad_out) {
-    if (!e || !pos || !bad_out || count > e->hist_count) return SGC_ERR_INVALID;
+static int validate_history_impl(sgc_pool* P, uint32_t gid, const uint32_t* eu, const uint32_t* ev,
+                                 const uint64_t* born, const uint64_t* died, uint64_t m, const uint64_t* pos,
+                                 uint32_t count, uint64_t* bad_out) {
+    GraphState* G = graph_of(P, gid);
+    if (!G || !pos || !bad_out || count > G->hist_count) return SGC_ERR_INVALID;
     if (count == 0) return SGC_OK;
     uint32_t *deu = nullptr, *dev = nullptr;
     uint64_t *dborn = nullptr, *ddied = nullptr, *dpos = nullptr;
@@ -870,20 +1214,25 @@ int sgc_validate_history(sgc_engine* e, const uint32_t* eu, const uint32_t* ev, 
     if (r == hipSuccess) r = hipMalloc(&ddied, mm * 8);
     if (r == hipSuccess) r = hipMalloc(&dpos, (size_t)count * 8);
     if (r == hipSuccess) r = hipMalloc(&dbad, (size_t)count * 8);
-    if (r == hipSuccess && m) r = hipMemcpyAsync(deu, eu, m * 4, hipMemcpyHostToDevice, e->stream);
-    if (r == hipSuccess && m) r = hipMemcpyAsync(dev, ev, m * 4, hipMemcpyHostToDevice, e->stream);
-    if (r == hipSuccess && m) r = hipMemcpyAsync(dborn, born, m * 8, hipMemcpyHostToDevice, e->stream);
-    if (r == hipSuccess && m) r = hipMemcpyAsync(ddied, died, m * 8, hipMemcpyHostToDevice, e->stream);
-    if (r == hipSuccess) r = hipMemcpyAsync(dpos, pos, (size_t)count * 8, hipMemcpyHostToDevice, e->stream);
-    if (r == hipSuccess) r = hipMemsetAsync(dbad, 0, (size_t)count * 8, e->stream);
-    if (r == hipSuccess)
-        r = launch_validate_history(e->stream, e->history, e->n, deu, dev, dborn, ddied, m, dpos, count, dbad);
-    if (r == hipSuccess) r = hipMemcpyAsync(bad_out, dbad, (size_t)count * 8, hipMemcpyDeviceToHost, e->stream);
-    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    hipStream_t s = P->stream;
+    if (r == hipSuccess && m) r = hipMemcpyAsync(deu, eu, m * 4, hipMemcpyHostToDevice, s);
+    if (r == hipSuccess && m) r = hipMemcpyAsync(dev, ev, m * 4, hipMemcpyHostToDevice, s);
+    if (r == hipSuccess && m) r = hipMemcpyAsync(dborn, born, m * 8, hipMemcpyHostToDevice, s);
+    if (r == hipSuccess && m) r = hipMemcpyAsync(ddied, died, m * 8, hipMemcpyHostToDevice, s);
+    if (r == hipSuccess) r = hipMemcpyAsync(dpos, pos, (size_t)count * 8, hipMemcpyHostToDevice, s);
+    if (r == hipSuccess) r = hipMemsetAsync(dbad, 0, (size_t)count * 8, s);
+    if (r == hipSuccess) r = launch_validate_history(s, G->history, G->n, deu, dev, dborn, ddied, m, dpos, count, dbad);
+    if (r == hipSuccess) r = hipMemcpyAsync(bad_out, dbad, (size_t)count * 8, hipMemcpyDeviceToHost, s);
+    if (r == hipSuccess) r = hipStreamSynchronize(s);
     for (void* p : {(void*)deu, (void*)dev, (void*)dborn, (void*)ddied, (void*)dpos, (void*)dbad})
         if (p) (void)hipFree(p);
-    if (r != hipSuccess) return e->hip_fail(r, "validate history");
+    if (r != hipSuccess) return P->hip_fail(r, "validate history");
     return SGC_OK;
+}
+int sgc_validate_history(sgc_engine* e, const uint32_t* eu, const uint32_t* ev, const uint64_t* born,
+                         const uint64_t* died, uint64_t m, const uint64_t* pos, uint32_t count, uint64_t* bad_out) {
+    if (!e) return SGC_ERR_INVALID;
+    return validate_history_impl(e, 0, eu, ev, born, died, m, pos, count, bad_out);
 }
 
 int sgc_device_upload(const void* host, uint64_t bytes, void** dev_out) {
@@ -904,32 +1253,37 @@ int sgc_device_free(void* dev) {
     return SGC_OK;
 }
 
-int sgc_last_timing(sgc_engine* e, double* ms4) {
-    if (!e || !ms4) return SGC_ERR_INVALID;
+static int last_timing_impl(sgc_pool* P, double* ms4) {
     double upd = 0, ep = 0;
-    for (auto& kv : e->ktimes) {
+    for (auto& kv : P->ktimes) {
         if (kv.first == "apply") upd += kv.second.ms;
         else ep += kv.second.ms;
     }
     ms4[0] = upd;
     ms4[1] = ep;
-    ms4[2] = e->last_ms[2];
-    ms4[3] = e->last_ms[3];
+    ms4[2] = P->last_ms[2];
+    ms4[3] = P->last_ms[3];
+    return SGC_OK;
+}
+int sgc_last_timing(sgc_engine* e, double* ms4) {
+    if (!e || !ms4) return SGC_ERR_INVALID;
+    return last_timing_impl(e, ms4);
+}
+static int timing_enable_impl(sgc_pool* P, int enable) {
+    P->resolve_timing();
+    P->ktiming = enable != 0;
+    if (enable) P->ktimes.clear();
     return SGC_OK;
 }
 int sgc_kernel_timing_enable(sgc_engine* e, int enable) {
     if (!e) return SGC_ERR_INVALID;
-    e->resolve_timing();
-    e->ktiming = enable != 0;
-    if (enable) e->ktimes.clear();
-    return SGC_OK;
+    return timing_enable_impl(e, enable);
 }
-int sgc_kernel_times(sgc_engine* e, char* names, uint32_t stride, double* total_ms, uint64_t* launches, uint32_t cap,
-                     uint32_t* count) {
-    if (!e || !count) return SGC_ERR_INVALID;
-    e->resolve_timing();
+static int kernel_times_impl(sgc_pool* P, char* names, uint32_t stride, double* total_ms, uint64_t* launches,
+                             uint32_t cap, uint32_t* count) {
+    P->resolve_timing();
     uint32_t k = 0;
-    for (auto& kv : e->ktimes) {
+    for (auto& kv : P->ktimes) {
         if (k < cap) {
             if (names && stride) {
                 strncpy(names + (size_t)k * stride, kv.first.c_str(), stride - 1);
@@ -942,6 +1296,181 @@ int sgc_kernel_times(sgc_engine* e, char* names, uint32_t stride, double* total_
     }
     *count = k;
     return SGC_OK;
+}
+int sgc_kernel_times(sgc_engine* e, char* names, uint32_t stride, double* total_ms, uint64_t* launches, uint32_t cap,
+                     uint32_t* count) {
+    if (!e || !count) return SGC_ERR_INVALID;
+    return kernel_times_impl(e, names, stride, total_ms, launches, cap, count);
+}
+
+// ---------------------------------------------------------------- pools
+int sgc_pool_create(const sgc_pool_config* cfg, sgc_pool** out) {
+    if (!cfg || !out || cfg->max_graphs == 0 || cfg->batch_capacity == 0)
+        return global_fail(SGC_ERR_INVALID, "pool create: bad argument");
+    if (cfg->flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM))
+        return global_fail(SGC_ERR_INVALID, "pool create: unknown flags");
+    sgc_pool* P = new sgc_pool;
+    int r = pool_init(P, cfg->max_graphs, cfg->batch_capacity, cfg->flags, cfg->hbm_budget);
+    if (r != SGC_OK) {
+        std::string m = P->err;
+        P->release();
+        delete P;
+        return global_fail(r, m);
+    }
+    *out = P;
+    return SGC_OK;
+}
+int sgc_pool_destroy(sgc_pool* p) {
+    if (!p) return SGC_ERR_INVALID;
+    p->release();
+    delete p;
+    return SGC_OK;
+}
+int sgc_pool_graph_bytes(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* deg0,
+                         uint64_t* bytes) {
+    if (!cfg || !row_ptr || !deg0 || !bytes) return SGC_ERR_INVALID;
+    std::vector<uint32_t> rp, hd, hubs;
+    uint64_t S = 0;
+    int r = plan_graph(cfg, n, row_ptr, deg0, rp, hd, hubs, S);
+    if (r) return r;
+    *bytes = graph_bytes(n, S, (uint32_t)hubs.size(), cfg->overflow_capacity);
+    return SGC_OK;
+}
+int sgc_pool_add(sgc_pool* p, const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+                 const uint32_t* deg0, uint32_t* gid) {
+    if (!p || !gid) return SGC_ERR_INVALID;
+    return p->add_graph(cfg, n, row_ptr, col_idx, deg0, gid);   // the pool's flags govern its graphs
+}
+int sgc_pool_remove(sgc_pool* p, uint32_t gid) {
+    if (!p) return SGC_ERR_INVALID;
+    return p->remove_graph(gid);
+}
+int sgc_pool_usage(sgc_pool* p, uint64_t* used, uint64_t* budget, uint32_t* graphs) {
+    if (!p) return SGC_ERR_INVALID;
+    if (used) *used = p->used_bytes;
+    if (budget) *budget = p->budget;
+    if (graphs) {
+        uint32_t k = 0;
+        for (auto& G : p->gs) k += G.used;
+        *graphs = k;
+    }
+    return SGC_OK;
+}
+int sgc_pool_set_flags(sgc_pool* p, uint32_t flags) {
+    if (!p || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM))) return SGC_ERR_INVALID;
+    p->flags = flags;
+    for (auto& G : p->gs) G.cfg.flags = flags;
+    return SGC_OK;
+}
+int sgc_pool_apply(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, int on_device) {
+    if (!p || (!updates && count)) return SGC_ERR_INVALID;
+    std::vector<uint64_t> pg;
+    uint64_t done = 0;
+    while (done < count) {
+        const uint32_t b = (uint32_t)std::min<uint64_t>(p->batch_cap, count - done);
+        int r = p->apply(updates + done, b, on_device != 0, pg);
+        if (r) return r;
+        done += b;
+    }
+    hipError_t h = hipStreamSynchronize(p->stream);
+    if (h != hipSuccess) return p->hip_fail(h, "pool apply");
+    p->resolve_timing();
+    return SGC_OK;
+}
+int sgc_pool_due(sgc_pool* p, uint32_t* gids, uint32_t cap, uint32_t* count) {
+    if (!p || !count) return SGC_ERR_INVALID;
+    uint32_t k = 0;
+    for (uint32_t g = 0; g < p->ng; g++)
+        if (p->gs[g].used && p->gs[g].since_epoch >= p->gs[g].cfg.epoch_size) {
+            if (gids && k < cap) gids[k] = g;
+            k++;
+        }
+    *count = k;
+    return SGC_OK;
+}
+static int pool_epoch_impl(sgc_pool* p, const uint32_t* gids, uint32_t count, sgc_epoch_stats* out, bool finish) {
+    if (!p || (!gids && count)) return SGC_ERR_INVALID;
+    std::vector<uint32_t> set;
+    for (uint32_t k = 0; k < count; k++) {
+        if (!graph_of(p, gids[k])) return p->fail(SGC_ERR_INVALID, "epoch: unknown graph");
+        if (std::find(set.begin(), set.end(), gids[k]) == set.end()) set.push_back(gids[k]);
+    }
+    std::vector<sgc_epoch_stats> st;
+    int r = p->epoch(set, finish, out ? &st : nullptr);
+    if (r) return r;
+    hipError_t h = hipStreamSynchronize(p->stream);
+    if (h != hipSuccess) return p->hip_fail(h, "pool epoch");
+    p->resolve_timing();
+    if (out)
+        for (uint32_t k = 0; k < count; k++) {
+            const size_t at = std::find(set.begin(), set.end(), gids[k]) - set.begin();
+            out[k] = st[at];
+        }
+    return SGC_OK;
+}
+int sgc_pool_epoch(sgc_pool* p, const uint32_t* gids, uint32_t count, sgc_epoch_stats* out) {
+    return pool_epoch_impl(p, gids, count, out, false);
+}
+int sgc_pool_finish(sgc_pool* p, const uint32_t* gids, uint32_t count, sgc_epoch_stats* out) {
+    return pool_epoch_impl(p, gids, count, out, true);
+}
+int sgc_pool_submit(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, int on_device,
+                    sgc_epoch_stats* stats_out, uint32_t max_stats, uint32_t* nstats) {
+    if (!p || (!updates && count)) return SGC_ERR_INVALID;
+    if (on_device) return p->fail(SGC_ERR_INVALID, "pool submit takes host records (split per graph on the host)");
+    return pool_submit_impl(p, updates, count, stats_out, max_stats, nstats);
+}
+int sgc_pool_query(sgc_pool* p, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors) {
+    if (!p) return SGC_ERR_INVALID;
+    return query_impl(p, gid, ids, count, out_colors);
+}
+int sgc_pool_download_words(sgc_pool* p, uint32_t gid, uint32_t* words) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || !words) return SGC_ERR_INVALID;
+    return download(p, words, G->word, (size_t)G->n * 4);
+}
+int sgc_pool_download_snapshot(sgc_pool* p, uint32_t gid, uint16_t* colors) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || !colors) return SGC_ERR_INVALID;
+    if (G->published < 0) {
+        memset(colors, 0, (size_t)G->n * 2);
+        return SGC_OK;
+    }
+    return download(p, colors, G->snap + (size_t)G->published * G->n, (size_t)G->n * 2);
+}
+int sgc_pool_validate(sgc_pool* p, uint32_t gid, int use_snapshot, uint64_t* violations, uint64_t* uncolored,
+                      uint32_t* max_color) {
+    if (!p) return SGC_ERR_INVALID;
+    return validate_impl(p, gid, use_snapshot, violations, uncolored, max_color);
+}
+int sgc_pool_history_reserve(sgc_pool* p, uint32_t gid, uint32_t capacity) {
+    if (!p) return SGC_ERR_INVALID;
+    return history_reserve_impl(p, gid, capacity);
+}
+int sgc_pool_history_count(sgc_pool* p, uint32_t gid, uint32_t* count) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || !count) return SGC_ERR_INVALID;
+    *count = G->hist_count;
+    return SGC_OK;
+}
+int sgc_pool_validate_history(sgc_pool* p, uint32_t gid, const uint32_t* eu, const uint32_t* ev, const uint64_t* born,
+                              const uint64_t* died, uint64_t m, const uint64_t* pos, uint32_t count,
+                              uint64_t* bad_out) {
+    if (!p) return SGC_ERR_INVALID;
+    return validate_history_impl(p, gid, eu, ev, born, died, m, pos, count, bad_out);
+}
+int sgc_pool_totals(sgc_pool* p, sgc_totals* out) {
+    if (!p || !out) return SGC_ERR_INVALID;
+    return totals_impl(p, out);
+}
+int sgc_pool_kernel_timing_enable(sgc_pool* p, int enable) {
+    if (!p) return SGC_ERR_INVALID;
+    return timing_enable_impl(p, enable);
+}
+int sgc_pool_kernel_times(sgc_pool* p, char* names, uint32_t stride, double* total_ms, uint64_t* launches,
+                          uint32_t cap, uint32_t* count) {
+    if (!p || !count) return SGC_ERR_INVALID;
+    return kernel_times_impl(p, names, stride, total_ms, launches, cap, count);
 }
 
 }  // extern "C"

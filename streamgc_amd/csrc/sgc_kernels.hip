@@ -157,13 +157,50 @@ __global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__
     if (grouped && totals && lane() == 0) add64(&totals[T_GROUPED], grouped);
 }
 
+// One wavefront per segment, the segment's updates in order; positions go
+// through `perm` when given (per-graph order inside a mixed batch).
 __global__ __launch_bounds__(256) void k_apply_seg(const GraphDev* __restrict__ graphs,
                                                    const uint4* __restrict__ upd, const uint32_t* __restrict__ seg,
-                                                   uint32_t nseg, int multi, uint64_t* totals) {
+                                                   uint32_t nseg, const uint32_t* __restrict__ perm, int multi,
+                                                   uint64_t* totals) {
     WaveAcc acc;
     for (uint32_t k = wave_id(); k < nseg; k += wave_count())
-        for (uint32_t i = seg[2 * k]; i < seg[2 * k + 1]; i++) apply_record(graphs, upd, i, multi, acc);
+        for (uint32_t i = seg[2 * k]; i < seg[2 * k + 1]; i++)
+            apply_record(graphs, upd, perm ? perm[i] : i, multi, acc);
     acc.flush(totals);
+}
+
+// ---------------------------------------------------------------- per-graph control
+__global__ void k_zero_ctr(const GraphDev* __restrict__ graphs, uint32_t ng, const uint32_t* __restrict__ active,
+                           uint32_t idx) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < ng && (!active || active[gid])) gp(graphs[gid].ctr)[idx] = 0;
+}
+
+// Maximum colour per graph into ctr[C_MAXC] (A/src/streamgc.cu:549-553).
+__global__ __launch_bounds__(256) void k_max_color(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                   const uint32_t* __restrict__ vbase, uint32_t V,
+                                                   const uint32_t* __restrict__ active) {
+    uint32_t cur = NONE, mx = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        uint32_t gid, v;
+        locate(vbase, ng, i, gid, v);
+        if (active && !active[gid]) continue;
+        if (gid != cur) {
+            if (cur != NONE && mx) max32(&graphs[cur].ctr[C_MAXC], mx);
+            cur = gid;
+            mx = 0;
+        }
+        const uint32_t c = color_of(gp(graphs[gid].word)[v]);
+        mx = c > mx ? c : mx;
+    }
+    const uint32_t c0 = bcast0(cur);
+    if (ballot(cur != c0) == 0) {              // the whole wave on one graph: one atomic
+        mx = wave_max32(mx);
+        if (lane() == 0 && cur != NONE && mx) max32(&graphs[cur].ctr[C_MAXC], mx);
+    } else if (cur != NONE && mx) {
+        max32(&graphs[cur].ctr[C_MAXC], mx);
+    }
 }
 
 // ---------------------------------------------------------------- snapshot
@@ -526,10 +563,20 @@ hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int
                                               mask, L.totals);
     return hipGetLastError();
 }
-hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg, int multi) {
+hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg,
+                            const uint32_t* perm, int multi) {
     if (nseg == 0) return hipSuccess;
     uint32_t grid = grid_for_waves(nseg, 1, L.max_blocks);
-    k_apply_seg<<<grid, 64, 0, L.stream>>>(L.graphs, (const uint4*)upd, seg, nseg, multi, L.totals);
+    k_apply_seg<<<grid, 64, 0, L.stream>>>(L.graphs, (const uint4*)upd, seg, nseg, perm, multi, L.totals);
+    return hipGetLastError();
+}
+hipError_t launch_zero_ctr(const LaunchCtx& L, uint32_t idx) {
+    k_zero_ctr<<<(L.ng + 63) / 64, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, idx);
+    return hipGetLastError();
+}
+hipError_t launch_max_color(const LaunchCtx& L) {
+    uint32_t grid = grid_for_waves(L.V, 4 * WAVE, 1024);
+    k_max_color<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
     return hipGetLastError();
 }
 hipError_t launch_snapshot(const LaunchCtx& L) {

@@ -34,7 +34,10 @@ struct EdgeGroups {
 };
 
 hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg);
-hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg, int multi);
+hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg,
+                            const uint32_t* perm, int multi);
+hipError_t launch_zero_ctr(const LaunchCtx& L, uint32_t idx);
+hipError_t launch_max_color(const LaunchCtx& L);
 hipError_t launch_snapshot(const LaunchCtx& L);
 hipError_t launch_compact(const LaunchCtx& L);
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
