@@ -322,7 +322,6 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
                         const uint32_t* deg0, uint32_t* gid_out) {
     if (!cfg || !row_ptr || !deg0 || n == 0) return fail(SGC_ERR_INVALID, "add graph: bad argument");
     if (cfg->batch_size == 0 || cfg->epoch_size == 0) return fail(SGC_ERR_INVALID, "add graph: zero batch/epoch");
-    if (cfg->batch_size > batch_cap) return fail(SGC_ERR_INVALID, "add graph: batch larger than the pool's");
     const uint64_t nnz = row_ptr[n];
     if (nnz && !col_idx) return fail(SGC_ERR_INVALID, "add graph: col_idx missing");
     for (uint32_t v = 0; v < n; v++)
@@ -1013,7 +1012,7 @@ int sgc_apply(sgc_engine* e, const sgc_edge_update* updates, uint64_t count, int
     std::vector<uint64_t> pg;
     uint64_t done = 0;
     while (done < count) {
-        const uint32_t b = (uint32_t)std::min<uint64_t>(e->batch_cap, count - done);
+        const uint32_t b = (uint32_t)std::min<uint64_t>(std::min(e->batch_cap, e->gs[0].cfg.batch_size), count - done);
         int r = e->apply(updates + done, b, on_device != 0, pg);
         if (r) return r;
         done += b;

@@ -55,12 +55,11 @@ def test_deterministic_pool_equals_each_graph_alone(repair, graph_path):
         assert np.array_equal(pool.words(gid[k]), solo.words()), NAMES[k]
         assert np.array_equal(pool.snapshot(gid[k]), solo.snapshot()), NAMES[k]
         flags = (O.FLAG_REPAIR_EVERY_EPOCH | O.FLAG_EXACT_HISTOGRAM) if repair else 0
-        orc = O.Engine(O.Csr(g.n, g.row_ptr, g.col_idx, g.degree), O.default_config(epoch_size=E, batch_size=E))
+        orc = O.Engine(O.Csr(g.n, g.row_ptr, g.col_idx, g.degree),
+                       O.default_config(epoch_size=E, batch_size=E, flags=flags))
         i0 = S.generate_init_stream(g)
-        orc.set_flags(flags & O.FLAG_EXACT_HISTOGRAM)
         orc.submit(i0["u"], i0["v"], i0["type"])
         orc.finish()
-        orc.set_flags(flags)
         b0 = S.generate_bench_stream_edges(g, 700, 50, 11 + k)
         orc.submit(b0["u"], b0["v"], b0["type"])
         assert np.array_equal(pool.words(gid[k]), orc.words()), NAMES[k]
