@@ -48,18 +48,30 @@ def make_graph(S, w):
     return S.gen_delaunay(w["points"], seed=42)
 
 
-def kernel_bytes(name, n, S_slots, nnz, upd_tot):
-    """Algorithmic bytes of one launch (per-unit figures of the roofline model)."""
+SWEEP_ROW_BYTES = {"compact": 8, "rebuild": 16, "tier1": 12, "t2_check": 12}
+
+
+def kernel_bytes(name, launches, n, tot, live_frac):
+    """Algorithmic bytes of one launch (per-unit figures of the roofline model).
+
+    apply: 48 B per update record + 4 B per row slot scanned + 4 B per histogram
+    endpoint + 4 B per recolour + 4 B per recolour row slot + 8 B per live
+    neighbour told of a recolour.  Sweeps (over the rows they visit, counted
+    on the device): row offsets and the per-row word (SWEEP_ROW_BYTES) + 4 B
+    per row slot + 4 B per live slot (the neighbour's word; compaction moves
+    slots instead and its minimum is the read).  build_list: 2 B per vertex
+    (dirty and sticky bytes).  snapshot: 6 B per vertex.
+    """
+    launches = max(launches, 1)
     if name == "apply":
-        u = upd_tot
-        return (48 * u["updates"] + 4 * u["scan_slots"] + 4 * u["hd_endpoints"] + 4 * u["recolor_cap_slots"]
-                + 8 * u["recolor_live"] + 4 * u["recolors"]) / max(u["launches"], 1)
-    if name == "compact":
-        return 8 * S_slots
-    if name == "rebuild":
-        return 4 * S_slots + 4 * nnz + 8 * n
-    if name in ("tier1", "t2_check"):
-        return 4 * S_slots + 4 * nnz + 4 * n
+        return (48 * tot["updates"] + 4 * tot["scan_slots"] + 4 * tot["hd_endpoints"] + 4 * tot["recolor_cap_slots"]
+                + 8 * tot["recolor_live"] + 4 * tot["recolors"]) / launches
+    if name in SWEEP_ROW_BYTES:
+        rows, slots = tot[f"rows_{name}"], tot[f"slots_{name}"]
+        live = 0 if name == "compact" else live_frac * slots
+        return (SWEEP_ROW_BYTES[name] * rows + 4 * slots + 4 * live) / launches
+    if name == "build_list":
+        return 2 * n
     if name == "snapshot":
         return 6 * n
     return 0
@@ -94,6 +106,8 @@ def main():
     ap.add_argument("--cpu-batches", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--full-sweeps", action="store_true",
+                    help="maintenance sweeps visit every vertex instead of the dirty worklists")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this "
@@ -115,7 +129,8 @@ def main():
     bench = S.generate_bench_stream_edges(g, (W + K) * E, 50, 42)
     log(f"streams: init {len(init)}, churn {len(bench)} ({time.perf_counter() - t:.1f}s)")
 
-    eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True))
+    eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True,
+                                             full_sweeps=args.full_sweeps))
     t = time.perf_counter()
     ds_init = S.DeviceStream(init)
     eng.process_stream(ds_init, max_epochs=0)
@@ -150,14 +165,10 @@ def main():
     # roofline of the dominant kernel
     d = {k: tot1[k] - tot0[k] for k in tot1}
     n = g.n
-    slots = int(np.sum(np.maximum(g.degree.astype(np.int64) * 4, 8)))   # slack 300%, floor 8
-    comp_launches = ktimes.get("compact", (0, 0))[1]
-    nnz_live = d["live_slots"] / comp_launches if comp_launches else g.num_edges
-    upd = dict(d)
-    upd["launches"] = ktimes.get("apply", (0, 1))[1]
+    live_frac = d["live_slots"] / d["slots_compact"] if d["slots_compact"] else 0.25
     per_kernel = {}
     for name, (ms, launches) in ktimes.items():
-        b = kernel_bytes(name, n, slots, nnz_live, upd)
+        b = kernel_bytes(name, launches, n, d, live_frac)
         avg_ms = ms / max(launches, 1)
         per_kernel[name] = dict(total_ms=round(ms, 3), launches=int(launches), avg_us=round(avg_ms * 1e3, 2),
                                 bytes_per_launch=int(b),
@@ -198,7 +209,9 @@ def main():
         "ms_per_step": round(elapsed * 1e3 / K, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": w["name"], "vertices": g.n, "edges": g.num_edges // 2, "epoch_size": E,
-                   "updates_timed": K * E, "maintenance": "full sweep", "epoch": "compact, drain, tier-1, "
+                   "updates_timed": K * E,
+                   "maintenance": "full sweeps" if args.full_sweeps else "dirty worklists",
+                   "epoch": "compact, drain, tier-1, "
                    "tier-2 check (host tier-2 when needed), publish"},
         "latency_ms": {"update_to_snapshot_p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "update_to_snapshot_p99": round(float(np.percentile(lat_ms, 99)), 3)},
@@ -206,6 +219,7 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "kernels": per_kernel,
+        "sweeps": {k: int(v) for k, v in d.items() if k.startswith(("rows_", "slots_"))},
     }
     print(json.dumps(out))
 

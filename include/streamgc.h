@@ -55,6 +55,7 @@ typedef struct sgc_edge_update {
 #define SGC_DETERMINISTIC (1u << 0)      /* one wavefront, updates in stream order, repair in vertex order */
 #define SGC_REPAIR_EVERY_EPOCH (1u << 1) /* epoch = compact, drain, Tier-1, Tier-2, publish (proper snapshots) */
 #define SGC_EXACT_HISTOGRAM (1u << 2)    /* hub histograms recounted wherever bitmaps are rebuilt */
+#define SGC_FULL_SWEEPS (1u << 3)        /* maintenance sweeps visit every vertex (no dirty tracking); same results */
 
 /* Engine configuration; the defaults (sgc_default_config) are the reference's. */
 typedef struct sgc_config {
@@ -98,9 +99,10 @@ typedef struct sgc_totals {
     uint64_t t1_sweeps;
     uint64_t t2_runs;
     uint64_t overflow_events;
-    uint64_t live_slots;         /* live row slots summed over compactions */
+    uint64_t live_slots;         /* live row slots summed over the rows compacted */
     uint64_t grouped;            /* parallel-batch updates applied in stream order by the wave owning their edge */
-    uint64_t reserved[4];
+    uint64_t sweep_rows[4];      /* rows visited by: compaction, bitmap/histogram rebuild, Tier-1, Tier-2 check */
+    uint64_t sweep_slots[4];     /* row slots read by those sweeps */
 } sgc_totals;
 
 typedef struct sgc_engine sgc_engine;

@@ -43,11 +43,13 @@ class StreamGCConfig:
     deterministic: bool = False
     repair_every_epoch: bool = False
     exact_histogram: bool = False
+    full_sweeps: bool = False
 
     def to_c(self):
         flags = (_lib.SGC_DETERMINISTIC if self.deterministic else 0) | \
                 (_lib.SGC_REPAIR_EVERY_EPOCH if self.repair_every_epoch else 0) | \
-                (_lib.SGC_EXACT_HISTOGRAM if self.exact_histogram else 0)
+                (_lib.SGC_EXACT_HISTOGRAM if self.exact_histogram else 0) | \
+                (_lib.SGC_FULL_SWEEPS if self.full_sweeps else 0)
         return _lib.Config(self.epoch_size, self.batch_size, self.histogram_threshold, self.slack_pct,
                            self.min_row_capacity, self.overflow_capacity, self.max_repair_iters,
                            self.plateau_iters, self.tier2_passes, flags)
@@ -296,7 +298,7 @@ class GPUPartition:
             records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
             self._ck(lib().sgc_apply(self.h, records.ctypes.data, len(records), 0))
 
-    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None):
+    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None, full_sweeps=None):
         """Switch the epoch procedure (and determinism) between phases."""
         if deterministic is not None:
             self.config.deterministic = bool(deterministic)
@@ -304,6 +306,8 @@ class GPUPartition:
             self.config.repair_every_epoch = bool(repair_every_epoch)
         if exact_histogram is not None:
             self.config.exact_histogram = bool(exact_histogram)
+        if full_sweeps is not None:
+            self.config.full_sweeps = bool(full_sweeps)
         self._ck(lib().sgc_set_flags(self.h, self.config.to_c().flags))
 
     def epoch(self):
@@ -434,9 +438,9 @@ class GraphPool:
     """
 
     def __init__(self, max_graphs=256, batch_capacity=1 << 18, hbm_budget=0, deterministic=False,
-                 repair_every_epoch=False, exact_histogram=False):
+                 repair_every_epoch=False, exact_histogram=False, full_sweeps=False):
         flags = StreamGCConfig(deterministic=deterministic, repair_every_epoch=repair_every_epoch,
-                               exact_histogram=exact_histogram).to_c().flags
+                               exact_histogram=exact_histogram, full_sweeps=full_sweeps).to_c().flags
         cfg = _lib.PoolConfig(max_graphs, batch_capacity, flags, 0, hbm_budget)
         h = C.c_void_p()
         check(lib().sgc_pool_create(C.byref(cfg), C.byref(h)))
@@ -481,16 +485,19 @@ class GraphPool:
         self._ck(lib().sgc_pool_usage(self.h, C.byref(u), C.byref(b), C.byref(k)))
         return u.value, b.value, k.value
 
-    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None):
+    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None, full_sweeps=None):
         c = StreamGCConfig(deterministic=bool(self.flags & _lib.SGC_DETERMINISTIC),
                            repair_every_epoch=bool(self.flags & _lib.SGC_REPAIR_EVERY_EPOCH),
-                           exact_histogram=bool(self.flags & _lib.SGC_EXACT_HISTOGRAM))
+                           exact_histogram=bool(self.flags & _lib.SGC_EXACT_HISTOGRAM),
+                           full_sweeps=bool(self.flags & _lib.SGC_FULL_SWEEPS))
         if deterministic is not None:
             c.deterministic = deterministic
         if repair_every_epoch is not None:
             c.repair_every_epoch = repair_every_epoch
         if exact_histogram is not None:
             c.exact_histogram = exact_histogram
+        if full_sweeps is not None:
+            c.full_sweeps = full_sweeps
         self.flags = c.to_c().flags
         self._ck(lib().sgc_pool_set_flags(self.h, self.flags))
 

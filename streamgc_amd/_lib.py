@@ -16,6 +16,7 @@ SGC_OK = 0
 SGC_DETERMINISTIC = 1 << 0
 SGC_REPAIR_EVERY_EPOCH = 1 << 1
 SGC_EXACT_HISTOGRAM = 1 << 2
+SGC_FULL_SWEEPS = 1 << 3
 
 # numpy view of the 16-byte update record (A/include/types.h:16-21)
 EDGE_UPDATE = np.dtype([("u", "<u4"), ("v", "<u4"), ("type", "u1"), ("pad", "u1", 3), ("graph", "<u4")])
@@ -42,10 +43,16 @@ class Totals(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "updates", "scan_slots", "hd_endpoints", "recolors", "recolor_cap_slots", "recolor_live",
         "epochs", "t1_sweeps", "t2_runs", "overflow_events", "live_slots", "grouped")] + \
-        [("reserved", C.c_uint64 * 4)]
+        [("sweep_rows", C.c_uint64 * 4), ("sweep_slots", C.c_uint64 * 4)]
+
+    SWEEPS = ("compact", "rebuild", "tier1", "t2_check")
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("sweep_")}
+        for i, name in enumerate(self.SWEEPS):
+            d[f"rows_{name}"] = self.sweep_rows[i]
+            d[f"slots_{name}"] = self.sweep_slots[i]
+        return d
 
 
 class PoolConfig(C.Structure):

@@ -40,10 +40,27 @@ enum : uint32_t {
     C_COUNT = 16
 };
 
+// Dirty marks.  Each maintenance sweep visits only the vertices marked for
+// it since it last ran, and clears their marks: compaction (M_ROW: the row
+// lost a slot), rebuild (M_BITS: the bitmap or histogram may be inexact), the
+// Tier-1 check (M_CHECK) and the Tier-2 check (M_T2: a colour in the closed
+// neighbourhood changed in a way that can change the vertex's verdict).  Each
+// kind is its own byte array (stride `dstride`), so a mark is a plain byte
+// store of 1 and concurrent marks never lose one another.  The endpoints of
+// an overflowed edge, whose two rows may disagree, are sticky (M_STICKY) and
+// visited by every sweep.
+enum : uint32_t { M_ROW = 0, M_BITS = 1, M_CHECK = 2, M_T2 = 3, M_STICKY = 4, M_ARRAYS = 5 };
+enum : uint32_t {
+    D_ROW = 1u << M_ROW, D_BITS = 1u << M_BITS, D_CHECK = 1u << M_CHECK, D_T2 = 1u << M_T2,
+    D_STATUS = D_CHECK | D_T2, D_ALL = D_ROW | D_BITS | D_CHECK | D_T2
+};
+
 // Launch-wide u64 totals (algorithmic counters).
 enum : uint32_t {
     T_UPDATES = 0, T_SCAN, T_HD, T_RECOLOR, T_RC_CAP, T_RC_LIVE, T_OVF_EVENTS, T_LIVE_SLOTS, T_GROUPED,
-    T_COUNT = 16
+    T_SWEEP_ROWS,                  // 4 counters: rows visited by compact, rebuild, tier-1, tier-2 check
+    T_SWEEP_SLOTS = T_SWEEP_ROWS + 4,  // 4 counters: row slots those sweeps read
+    T_COUNT = T_SWEEP_SLOTS + 4
 };
 
 struct GraphDev {
@@ -58,10 +75,11 @@ struct GraphDev {
     uint32_t* ovf;            // 2 * ovf_cap (vertex, neighbour) pairs
     uint32_t* ctr;            // C_COUNT counters
     uint16_t* snap_write;     // snapshot buffer the next publish writes
+    uint8_t* dirty;           // M_ARRAYS mark arrays of dstride bytes (null = no marking, full sweeps)
     uint32_t n;
     uint32_t ovf_cap;
     uint32_t nhd;
-    uint32_t pad;
+    uint32_t dstride;         // n rounded up to a multiple of 4
 };
 
 // ---------------------------------------------------------------- wave helpers
@@ -128,6 +146,22 @@ __device__ __forceinline__ void add64(uint64_t* p, uint64_t v) {
 }
 __device__ __forceinline__ void max32(uint32_t* p, uint32_t v) {
     __hip_atomic_fetch_max(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Mark v for the sweeps in `bits`.
+__device__ __forceinline__ void mark(const GraphDev& g, uint32_t v, uint32_t bits) {
+    if (!g.dirty) return;
+    SGC_G uint8_t* d = gp(g.dirty) + v;
+    const size_t st = g.dstride;
+    if (bits & D_ROW) d[M_ROW * st] = 1;
+    if (bits & D_BITS) d[M_BITS * st] = 1;
+    if (bits & D_CHECK) d[M_CHECK * st] = 1;
+    if (bits & D_T2) d[M_T2 * st] = 1;
+}
+__device__ __forceinline__ void mark_sticky(const GraphDev& g, uint32_t v) {
+    if (!g.dirty) return;
+    mark(g, v, D_ALL);
+    gp(g.dirty)[(size_t)M_STICKY * g.dstride + v] = 1;
 }
 
 __device__ __forceinline__ uint16_t color_of(uint32_t w) { return (uint16_t)(w & 0xFFFFu); }
@@ -226,14 +260,18 @@ __device__ __forceinline__ void on_removed_neighbor(const GraphDev& g, uint32_t 
     if (hd != INVALID && c <= HIST_BINS) add32(&g.hist[(size_t)hd * HIST_BINS + c - 1], 0xFFFFFFFFu);
 }
 // A/include/color_select.cuh:120-157 (per lane: each lane owns one neighbour)
-__device__ __forceinline__ void on_neighbor_recolor(const GraphDev& g, uint32_t nb, uint16_t oldc, uint16_t newc) {
+// Returns true when nb is a bitmap row (its old colour bit stays set, so the
+// bitmap may be inexact until a rebuild; a histogram stays exact).
+__device__ __forceinline__ bool on_neighbor_recolor(const GraphDev& g, uint32_t nb, uint16_t oldc, uint16_t newc) {
     if (newc != 0 && newc <= 64) or64(&g.bitmap[nb], 1ull << (newc - 1));
     uint32_t hd = gp(g.hd_map)[nb];
     if (hd != INVALID) {
         uint32_t* h = g.hist + (size_t)hd * HIST_BINS;
         if (oldc != 0 && oldc <= HIST_BINS) add32(&h[oldc - 1], 0xFFFFFFFFu);
         if (newc != 0 && newc <= HIST_BINS) add32(&h[newc - 1], 1u);
+        return false;
     }
+    return true;
 }
 
 // ---------------------------------------------------------------- slack-CSR rows
@@ -267,12 +305,31 @@ __device__ uint32_t row_insert(const GraphDev& g, uint32_t x, uint32_t nb, bool 
                 st(&g.ovf[2 * (size_t)pos], x);
                 st(&g.ovf[2 * (size_t)pos + 1], nb);
             }
+            // the two rows of this edge may now disagree: always maintain both
+            mark_sticky(g, x);
+            mark_sticky(g, nb);
         }
         if (acc) acc->ovf++;
     } else if (failed) {
         *failed = true;
     }
     return e - s;
+}
+
+// v changed colour oldc -> newc without telling its neighbours (first
+// colouring, Tier-1): every neighbour's bitmap or histogram may be stale, and
+// a neighbour whose colour is oldc, newc or none may have a new verdict.
+__device__ void mark_neighbors(const GraphDev& g, uint32_t v, uint16_t oldc, uint16_t newc) {
+    if (!g.dirty) return;
+    const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+    for (uint32_t base = s; base < e; base += WAVE) {
+        const uint32_t j = base + lane();
+        const uint32_t nb = j < e ? ld(&g.col[j]) : INVALID;
+        if (nb != INVALID && nb < g.n) {
+            const uint16_t c = color_of(ld(&g.word[nb]));
+            mark(g, nb, D_BITS | ((c == oldc || c == newc || c == 0) ? D_STATUS : 0u));
+        }
+    }
 }
 
 // A/src/kernels/local_update.cu:42-56: the first slot holding nb becomes free.
@@ -304,10 +361,15 @@ __device__ uint32_t propagate(const GraphDev& g, uint32_t x, uint16_t oldc, uint
         const uint32_t j = base + lane();
         const uint32_t nb = j < e ? ld(&g.col[j]) : INVALID;
         if (nb != INVALID && nb < g.n) {
-            on_neighbor_recolor(g, nb, oldc, newc);
+            const bool bitmap_row = on_neighbor_recolor(g, nb, oldc, newc);
+            if (g.dirty) {
+                const uint16_t c = color_of(ld(&g.word[nb]));
+                mark(g, nb, (bitmap_row ? D_BITS : 0u) | ((c == oldc || c == newc || c == 0) ? D_STATUS : 0u));
+            }
             live++;
         }
     }
+    if (lane() == 0) mark(g, x, D_STATUS);
     return wave_sum32(live);
 }
 
@@ -342,14 +404,22 @@ __device__ void try_reduce(const GraphDev& g, uint32_t x, uint16_t freed, WaveAc
 }
 
 // Colour an uncoloured endpoint: pack(c, 1) (A/src/kernels/local_update.cu:90-101).
+// The reference tells only the update's other endpoint about the new colour.
 __device__ __forceinline__ void color_fresh(const GraphDev& g, uint32_t x, uint32_t w) {
     const uint16_t c = find_color(g, x);
     if (lane() == 0) cas32(&g.word[x], w, (1u << 16) | c);
+    mark_neighbors(g, x, 0, c);
 }
 
 // One update, A/src/kernels/local_update.cu:71-146.
 __device__ void apply_update(const GraphDev& g, uint32_t u, uint32_t v, uint32_t type, WaveAcc& acc) {
     if (u >= g.n || v >= g.n) return;  // hash miss: skipped (:77)
+    if (type <= 1 && lane() == 0) {
+        // an insert fills the first free slot (the row stays packed); a delete leaves a hole
+        const uint32_t bits = type == 1 ? D_ALL : (D_BITS | D_STATUS);
+        mark(g, u, bits);
+        mark(g, v, bits);
+    }
     if (type == 0) {
         uint32_t p = row_insert(g, u, v, true, nullptr, &acc);
         p += row_insert(g, v, u, true, nullptr, &acc);

@@ -10,8 +10,12 @@
 //
 // Per-graph layout: word[n] u32, deg0[n] u32, row_ptr[n+1] u32, col[S] u32,
 // bitmap[n] u64, hd_map[n] u32, hubs[nhd] u32, hist[nhd*1024] u32,
-// overflow 2*cap u32, snapshot 2 x n u16; counters of all graphs in one
-// array (one copy reads them all).
+// overflow 2*cap u32, snapshot 2 x n u16, 5 mark arrays of n (rounded to 4) u8;
+// counters of all graphs in one array (one copy reads them all).
+//
+// Maintenance sweeps (compaction, rebuild, Tier-1, Tier-2 check) visit only
+// the vertices marked dirty since the sweep last ran: a worklist is built on
+// the device from the dirty bytes, and the sweep reads its length there.
 //
 // All work of a pool is enqueued on one HIP stream; the host synchronises only
 // for data-dependent decisions (Tier-1 convergence, the Tier-2 check).
@@ -53,9 +57,12 @@ struct PendingTiming {
     hipEvent_t a, b;
 };
 
+uint32_t mark_stride(uint32_t n) { return (n + 3u) & ~3u; }
+
 // Bytes one graph occupies on the device (admission control).
 uint64_t graph_bytes(uint32_t n, uint64_t S, uint32_t nhd, uint32_t ovf_cap) {
-    return 28ull * n + 4 + 4 * S + 4ull * nhd + 4096ull * nhd + 8ull * (ovf_cap ? ovf_cap : 1);
+    return 28ull * n + 4 + (uint64_t)M_ARRAYS * mark_stride(n) + 4 * S + 4ull * nhd + 4096ull * nhd +
+           8ull * (ovf_cap ? ovf_cap : 1);
 }
 
 // Slack capacities (A/src/streamgc.cu:373-378) and histogram rows
@@ -94,6 +101,7 @@ struct GraphState {
     uint32_t *hubs = nullptr, *hist = nullptr, *ovf = nullptr;
     uint64_t* bitmap = nullptr;
     uint16_t* snap = nullptr;  // 2 * n
+    uint8_t* dirty = nullptr;  // M_ARRAYS x mark_stride(n)
     int published = -1;
     uint64_t since_epoch = 0;
     uint32_t epochs = 0;
@@ -108,7 +116,7 @@ struct GraphState {
 
     void release() {
         for (void* p : {(void*)word, (void*)deg0, (void*)row_ptr, (void*)col, (void*)hd_map, (void*)hubs, (void*)hist,
-                        (void*)ovf, (void*)bitmap, (void*)snap, (void*)history})
+                        (void*)ovf, (void*)bitmap, (void*)snap, (void*)history, (void*)dirty})
             if (p) (void)hipFree(p);
         for (int i = 0; i < 2; i++) {
             if (pub_ev[i]) (void)hipEventDestroy(pub_ev[i]);
@@ -167,6 +175,9 @@ struct sgc_pool {
     uint64_t* h_totals = nullptr;
     EdgeGroups eg;
     sgc_edge_update* d_batch = nullptr;   // staging of host batches
+    uint32_t* d_list = nullptr;           // sweep worklist (virtual vertex ids)
+    uint32_t* d_list_count = nullptr;
+    uint32_t list_cap = 0;
     uint32_t ng = 0, V = 0, H = 0;        // slots in use, vertices, hubs
     uint16_t* d_tmp16 = nullptr;
     uint32_t* d_qids = nullptr;
@@ -184,6 +195,7 @@ struct sgc_pool {
     bool deterministic() const { return (flags & SGC_DETERMINISTIC) != 0; }
     bool exact_hist() const { return (flags & SGC_EXACT_HISTOGRAM) != 0; }
     bool repair_every_epoch() const { return (flags & SGC_REPAIR_EVERY_EPOCH) != 0; }
+    bool tracking() const { return (flags & SGC_FULL_SWEEPS) == 0; }
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -290,6 +302,40 @@ struct sgc_pool {
         e = vbase.upload(stream, 0, ng + 1);
         if (e == hipSuccess) e = hbase.upload(stream, 0, ng + 1);
         if (e == hipSuccess) e = gbase.upload(stream, 0, ng + 1);
+        if (e == hipSuccess && V > list_cap) {
+            e = hipStreamSynchronize(stream);
+            if (d_list) (void)hipFree(d_list);
+            d_list = nullptr;
+            list_cap = 0;
+            if (e == hipSuccess) e = hipMalloc((void**)&d_list, (size_t)V * 4);
+            if (e == hipSuccess) list_cap = V;
+        }
+        return e;
+    }
+
+    // The sweep context: the vertices of the active graphs marked for sweep
+    // `kind` (their marks are cleared) or sticky; every vertex without tracking.
+    hipError_t sweep(LaunchCtx& L, uint32_t kind) {
+        if (!tracking()) return hipSuccess;
+        L.list = d_list;
+        L.list_count = d_list_count;
+        return timed("build_list", [&] { return launch_build_list(L, kind, d_list, d_list_count); });
+    }
+    // All sweeps must visit every vertex of graph gid (its state changed from the host).
+    hipError_t mark_all(uint32_t gid) {
+        GraphState& G = gs[gid];
+        return hipMemsetAsync(G.dirty, 1, (size_t)M_STICKY * mark_stride(G.n), stream);
+    }
+    // Descriptors follow the tracking flag (dirty == null: no marking).
+    hipError_t set_tracking(bool on) {
+        hipError_t e = graphs.writable();
+        for (uint32_t g = 0; g < ng && e == hipSuccess; g++) {
+            if (!gs[g].used) continue;
+            graphs.h[g].dirty = on ? gs[g].dirty : nullptr;
+            if (on) e = mark_all(g);
+        }
+        if (e == hipSuccess && ng) e = graphs.upload(stream, 0, ng);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
         return e;
     }
 
@@ -303,9 +349,9 @@ struct sgc_pool {
     // ------------------------------------------------------------ hot path
     int apply(const sgc_edge_update* upd, uint32_t count, bool on_device, std::vector<uint64_t>& per_graph);
     int epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<sgc_epoch_stats>* out);
-    hipError_t rebuild(const LaunchCtx& L) {
-        hipError_t e = timed("rebuild", [&] { return launch_rebuild(L); });
-        if (e == hipSuccess && exact_hist() && H) e = timed("rebuild_hist", [&] { return launch_rebuild_hist(L); });
+    hipError_t rebuild(LaunchCtx L) {
+        hipError_t e = sweep(L, M_BITS);
+        if (e == hipSuccess) e = timed("rebuild", [&] { return launch_rebuild(L, exact_hist()); });
         return e;
     }
     int tier1(const std::vector<uint32_t>& gids);
@@ -367,6 +413,7 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     A((void**)&G.hist, (size_t)nhd * HIST_BINS * 4);
     A((void**)&G.ovf, (size_t)cfg->overflow_capacity * 8);
     A((void**)&G.snap, (size_t)n * 4);
+    A((void**)&G.dirty, (size_t)M_ARRAYS * mark_stride(n));
     for (int i = 0; i < 2 && e == hipSuccess; i++) {
         e = hipEventCreateWithFlags(&G.pub_ev[i], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&G.read_ev[i], hipEventDisableTiming);
@@ -375,6 +422,8 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     if (e == hipSuccess) e = hipMemsetAsync(G.bitmap, 0, (size_t)n * 8, s);
     if (e == hipSuccess && nhd) e = hipMemsetAsync(G.hist, 0, (size_t)nhd * HIST_BINS * 4, s);
     if (e == hipSuccess) e = hipMemsetAsync(G.snap, 0, (size_t)n * 4, s);
+    if (e == hipSuccess) e = hipMemsetAsync(G.dirty, 1, (size_t)M_STICKY * mark_stride(n), s);
+    if (e == hipSuccess) e = hipMemsetAsync(G.dirty + (size_t)M_STICKY * mark_stride(n), 0, mark_stride(n), s);
     if (e == hipSuccess) e = hipMemsetAsync(d_ctr + (size_t)gid * C_COUNT, 0, C_COUNT * 4, s);
     if (e == hipSuccess) e = hipMemcpyAsync(G.deg0, deg0, (size_t)n * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(G.row_ptr, rp.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, s);
@@ -408,6 +457,8 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     d.ovf = G.ovf;
     d.ctr = d_ctr + (size_t)gid * C_COUNT;
     d.snap_write = G.snap;
+    d.dirty = tracking() ? G.dirty : nullptr;
+    d.dstride = mark_stride(n);
     d.n = n;
     d.ovf_cap = cfg->overflow_capacity;
     d.nhd = nhd;
@@ -554,7 +605,9 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
         LaunchCtx L = ctx(true);
         if (e == hipSuccess) e = rebuild(L);
         if (e == hipSuccess) e = launch_zero_ctr(L, C_CONFLICTS);
-        if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L, deterministic()); });
+        LaunchCtx L1 = L;
+        if (e == hipSuccess && !deterministic()) e = sweep(L1, M_CHECK);
+        if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, deterministic()); });
         if (e == hipSuccess) e = read_ctr();
         if (e != hipSuccess) return hip_fail(e, "tier-1");
         t1_sweeps++;
@@ -582,9 +635,10 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
 // (elsewhere it would change nothing).
 int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
     hipError_t e = set_active(gids);
-    LaunchCtx L = ctx(true);
+    LaunchCtx L = ctx(true), Lc = L;
     if (e == hipSuccess) e = launch_zero_ctr(L, C_T2BAD);
-    if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(L); });
+    if (e == hipSuccess) e = sweep(Lc, M_T2);
+    if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(Lc); });
     if (e == hipSuccess) e = read_ctr();
     if (e != hipSuccess) return hip_fail(e, "tier-2 check");
     std::vector<uint32_t> ran;
@@ -603,6 +657,7 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
         G.st.t2_ran = 1;
         t2_runs++;
         e = hipMemcpyAsync(G.word, w.data(), (size_t)G.n * 4, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = mark_all(g);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return hip_fail(e, "tier-2 upload");
         last_ms[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -610,8 +665,10 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
     }
     if (ran.empty()) return SGC_OK;
     e = set_active(ran);
+    Lc = L;
     if (e == hipSuccess) e = launch_zero_ctr(L, C_T2BAD);
-    if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(L); });
+    if (e == hipSuccess) e = sweep(Lc, M_T2);
+    if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(Lc); });
     if (e == hipSuccess) e = read_ctr();
     if (e != hipSuccess) return hip_fail(e, "tier-2 check");
     for (uint32_t g : ran) gs[g].st.t2_remaining = ctr(g, C_T2BAD);
@@ -686,7 +743,9 @@ int sgc_pool::epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<
         if ((r = publish(gids))) return r;
         e = set_active(gids);
     }
-    if (e == hipSuccess) e = timed("compact", [&] { return launch_compact(L); });
+    LaunchCtx Lc = L;
+    if (e == hipSuccess) e = sweep(Lc, M_ROW);
+    if (e == hipSuccess) e = timed("compact", [&] { return launch_compact(Lc); });
     uint32_t ovf_cap = 0;
     for (uint32_t g : gids) ovf_cap = std::max(ovf_cap, gs[g].cfg.overflow_capacity);
     if (e == hipSuccess) e = timed("drain", [&] { return launch_drain(L, deterministic(), ovf_cap); });
@@ -725,7 +784,7 @@ void sgc_pool::release() {
     perm.release();
     seg.release();
     for (void* p : {(void*)d_ctr, (void*)d_totals, (void*)d_batch, (void*)d_tmp16, (void*)d_qids, (void*)d_qout,
-                    (void*)eg.keys, (void*)eg.head, (void*)eg.next})
+                    (void*)eg.keys, (void*)eg.head, (void*)eg.next, (void*)d_list, (void*)d_list_count})
         if (p) (void)hipFree(p);
     for (void* p : {(void*)h_ctr, (void*)h_totals})
         if (p) (void)hipHostFree(p);
@@ -777,6 +836,7 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipHostMalloc(&P->h_totals, T_COUNT * 8);
     if (e == hipSuccess) e = hipMalloc(&P->d_batch, (size_t)batch_cap * sizeof(sgc_edge_update));
     if (e == hipSuccess) e = hipMalloc(&P->d_tmp16, 64);
+    if (e == hipSuccess) e = hipMalloc(&P->d_list_count, 4);
     P->eg.size = 16;
     while (P->eg.size < 2u * batch_cap) P->eg.size <<= 1;
     P->eg.next_cap = batch_cap;
@@ -896,6 +956,16 @@ int upload(sgc_pool* P, void* dst, const void* src, size_t bytes) {
     return SGC_OK;
 }
 
+// State uploaded from the host: every sweep must visit every vertex again.
+int mark_all_sync(sgc_pool* P) {
+    hipError_t r = hipSuccess;
+    for (uint32_t g = 0; g < P->ng && r == hipSuccess; g++)
+        if (P->gs[g].used) r = P->mark_all(g);
+    if (r == hipSuccess) r = hipStreamSynchronize(P->stream);
+    if (r != hipSuccess) return P->hip_fail(r, "mark");
+    return SGC_OK;
+}
+
 GraphState* graph_of(sgc_pool* P, uint32_t gid) {
     if (!P || gid >= P->max_graphs || !P->gs[gid].used) return nullptr;
     return &P->gs[gid];
@@ -1001,9 +1071,14 @@ int sgc_destroy(sgc_engine* e) {
 }
 
 int sgc_set_flags(sgc_engine* e, uint32_t flags) {
-    if (!e || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM))) return SGC_ERR_INVALID;
+    if (!e || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))) return SGC_ERR_INVALID;
+    const bool was = e->tracking();
     e->flags = flags;
     e->gs[0].cfg.flags = flags;
+    if (e->tracking() != was) {
+        hipError_t r = e->set_tracking(e->tracking());
+        if (r != hipSuccess) return e->hip_fail(r, "set flags");
+    }
     return SGC_OK;
 }
 
@@ -1107,17 +1182,20 @@ int sgc_download_hist(sgc_engine* e, uint32_t* h) {
 }
 int sgc_upload_words(sgc_engine* e, const uint32_t* w) {
     if (!e || !w) return SGC_ERR_INVALID;
-    return upload(e, e->gs[0].word, w, (size_t)e->gs[0].n * 4);
+    int r = upload(e, e->gs[0].word, w, (size_t)e->gs[0].n * 4);
+    return r ? r : mark_all_sync(e);
 }
 int sgc_upload_bitmaps(sgc_engine* e, const uint64_t* b) {
     if (!e || !b) return SGC_ERR_INVALID;
-    return upload(e, e->gs[0].bitmap, b, (size_t)e->gs[0].n * 8);
+    int r = upload(e, e->gs[0].bitmap, b, (size_t)e->gs[0].n * 8);
+    return r ? r : mark_all_sync(e);
 }
 int sgc_upload_hist(sgc_engine* e, const uint32_t* h) {
     if (!e) return SGC_ERR_INVALID;
     if (e->gs[0].nhd == 0) return SGC_OK;
     if (!h) return SGC_ERR_INVALID;
-    return upload(e, e->gs[0].hist, h, (size_t)e->gs[0].nhd * HIST_BINS * 4);
+    int r = upload(e, e->gs[0].hist, h, (size_t)e->gs[0].nhd * HIST_BINS * 4);
+    return r ? r : mark_all_sync(e);
 }
 int sgc_overflow_count(sgc_engine* e, uint32_t* c) {
     if (!e || !c) return SGC_ERR_INVALID;
@@ -1137,6 +1215,10 @@ static int totals_impl(sgc_pool* P, sgc_totals* out) {
     out->overflow_events = P->h_totals[T_OVF_EVENTS];
     out->live_slots = P->h_totals[T_LIVE_SLOTS];
     out->grouped = P->h_totals[T_GROUPED];
+    for (int k = 0; k < 4; k++) {
+        out->sweep_rows[k] = P->h_totals[T_SWEEP_ROWS + k];
+        out->sweep_slots[k] = P->h_totals[T_SWEEP_SLOTS + k];
+    }
     for (auto& G : P->gs) out->epochs += G.epochs;
     out->t1_sweeps = P->t1_sweeps;
     out->t2_runs = P->t2_runs;
@@ -1306,7 +1388,7 @@ int sgc_kernel_times(sgc_engine* e, char* names, uint32_t stride, double* total_
 int sgc_pool_create(const sgc_pool_config* cfg, sgc_pool** out) {
     if (!cfg || !out || cfg->max_graphs == 0 || cfg->batch_capacity == 0)
         return global_fail(SGC_ERR_INVALID, "pool create: bad argument");
-    if (cfg->flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM))
+    if (cfg->flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))
         return global_fail(SGC_ERR_INVALID, "pool create: unknown flags");
     sgc_pool* P = new sgc_pool;
     int r = pool_init(P, cfg->max_graphs, cfg->batch_capacity, cfg->flags, cfg->hbm_budget);
@@ -1356,9 +1438,14 @@ int sgc_pool_usage(sgc_pool* p, uint64_t* used, uint64_t* budget, uint32_t* grap
     return SGC_OK;
 }
 int sgc_pool_set_flags(sgc_pool* p, uint32_t flags) {
-    if (!p || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM))) return SGC_ERR_INVALID;
+    if (!p || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))) return SGC_ERR_INVALID;
+    const bool was = p->tracking();
     p->flags = flags;
     for (auto& G : p->gs) G.cfg.flags = flags;
+    if (p->tracking() != was) {
+        hipError_t r = p->set_tracking(p->tracking());
+        if (r != hipSuccess) return p->hip_fail(r, "set flags");
+    }
     return SGC_OK;
 }
 int sgc_pool_apply(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, int on_device) {

@@ -33,6 +33,77 @@ __device__ __forceinline__ void locate(const uint32_t* __restrict__ vbase, uint3
     v = i - vbase[lo];
 }
 
+// ---------------------------------------------------------------- sweep sets
+// A maintenance sweep visits either every virtual vertex (list == null) or
+// the worklist a k_build_list launch left on the device (its length is read
+// by the sweep itself, so the host never waits for it).
+__device__ __forceinline__ uint32_t sweep_size(const uint32_t* list, const uint32_t* count, uint32_t V) {
+    return list ? *count : V;
+}
+__device__ __forceinline__ uint32_t sweep_item(const uint32_t* list, uint32_t k) { return list ? list[k] : k; }
+
+// Block-wide sum of per-wave values into one u64 total (one atomic per block).
+// Every thread of the block must call it.
+__device__ void block_add64(uint64_t* totals, uint32_t idx, uint64_t mine) {
+    __shared__ uint64_t part[16];
+    const uint32_t w = threadIdx.x / WAVE, nw = (blockDim.x + WAVE - 1) / WAVE;
+    if (lane() == 0) part[w] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0 && totals) {
+        uint64_t t = 0;
+        for (uint32_t k = 0; k < nw; k++) t += part[k];
+        if (t) add64(&totals[idx], t);
+    }
+    __syncthreads();
+}
+
+// Worklist of the vertices marked in array `kind` (or sticky); their marks
+// are cleared.  One lane per vertex; each wavefront covers LIST_TILE chunks of
+// 64 vertices and appends all its hits with one atomic on the list counter.
+constexpr uint32_t LIST_TILE = 16;
+__global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                    const uint32_t* __restrict__ vbase, uint32_t V,
+                                                    const uint32_t* __restrict__ active, uint32_t kind,
+                                                    uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    const uint64_t lt_mask = (1ull << lane()) - 1ull;
+    const uint32_t step = wave_count() * WAVE * LIST_TILE;
+    for (uint32_t base = wave_id() * WAVE * LIST_TILE; base < V; base += step) {
+        uint64_t masks[LIST_TILE];
+        uint32_t total = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < LIST_TILE; t++) {
+            const uint32_t i = base + t * WAVE + lane();
+            bool hit = false;
+            if (i < V) {
+                uint32_t gid, v;
+                locate(vbase, ng, i, gid, v);
+                if (!active || active[gid]) {
+                    const GraphDev& g = graphs[gid];
+                    SGC_G uint8_t* d = gp(g.dirty) + v;
+                    const size_t st = g.dstride;
+                    const bool marked = d[kind * st] != 0;
+                    hit = marked || d[M_STICKY * st] != 0;
+                    if (marked) d[kind * st] = 0;
+                }
+            }
+            masks[t] = ballot(hit);
+            total += (uint32_t)__builtin_popcountll(masks[t]);
+        }
+        if (total == 0) continue;
+        uint32_t at = 0;
+        if (lane() == 0) at = add32(count, total);
+        at = bcast0(at);
+#pragma unroll
+        for (uint32_t t = 0; t < LIST_TILE; t++) {
+            if ((masks[t] >> lane()) & 1ull)
+                list[at + (uint32_t)__builtin_popcountll(masks[t] & lt_mask)] = base + t * WAVE + lane();
+            at += (uint32_t)__builtin_popcountll(masks[t]);
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0; }
+
 // ---------------------------------------------------------------- per-edge ordering
 // Parallel waves apply a batch in no particular order, so two updates of one
 // edge in one batch (a delete and its re-insert, frequent in churn streams)
@@ -225,12 +296,15 @@ __global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ g
 // later chunk may refill it, in program order).
 __global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ graphs, uint32_t ng,
                                                  const uint32_t* __restrict__ vbase, uint32_t V,
-                                                 const uint32_t* __restrict__ active, uint64_t* totals) {
-    uint64_t live_total = 0;
+                                                 const uint32_t* __restrict__ active,
+                                                 const uint32_t* __restrict__ list, const uint32_t* count,
+                                                 uint64_t* totals) {
+    uint64_t live_total = 0, rows = 0, slots = 0;
     const uint64_t lt_mask = (1ull << lane()) - 1ull;
-    for (uint32_t i = wave_id(); i < V; i += wave_count()) {
+    const uint32_t total = sweep_size(list, count, V);
+    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
         uint32_t gid, v;
-        locate(vbase, ng, i, gid, v);
+        locate(vbase, ng, sweep_item(list, k), gid, v);
         if (active && !active[gid]) continue;
         const GraphDev& g = graphs[gid];
         const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
@@ -248,8 +322,12 @@ __global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ gr
             w = w_after;
         }
         live_total += w - s;
+        rows++;
+        slots += e - s;
     }
-    if (totals && lane() == 0 && live_total) add64(&totals[T_LIVE_SLOTS], live_total);
+    block_add64(totals, T_LIVE_SLOTS, live_total);
+    block_add64(totals, T_SWEEP_ROWS + 0, rows);
+    block_add64(totals, T_SWEEP_SLOTS + 0, slots);
 }
 
 // ---------------------------------------------------------------- overflow drain
@@ -275,7 +353,9 @@ __global__ __launch_bounds__(256) void k_drain_elem(const GraphDev* __restrict__
     uint32_t failed = 0;
     for (uint32_t k = wave_id(); k < count; k += wave_count()) {
         bool f = false;
-        row_insert(g, gp(g.ovf)[2 * (size_t)k], gp(g.ovf)[2 * (size_t)k + 1], false, &f, nullptr);
+        const uint32_t x = gp(g.ovf)[2 * (size_t)k];
+        row_insert(g, x, gp(g.ovf)[2 * (size_t)k + 1], false, &f, nullptr);
+        if (lane() == 0 && x < g.n) mark(g, x, D_BITS | D_STATUS);
         failed += f;
     }
     if (failed && lane() == 0) add32(&g.ctr[C_DRAIN_FAILED], failed);
@@ -291,60 +371,58 @@ __global__ __launch_bounds__(64) void k_drain_serial(const GraphDev* __restrict_
         uint32_t failed = 0;
         for (uint32_t k = 0; k < count; k++) {
             bool f = false;
-            row_insert(g, gp(g.ovf)[2 * (size_t)k], gp(g.ovf)[2 * (size_t)k + 1], false, &f, nullptr);
+            const uint32_t x = gp(g.ovf)[2 * (size_t)k];
+            row_insert(g, x, gp(g.ovf)[2 * (size_t)k + 1], false, &f, nullptr);
+            if (lane() == 0 && x < g.n) mark(g, x, D_BITS | D_STATUS);
             failed += f;
         }
         if (lane() == 0) gp(g.ctr)[C_DRAIN_FAILED] = failed;
     }
 }
 
-// ---------------------------------------------------------------- bitmap rebuild
+// ---------------------------------------------------------------- bitmap / histogram rebuild
 // A/include/color_select.cuh:213-251, A/src/kernels/epoch_maintenance.cu:113-128:
 // bitmap-mode rows get the exact OR of their live neighbours' colours <= 64.
+// Exact-histogram mode also recounts every hub's 1024 counters from its live
+// row (colours 1..1024): one wavefront per hub, the bins in that wavefront's
+// own 4 KiB of LDS, written out as 16 coalesced dwords per lane.  The
+// reference never rebuilds histograms (A/src/kernels/epoch_maintenance.cu:120-123),
+// so its counts drift; this mode keeps them exact at every rebuild.
 __global__ __launch_bounds__(256) void k_rebuild(const GraphDev* __restrict__ graphs, uint32_t ng,
                                                  const uint32_t* __restrict__ vbase, uint32_t V,
-                                                 const uint32_t* __restrict__ active) {
-    for (uint32_t i = wave_id(); i < V; i += wave_count()) {
-        uint32_t gid, v;
-        locate(vbase, ng, i, gid, v);
-        if (active && !active[gid]) continue;
-        const GraphDev& g = graphs[gid];
-        if (gp(g.hd_map)[v] != INVALID) continue;
-        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
-        uint64_t bm = 0;
-        for (uint32_t base = s; base < e; base += WAVE) {
-            const uint32_t j = base + lane();
-            const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
-            if (nb != INVALID && nb < g.n) {
-                const uint32_t c = color_of(gp(g.word)[nb]);
-                if (c != 0 && c <= 64) bm |= 1ull << (c - 1);
-            }
-        }
-        bm = wave_or64(bm);
-        if (lane() == 0) gp(g.bitmap)[v] = bm;
-    }
-}
-
-// ---------------------------------------------------------------- histogram rebuild in LDS
-// Exact-histogram mode: every hub's 1024 counters are recounted from its live
-// row (colours 1..1024), one wavefront per hub, the bins in that wavefront's
-// own 4 KiB of LDS, then written out as 16 coalesced dwords per lane.  The
-// reference never rebuilds histograms (A/src/kernels/epoch_maintenance.cu:120-123),
-// so counts drift; this mode keeps them exact at every rebuild.
-__global__ __launch_bounds__(256) void k_rebuild_hist(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                      const uint32_t* __restrict__ hbase, uint32_t H,
-                                                      const uint32_t* __restrict__ active) {
+                                                 const uint32_t* __restrict__ active,
+                                                 const uint32_t* __restrict__ list, const uint32_t* count,
+                                                 int exact_hist, uint64_t* totals) {
     __shared__ uint32_t bins[4][HIST_BINS];
     uint32_t* b = bins[threadIdx.x / WAVE];
-    for (uint32_t i = wave_id(); i < H; i += wave_count()) {
-        uint32_t gid, hd;
-        locate(hbase, ng, i, gid, hd);
+    uint64_t rows = 0, slots = 0;
+    const uint32_t total = sweep_size(list, count, V);
+    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
+        uint32_t gid, v;
+        locate(vbase, ng, sweep_item(list, k), gid, v);
         if (active && !active[gid]) continue;
         const GraphDev& g = graphs[gid];
-        const uint32_t v = gp(g.hubs)[hd];
-        for (uint32_t k = lane(); k < HIST_BINS; k += WAVE) b[k] = 0;
-        __builtin_amdgcn_wave_barrier();
+        const uint32_t hd = gp(g.hd_map)[v];
+        if (hd != INVALID && !exact_hist) continue;
         const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+        rows++;
+        slots += e - s;
+        if (hd == INVALID) {
+            uint64_t bm = 0;
+            for (uint32_t base = s; base < e; base += WAVE) {
+                const uint32_t j = base + lane();
+                const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
+                if (nb != INVALID && nb < g.n) {
+                    const uint32_t c = color_of(gp(g.word)[nb]);
+                    if (c != 0 && c <= 64) bm |= 1ull << (c - 1);
+                }
+            }
+            bm = wave_or64(bm);
+            if (lane() == 0) gp(g.bitmap)[v] = bm;
+            continue;
+        }
+        for (uint32_t t = lane(); t < HIST_BINS; t += WAVE) b[t] = 0;
+        __builtin_amdgcn_wave_barrier();
         for (uint32_t base = s; base < e; base += WAVE) {
             const uint32_t j = base + lane();
             const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
@@ -355,9 +433,11 @@ __global__ __launch_bounds__(256) void k_rebuild_hist(const GraphDev* __restrict
         }
         __builtin_amdgcn_wave_barrier();
         SGC_G uint32_t* dst = gp(g.hist) + (size_t)hd * HIST_BINS;
-        for (uint32_t k = lane(); k < HIST_BINS; k += WAVE) dst[k] = b[k];
+        for (uint32_t t = lane(); t < HIST_BINS; t += WAVE) dst[t] = b[t];
         __builtin_amdgcn_wave_barrier();
     }
+    block_add64(totals, T_SWEEP_ROWS + 1, rows);
+    block_add64(totals, T_SWEEP_SLOTS + 1, slots);
 }
 
 // ---------------------------------------------------------------- tier-1 repair
@@ -374,7 +454,11 @@ __device__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) {
         }
         if (!has_nb) return 0;
         const uint16_t c = find_color(g, v);
-        if (lane() == 0) st(&g.word[v], bump(w, c));
+        if (lane() == 0) {
+            st(&g.word[v], bump(w, c));
+            mark(g, v, D_BITS | D_STATUS);
+        }
+        mark_neighbors(g, v, 0, c);
         return 1;
     }
     const uint32_t my_deg = gp(g.deg0)[v];
@@ -392,49 +476,100 @@ __device__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) {
     if (!need) return 0;
     if (mine <= 64 && lane() == 0) or64(&g.bitmap[v], 1ull << (mine - 1));
     const uint16_t c = find_color(g, v);
-    if (lane() == 0) st(&g.word[v], bump(w, c));
+    if (lane() == 0) {
+        st(&g.word[v], bump(w, c));
+        mark(g, v, D_BITS | D_STATUS);   // its own colour bit was OR-ed in
+    }
+    mark_neighbors(g, v, mine, c);
     return 1;
 }
 
 __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__ graphs, uint32_t ng,
                                                     const uint32_t* __restrict__ vbase, uint32_t V,
-                                                    const uint32_t* __restrict__ active) {
+                                                    const uint32_t* __restrict__ active,
+                                                    const uint32_t* __restrict__ list, const uint32_t* count,
+                                                    uint64_t* totals) {
     uint32_t cnt = 0, cur_gid = 0xFFFFFFFFu;
-    for (uint32_t i = wave_id(); i < V; i += wave_count()) {
+    uint64_t rows = 0, slots = 0;
+    const uint32_t total = sweep_size(list, count, V);
+    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
         uint32_t gid, v;
-        locate(vbase, ng, i, gid, v);
+        locate(vbase, ng, sweep_item(list, k), gid, v);
         if (active && !active[gid]) continue;
         if (gid != cur_gid) {
             if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
             cnt = 0;
             cur_gid = gid;
         }
-        cnt += tier1_vertex(graphs[gid], v);
+        const GraphDev& g = graphs[gid];
+        rows++;
+        slots += gp(g.row_ptr)[v + 1] - gp(g.row_ptr)[v];
+        cnt += tier1_vertex(g, v);
     }
     if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
+    block_add64(totals, T_SWEEP_ROWS + 2, rows);
+    block_add64(totals, T_SWEEP_SLOTS + 2, slots);
 }
 
 // Vertex order on one wavefront per graph (Gauss-Seidel, deterministic).
+// With dirty tracking, a vertex is checked only if its check bit is set (or
+// it is sticky) when the sweep reaches it; the bit is read again after every
+// check, since a recolour marks later vertices of the same 64-vertex chunk.
+// A vertex whose bit is clear would be found conflict-free, so the sweep
+// changes exactly what the full sweep changes.
+__device__ __forceinline__ uint32_t mark_byte(const GraphDev& g, uint32_t kind, uint32_t v) {
+    const uint32_t w = ld((const uint32_t*)(g.dirty + (size_t)kind * g.dstride + (v & ~3u)));
+    return (w >> (8 * (v & 3u))) & 0xFFu;
+}
+
 __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                     const uint32_t* __restrict__ active) {
+                                                     const uint32_t* __restrict__ active, uint64_t* totals) {
+    uint64_t rows = 0;
     for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
         if (active && !active[gid]) continue;
         const GraphDev g = graphs[gid];
         uint32_t cnt = 0;
-        for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex(g, v);
+        if (!g.dirty) {
+            for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex(g, v);
+            rows += g.n;
+        } else {
+            for (uint32_t base = 0; base < g.n; base += WAVE) {
+                const uint32_t v = base + lane();
+                const bool sticky = v < g.n && gp(g.dirty)[(size_t)M_STICKY * g.dstride + v];
+                uint32_t from = 0;
+                for (;;) {
+                    const bool hit = lane() >= from && v < g.n && (sticky || mark_byte(g, M_CHECK, v));
+                    const uint64_t m = ballot(hit);
+                    if (!m) break;
+                    const uint32_t k = (uint32_t)__builtin_ctzll(m);
+                    const uint32_t u = base + k;
+                    if (lane() == k) gp(g.dirty)[(size_t)M_CHECK * g.dstride + u] = 0;
+                    cnt += tier1_vertex(g, u);
+                    rows++;
+                    from = k + 1;
+                }
+            }
+        }
         if (lane() == 0) add32(&g.ctr[C_CONFLICTS], cnt);
     }
+    if (lane() == 0 && totals && rows) add64(&totals[T_SWEEP_ROWS + 2], rows);
 }
 
 // ---------------------------------------------------------------- checks
 // Tier-2 has work iff some row slot joins an uncoloured endpoint or two
 // endpoints of one colour (A/src/streamgc.cu:1190-1218 then change nothing).
+// A vertex found with a bad slot keeps its check bit, so the next check
+// counts it again until a repair changes it.
 __global__ __launch_bounds__(256) void k_t2_check(const GraphDev* __restrict__ graphs, uint32_t ng,
                                                   const uint32_t* __restrict__ vbase, uint32_t V,
-                                                  const uint32_t* __restrict__ active) {
-    for (uint32_t i = wave_id(); i < V; i += wave_count()) {
+                                                  const uint32_t* __restrict__ active,
+                                                  const uint32_t* __restrict__ list, const uint32_t* count,
+                                                  uint64_t* totals) {
+    uint64_t rows = 0, slots = 0;
+    const uint32_t total = sweep_size(list, count, V);
+    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
         uint32_t gid, v;
-        locate(vbase, ng, i, gid, v);
+        locate(vbase, ng, sweep_item(list, k), gid, v);
         if (active && !active[gid]) continue;
         const GraphDev& g = graphs[gid];
         const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
@@ -449,8 +584,15 @@ __global__ __launch_bounds__(256) void k_t2_check(const GraphDev* __restrict__ g
             }
         }
         bad = wave_sum32(bad);
-        if (bad && lane() == 0) add32(&g.ctr[C_T2BAD], bad);
+        rows++;
+        slots += e - s;
+        if (bad && lane() == 0) {
+            add32(&g.ctr[C_T2BAD], bad);
+            if (g.dirty) gp(g.dirty)[(size_t)M_T2 * g.dstride + v] = 1;
+        }
     }
+    block_add64(totals, T_SWEEP_ROWS + 3, rows);
+    block_add64(totals, T_SWEEP_SLOTS + 3, slots);
 }
 
 // check_coloring_valid on the engine's rows (A/src/streamgc.cu:448-556);
@@ -584,9 +726,21 @@ hipError_t launch_snapshot(const LaunchCtx& L) {
     k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
     return hipGetLastError();
 }
+// Sweep grids: one wavefront per row, grid-stride; a worklist sweep takes the
+// grid of a full sweep capped at 8192 wavefronts (its length is on the device).
+static inline uint32_t sweep_grid(const LaunchCtx& L) {
+    return grid_for_waves(L.V, 4, L.list ? 2048 : L.max_blocks);
+}
+hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t* count) {
+    if (kind >= M_STICKY) return hipErrorInvalidValue;
+    k_zero_u32<<<1, 1, 0, L.stream>>>(count);
+    uint32_t grid = grid_for_waves((L.V + WAVE * LIST_TILE - 1) / (WAVE * LIST_TILE), 4, 4096);
+    k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, kind, list, count);
+    return hipGetLastError();
+}
 hipError_t launch_compact(const LaunchCtx& L) {
-    uint32_t grid = grid_for_waves(L.V, 4, L.max_blocks);
-    k_compact<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.totals);
+    k_compact<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.list, L.list_count,
+                                                   L.totals);
     return hipGetLastError();
 }
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
@@ -599,29 +753,23 @@ hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
     }
     return hipGetLastError();
 }
-hipError_t launch_rebuild(const LaunchCtx& L) {
-    uint32_t grid = grid_for_waves(L.V, 4, L.max_blocks);
-    k_rebuild<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
-    return hipGetLastError();
-}
-hipError_t launch_rebuild_hist(const LaunchCtx& L) {
-    if (L.H == 0) return hipSuccess;
-    uint32_t grid = grid_for_waves(L.H, 4, L.max_blocks);
-    k_rebuild_hist<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.hbase, L.H, L.active);
+hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist) {
+    k_rebuild<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.list, L.list_count,
+                                                   exact_hist ? 1 : 0, L.totals);
     return hipGetLastError();
 }
 hipError_t launch_tier1(const LaunchCtx& L, bool serial) {
     if (serial) {
-        k_tier1_serial<<<L.ng < 1024 ? L.ng : 1024, 64, 0, L.stream>>>(L.graphs, L.ng, L.active);
+        k_tier1_serial<<<L.ng < 1024 ? L.ng : 1024, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, L.totals);
     } else {
-        uint32_t grid = grid_for_waves(L.V, 4, L.max_blocks);
-        k_tier1_elem<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
+        k_tier1_elem<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.list,
+                                                          L.list_count, L.totals);
     }
     return hipGetLastError();
 }
 hipError_t launch_t2_check(const LaunchCtx& L) {
-    uint32_t grid = grid_for_waves(L.V, 4, L.max_blocks);
-    k_t2_check<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
+    k_t2_check<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.list, L.list_count,
+                                                    L.totals);
     return hipGetLastError();
 }
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors) {

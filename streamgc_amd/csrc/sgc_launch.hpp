@@ -21,6 +21,8 @@ struct LaunchCtx {
     uint32_t H = 0;                    // total hubs
     uint64_t* totals = nullptr;        // device u64 counters
     uint32_t max_blocks = 16384;
+    const uint32_t* list = nullptr;    // device worklist of virtual vertices (null = sweep all)
+    const uint32_t* list_count = nullptr;
 };
 
 // Per-edge grouping of a parallel batch: hash table (size: power of two >=
@@ -41,8 +43,8 @@ hipError_t launch_max_color(const LaunchCtx& L);
 hipError_t launch_snapshot(const LaunchCtx& L);
 hipError_t launch_compact(const LaunchCtx& L);
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
-hipError_t launch_rebuild(const LaunchCtx& L);
-hipError_t launch_rebuild_hist(const LaunchCtx& L);
+hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t* count);
+hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist);
 hipError_t launch_tier1(const LaunchCtx& L, bool serial);
 hipError_t launch_t2_check(const LaunchCtx& L);
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors);

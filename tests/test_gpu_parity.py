@@ -96,11 +96,16 @@ def test_deterministic_histogram_path_bit_exact(threshold, exact, graph_path):
         _same_state(gpu, orc, f"{name} threshold {threshold}")
 
 
+@pytest.mark.parametrize("sweeps", ["tracked", "full"])
+@pytest.mark.parametrize("repair", [False, True])
 @pytest.mark.parametrize("slack,min_cap,cap", [(0, 1, 8), (0, 2, 4096), (50, 2, 64)])
-def test_deterministic_overflow_and_tier2_bit_exact(slack, min_cap, cap, graph_path):
+def test_deterministic_overflow_and_tier2_bit_exact(slack, min_cap, cap, repair, sweeps, graph_path):
+    # overflowed edges leave the two rows disagreeing (sticky endpoints of the
+    # dirty tracking); both sweep modes must match the oracle's full sweeps
     g = S.load_graph_csr(graph_path("er2k"))
     gpu, orc, og = _pair(g, deterministic=True, epoch_size=128, batch_size=128, slack_pct=slack,
-                         min_row_capacity=min_cap, overflow_capacity=cap)
+                         min_row_capacity=min_cap, overflow_capacity=cap, repair_every_epoch=repair,
+                         full_sweeps=(sweeps == "full"))
     init = S.generate_init_stream(g)
     assert gpu.process_stream(init) == orc.submit(init["u"], init["v"], init["type"])
     assert gpu.finish() == orc.finish()
@@ -270,3 +275,43 @@ def test_device_resident_stream_equals_host_stream(graph_path):
     b.process_stream(ds)
     assert np.array_equal(a.words(), b.words())
     assert np.array_equal(a.csr()[1], b.csr()[1])
+
+
+@pytest.mark.parametrize("name", ["delaunay1k", "rmat_s10", "chunglu4k"])
+def test_switching_sweep_mode_mid_stream_stays_bit_exact(name, graph_path):
+    g = S.load_graph_csr(graph_path(name))
+    gpu, orc, og = _pair(g, deterministic=True, epoch_size=100, batch_size=100, repair_every_epoch=True,
+                         exact_histogram=True)
+    init = S.generate_init_stream(g)
+    assert gpu.process_stream(init) == orc.submit(init["u"], init["v"], init["type"])
+    assert gpu.finish() == orc.finish()
+    bench = S.generate_bench_stream_edges(g, 2400, 60, 11)
+    parts = np.array_split(np.arange(len(bench)), 4)
+    for k, idx in enumerate(parts):
+        gpu.set_mode(full_sweeps=(k % 2 == 1))
+        chunk = bench[idx]
+        assert gpu.process_stream(chunk) == orc.submit(chunk["u"], chunk["v"], chunk["type"])
+        _same_state(gpu, orc, f"{name} part {k}")
+    assert gpu.finish() == orc.finish()
+    assert np.array_equal(gpu.snapshot(), orc.snapshot())
+
+
+def test_tracked_sweeps_visit_fewer_rows(graph_path):
+    g = S.load_graph_csr(graph_path("delaunay1k"))
+    rows = {}
+    for mode in ("tracked", "full"):
+        p = S.GPUPartition(g, S.StreamGCConfig(epoch_size=64, batch_size=64, repair_every_epoch=True,
+                                                full_sweeps=(mode == "full")))
+        init = S.generate_init_stream(g)
+        p.process_stream(init)
+        p.finish()
+        before = p.totals()
+        p.process_stream(S.generate_bench_stream_edges(g, 640, 40, 3))
+        after = p.totals()
+        rows[mode] = {k: after[k] - before[k] for k in after if k.startswith("rows_")}
+        violations, uncoloured, _ = p.validate()
+        assert violations == 0 and uncoloured == 0
+    full, tracked = rows["full"], rows["tracked"]
+    assert full["rows_t2_check"] >= 10 * g.n
+    assert tracked["rows_t2_check"] < full["rows_t2_check"] / 2
+    assert tracked["rows_compact"] < full["rows_compact"] / 2
