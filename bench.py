@@ -5,8 +5,10 @@ Workload (default "c2"): Chung-Lu graph of com-LiveJournal shape (3,997,962
 vertices, 34,681,189 edges, degree exponent 2.3, max degree ~15K, seed 42,
 random vertex permutation), churn stream at 50% insert (seed 42), epoch
 E = 32,768, histogram threshold 64, plain per-update GPU engine.  The init
-stream and its repair run first, in the reference's epoch order; the
-measured phase repairs and publishes a proper snapshot every epoch.
+stream and its repair run first, untimed, in the reference's epoch order
+(launches of --init-batch updates: fewer concurrent first colourings give a
+colouring as tight as the sequential replay's); the measured phase repairs and
+publishes a proper snapshot every epoch.
 
 A step is one batch of E updates: the per-update kernel over the batch, then
 the epoch (compact, drain, Tier-1, Tier-2 check, publish).  Inputs are
@@ -108,6 +110,11 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--full-sweeps", action="store_true",
                     help="maintenance sweeps visit every vertex instead of the dirty worklists")
+    ap.add_argument("--long-row", type=int, default=None, help="tuning: worklist rows longer than this go to a block")
+    ap.add_argument("--sweep-blocks", type=int, default=None, help="tuning: grid cap of worklist sweeps")
+    ap.add_argument("--serial-repair", type=int, default=None, help="tuning: 1 = Tier-1 in vertex order")
+    ap.add_argument("--init-batch", type=int, default=256,
+                    help="apply the (untimed) init stream in launches of this many updates, epochs still every E")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this "
@@ -131,9 +138,20 @@ def main():
 
     eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True,
                                              full_sweeps=args.full_sweeps))
+    knobs = {k: v for k, v in (("long_row", args.long_row), ("sweep_blocks", args.sweep_blocks),
+                               ("serial_repair", args.serial_repair)) if v is not None}
+    if knobs:
+        eng.set_tuning(**knobs)
     t = time.perf_counter()
     ds_init = S.DeviceStream(init)
-    eng.process_stream(ds_init, max_epochs=0)
+    if args.init_batch and args.init_batch < E:
+        for k in range(0, len(init), E):
+            for j in range(k, min(k + E, len(init)), args.init_batch):
+                eng.apply_device(ds_init.at(j), min(args.init_batch, len(init) - j))
+            if k + E <= len(init):
+                eng.epoch()
+    else:
+        eng.process_stream(ds_init, max_epochs=0)
     fin = eng.finish()
     ds_init.close()
     log(f"init phase: {time.perf_counter() - t:.1f}s, max colour {fin['max_color']}, t2_remaining {fin['t2_remaining']}")
@@ -210,6 +228,8 @@ def main():
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": w["name"], "vertices": g.n, "edges": g.num_edges // 2, "epoch_size": E,
                    "updates_timed": K * E,
+                   "init": f"init stream ({len(init)} inserts, untimed) applied in launches of "
+                           f"{min(args.init_batch or E, E)} updates, reference-order epochs every {E}, then repaired",
                    "maintenance": "full sweeps" if args.full_sweeps else "dirty worklists",
                    "epoch": "compact, drain, tier-1, "
                    "tier-2 check (host tier-2 when needed), publish"},
@@ -220,6 +240,9 @@ def main():
         "parity": parity,
         "kernels": per_kernel,
         "sweeps": {k: int(v) for k, v in d.items() if k.startswith(("rows_", "slots_"))},
+        "update_path": {k: int(d[k]) for k in ("updates", "scan_slots", "hd_endpoints", "recolors",
+                                               "recolor_cap_slots", "recolor_live", "grouped")},
+        "tuning": knobs,
     }
     print(json.dumps(out))
 

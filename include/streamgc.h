@@ -93,7 +93,7 @@ typedef struct sgc_totals {
     uint64_t scan_slots;         /* slots scanned by inserts and deletes (p_u + p_v) */
     uint64_t hd_endpoints;       /* endpoints in histogram mode */
     uint64_t recolors;           /* vertex recolours during updates */
-    uint64_t recolor_cap_slots;  /* row capacity scanned by those recolours */
+    uint64_t recolor_cap_slots;  /* row slots scanned by those recolours (the filled prefix of the row) */
     uint64_t recolor_live;       /* live neighbours updated by those recolours */
     uint64_t epochs;
     uint64_t t1_sweeps;
@@ -108,6 +108,16 @@ typedef struct sgc_totals {
 typedef struct sgc_engine sgc_engine;
 
 void sgc_default_config(sgc_config* cfg);
+
+/* Launch-shape knobs (no reference counterpart: the autotuner's search space).
+ * They change only how work is spread over the GPU, never a result. */
+typedef struct sgc_tuning {
+    uint32_t long_row;      /* worklist rows with more filled slots are swept by a 256-thread block (default 1024) */
+    uint32_t sweep_blocks;  /* grid cap of worklist sweeps, in 256-thread blocks (default 2048) */
+    uint32_t apply_blocks;  /* grid cap of the per-update launch and of full sweeps (default 16384) */
+    uint32_t serial_repair; /* 1: Tier-1 sweeps in vertex order on one wavefront per graph even in parallel mode */
+} sgc_tuning;
+void sgc_default_tuning(sgc_tuning* t);
 const char* sgc_last_error(const sgc_engine* e); /* NULL engine: last global error */
 const char* sgc_version(void);
 
@@ -142,6 +152,7 @@ int sgc_finish(sgc_engine* e, sgc_epoch_stats* out);
  * during the init stream, repair every epoch during the measured stream).
  * Any SGC_* flag may change; the layout does not. */
 int sgc_set_flags(sgc_engine* e, uint32_t flags);
+int sgc_set_tuning(sgc_engine* e, const sgc_tuning* t);
 
 /* = point_query (A/src/host/query_server.cpp:23-28) over the published snapshot. */
 int sgc_query(sgc_engine* e, const uint32_t* ids, uint32_t count, uint16_t* out_colors);
@@ -226,6 +237,7 @@ int sgc_pool_add(sgc_pool* p, const sgc_config* cfg, uint32_t n, const uint32_t*
 int sgc_pool_remove(sgc_pool* p, uint32_t gid);
 int sgc_pool_usage(sgc_pool* p, uint64_t* used_bytes, uint64_t* budget, uint32_t* graphs);
 int sgc_pool_set_flags(sgc_pool* p, uint32_t flags);
+int sgc_pool_set_tuning(sgc_pool* p, const sgc_tuning* t);
 /* One launch per batch of up to batch_capacity records, no epoch step. */
 int sgc_pool_apply(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, int on_device);
 /* Graphs whose updates since their last epoch reached their epoch_size. */

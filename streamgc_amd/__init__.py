@@ -55,6 +55,17 @@ class StreamGCConfig:
                            self.plateau_iters, self.tier2_passes, flags)
 
 
+def make_tuning(**knobs):
+    """sgc_tuning from the defaults with `knobs` overridden."""
+    t = _lib.Tuning()
+    lib().sgc_default_tuning(C.byref(t))
+    for k, v in knobs.items():
+        if k not in ("long_row", "sweep_blocks", "apply_blocks", "serial_repair"):
+            raise TypeError(f"unknown tuning knob {k}")
+        setattr(t, k, int(v))
+    return t
+
+
 def default_config(**kw):
     return StreamGCConfig(**kw)
 
@@ -298,6 +309,14 @@ class GPUPartition:
             records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
             self._ck(lib().sgc_apply(self.h, records.ctypes.data, len(records), 0))
 
+    def apply_device(self, ptr, count):
+        """One batch already resident on the device (no epoch step)."""
+        self._ck(lib().sgc_apply(self.h, ptr, count, 1))
+
+    def set_tuning(self, **knobs):
+        """Launch-shape knobs (long_row, sweep_blocks, apply_blocks); results never change."""
+        self._ck(lib().sgc_set_tuning(self.h, C.byref(make_tuning(**knobs))))
+
     def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None, full_sweeps=None):
         """Switch the epoch procedure (and determinism) between phases."""
         if deterministic is not None:
@@ -484,6 +503,9 @@ class GraphPool:
         u, b, k = C.c_uint64(), C.c_uint64(), C.c_uint32()
         self._ck(lib().sgc_pool_usage(self.h, C.byref(u), C.byref(b), C.byref(k)))
         return u.value, b.value, k.value
+
+    def set_tuning(self, **knobs):
+        self._ck(lib().sgc_pool_set_tuning(self.h, C.byref(make_tuning(**knobs))))
 
     def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None, full_sweeps=None):
         c = StreamGCConfig(deterministic=bool(self.flags & _lib.SGC_DETERMINISTIC),

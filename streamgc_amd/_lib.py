@@ -55,6 +55,11 @@ class Totals(C.Structure):
         return d
 
 
+class Tuning(C.Structure):
+    _fields_ = [("long_row", C.c_uint32), ("sweep_blocks", C.c_uint32), ("apply_blocks", C.c_uint32),
+                ("serial_repair", C.c_uint32)]
+
+
 class PoolConfig(C.Structure):
     _fields_ = [("max_graphs", C.c_uint32), ("batch_capacity", C.c_uint32), ("flags", C.c_uint32),
                 ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64)]
@@ -69,6 +74,9 @@ _vp = C.c_void_p
 # name -> (argtypes); every function returns int unless listed in _RESTYPE
 SIGNATURES = {
     "sgc_default_config": [C.POINTER(Config)],
+    "sgc_default_tuning": [C.POINTER(Tuning)],
+    "sgc_set_tuning": [_vp, C.POINTER(Tuning)],
+    "sgc_pool_set_tuning": [_vp, C.POINTER(Tuning)],
     "sgc_last_error": [_vp],
     "sgc_version": [],
     "sgc_device_count": [],
@@ -145,7 +153,7 @@ SIGNATURES = {
     "sgc_static_greedy": [C.c_uint32, _u32p, _u32p, C.POINTER(C.c_uint32)],
     "sgc_edge_intervals": [C.c_uint32, _u32p, _u32p, _vp, C.c_uint64, C.POINTER(C.c_uint64), _vp, _vp, _vp, _vp],
 }
-_RESTYPE = {"sgc_default_config": None, "sgc_last_error": C.c_char_p, "sgc_version": C.c_char_p,
+_RESTYPE = {"sgc_default_config": None, "sgc_default_tuning": None, "sgc_last_error": C.c_char_p, "sgc_version": C.c_char_p,
             "sgc_pool_last_error": C.c_char_p}
 
 _lib = None

@@ -24,6 +24,9 @@ namespace sgc {
 constexpr uint32_t INVALID = 0xFFFFFFFFu;   // empty slot / no histogram row (A/include/types.h:24-25)
 constexpr uint32_t HIST_BINS = 1024;        // A/include/partition.cuh:6-9
 constexpr uint32_t WAVE = 64;
+// Row scans issue ROW_UNROLL 64-slot loads before consuming any of them, so a
+// long row costs ROW_UNROLL times fewer memory round trips.
+constexpr uint32_t ROW_UNROLL = 4;
 constexpr int MAX_CAS_RETRIES = 64;          // recolour CAS retries before leaving it to repair
 
 // Per-graph u32 counters (device).
@@ -37,6 +40,7 @@ enum : uint32_t {
     C_VIOL,             // validator: conflicting slots
     C_UNCOL,            // validator: uncoloured vertices with a live slot
     C_MAXC,             // validator: max colour
+    C_ASYM,             // overflow events ever (edges whose two rows may disagree); never reset
     C_COUNT = 16
 };
 
@@ -68,6 +72,7 @@ struct GraphDev {
     const uint32_t* deg0;     // n loaded-graph degrees (priority)
     const uint32_t* row_ptr;  // n + 1 slot offsets
     uint32_t* col;            // S slots, INVALID = free
+    uint32_t* fill;           // n: slots of the row that may hold a neighbour (the rest are INVALID)
     uint64_t* bitmap;         // n neighbour-colour bitmaps (bit c-1 = colour c)
     const uint32_t* hd_map;   // n histogram row or INVALID
     const uint32_t* hubs;     // nhd vertices in histogram mode (inverse of hd_map)
@@ -178,7 +183,7 @@ __device__ __forceinline__ bool pi_less(const GraphDev& g, uint32_t a, uint32_t 
 // Per-wave counters, flushed once per wave.
 struct WaveAcc {
     uint64_t updates = 0, scan = 0, hd = 0, recolor = 0, rc_cap = 0, rc_live = 0, ovf = 0;
-    __device__ void flush(uint64_t* totals) {
+    __device__ __forceinline__ void flush(uint64_t* totals) {
         if (!totals || lane() != 0) return;
         if (updates) add64(&totals[T_UPDATES], updates);
         if (scan) add64(&totals[T_SCAN], scan);
@@ -194,23 +199,31 @@ struct WaveAcc {
 // A/include/color_select.cuh:10-53.  Bitmap: lowest clear bit.  All 64 set:
 // lane-parallel gather of row colours 1..255 into a 256-bit set, first free,
 // else 256.
-__device__ uint16_t fas_bitmap(const GraphDev& g, uint32_t x) {
+__device__ __forceinline__ uint16_t fas_bitmap(const GraphDev& g, uint32_t x) {
     uint64_t freeb = ~ld(&g.bitmap[x]);
     if (freeb) return (uint16_t)(__builtin_ctzll(freeb) + 1);
-    const uint32_t s = gp(g.row_ptr)[x], e = gp(g.row_ptr)[x + 1];
+    const uint32_t s = gp(g.row_ptr)[x], e = s + ld(&g.fill[x]);
     uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    for (uint32_t base = s; base < e; base += WAVE) {
-        uint32_t j = base + lane();
-        uint32_t nb = j < e ? ld(&g.col[j]) : INVALID;
-        uint32_t c = 0;
-        if (nb != INVALID && nb < g.n) c = color_of(ld(&g.word[nb]));
-        if (c > 0 && c < 256) {
-            uint64_t b = 1ull << (c & 63);
-            uint32_t q = c >> 6;
-            m0 |= q == 0 ? b : 0;
-            m1 |= q == 1 ? b : 0;
-            m2 |= q == 2 ? b : 0;
-            m3 |= q == 3 ? b : 0;
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * WAVE) {
+        uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            nb[u] = j < e ? ld(&g.col[j]) : INVALID;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++)
+            c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(ld(&g.word[nb[u]])) : 0;
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            if (c[u] > 0 && c[u] < 256) {
+                uint64_t b = 1ull << (c[u] & 63);
+                uint32_t q = c[u] >> 6;
+                m0 |= q == 0 ? b : 0;
+                m1 |= q == 1 ? b : 0;
+                m2 |= q == 2 ? b : 0;
+                m3 |= q == 3 ? b : 0;
+            }
         }
     }
     m0 = wave_or64(m0) | 1ull;  // colour 0 is not a colour
@@ -226,7 +239,7 @@ __device__ uint16_t fas_bitmap(const GraphDev& g, uint32_t x) {
 
 // A/include/color_select.cuh:59-89.  First zero bin in ascending order (one
 // ballot per 64 bins), else the minimum count with the lowest index.
-__device__ uint16_t fas_hist(const GraphDev& g, uint32_t hd) {
+__device__ __forceinline__ uint16_t fas_hist(const GraphDev& g, uint32_t hd) {
     const uint32_t* h = g.hist + (size_t)hd * HIST_BINS;
     uint64_t best = ~0ull;
     for (uint32_t k = 0; k < HIST_BINS; k += WAVE) {
@@ -260,18 +273,16 @@ __device__ __forceinline__ void on_removed_neighbor(const GraphDev& g, uint32_t 
     if (hd != INVALID && c <= HIST_BINS) add32(&g.hist[(size_t)hd * HIST_BINS + c - 1], 0xFFFFFFFFu);
 }
 // A/include/color_select.cuh:120-157 (per lane: each lane owns one neighbour)
-// Returns true when nb is a bitmap row (its old colour bit stays set, so the
-// bitmap may be inexact until a rebuild; a histogram stays exact).
-__device__ __forceinline__ bool on_neighbor_recolor(const GraphDev& g, uint32_t nb, uint16_t oldc, uint16_t newc) {
+// hd = hd_map[nb].  A bitmap row keeps its old colour bit (inexact until a
+// rebuild); a histogram stays exact.
+__device__ __forceinline__ void on_neighbor_recolor(const GraphDev& g, uint32_t nb, uint32_t hd, uint16_t oldc,
+                                                    uint16_t newc) {
     if (newc != 0 && newc <= 64) or64(&g.bitmap[nb], 1ull << (newc - 1));
-    uint32_t hd = gp(g.hd_map)[nb];
     if (hd != INVALID) {
         uint32_t* h = g.hist + (size_t)hd * HIST_BINS;
         if (oldc != 0 && oldc <= HIST_BINS) add32(&h[oldc - 1], 0xFFFFFFFFu);
         if (newc != 0 && newc <= HIST_BINS) add32(&h[newc - 1], 1u);
-        return false;
     }
-    return true;
 }
 
 // ---------------------------------------------------------------- slack-CSR rows
@@ -279,28 +290,40 @@ __device__ __forceinline__ bool on_neighbor_recolor(const GraphDev& g, uint32_t 
 // CAS) or already holds nb.  Returns the slots scanned; a full row appends to
 // the overflow buffer (entries past capacity are dropped, the counter keeps
 // counting) or, with overflow disabled, reports failure.
-__device__ uint32_t row_insert(const GraphDev& g, uint32_t x, uint32_t nb, bool use_overflow, bool* failed,
+__device__ __forceinline__ uint32_t row_insert(const GraphDev& g, uint32_t x, uint32_t nb, bool use_overflow, bool* failed,
                                WaveAcc* acc) {
     const uint32_t s = gp(g.row_ptr)[x], e = gp(g.row_ptr)[x + 1];
-    for (uint32_t base = s; base < e; base += WAVE) {
-        const uint32_t j = base + lane();
-        const bool in = j < e;
-        const uint32_t val = in ? ld(&g.col[j]) : 0u;
-        uint64_t m = ballot(in && (val == INVALID || val == nb));
-        while (m) {
-            const uint32_t k = (uint32_t)__builtin_ctzll(m);
-            const uint32_t v0 = __shfl(val, (int)k);
-            if (v0 == nb) return base + k - s + 1;
-            uint32_t old = 0;
-            if (lane() == 0) old = cas32(&g.col[base + k], INVALID, nb);
-            old = bcast0(old);
-            if (old == INVALID || old == nb) return base + k - s + 1;
-            m &= m - 1;
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * WAVE) {
+        uint32_t val[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            val[u] = j < e ? ld(&g.col[j]) : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t b = base + u * WAVE;
+            const bool in = b + lane() < e;
+            uint64_t m = ballot(in && (val[u] == INVALID || val[u] == nb));
+            while (m) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(m);
+                const uint32_t v0 = __shfl(val[u], (int)k);
+                if (v0 == nb) return b + k - s + 1;
+                uint32_t old = 0;
+                if (lane() == 0) {
+                    old = cas32(&g.col[b + k], INVALID, nb);
+                    if (old == INVALID) max32(&g.fill[x], b + k - s + 1);
+                }
+                old = bcast0(old);
+                if (old == INVALID || old == nb) return b + k - s + 1;
+                m &= m - 1;
+            }
         }
     }
     if (use_overflow) {
         if (lane() == 0) {
             uint32_t pos = add32(&g.ctr[C_OVF], 1u);
+            add32(&g.ctr[C_ASYM], 1u);
             if (pos < g.ovf_cap) {
                 st(&g.ovf[2 * (size_t)pos], x);
                 st(&g.ovf[2 * (size_t)pos + 1], nb);
@@ -319,53 +342,77 @@ __device__ uint32_t row_insert(const GraphDev& g, uint32_t x, uint32_t nb, bool 
 // v changed colour oldc -> newc without telling its neighbours (first
 // colouring, Tier-1): every neighbour's bitmap or histogram may be stale, and
 // a neighbour whose colour is oldc, newc or none may have a new verdict.
-__device__ void mark_neighbors(const GraphDev& g, uint32_t v, uint16_t oldc, uint16_t newc) {
+__device__ __forceinline__ void mark_neighbors(const GraphDev& g, uint32_t v, uint16_t oldc, uint16_t newc) {
     if (!g.dirty) return;
-    const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
-    for (uint32_t base = s; base < e; base += WAVE) {
-        const uint32_t j = base + lane();
-        const uint32_t nb = j < e ? ld(&g.col[j]) : INVALID;
-        if (nb != INVALID && nb < g.n) {
-            const uint16_t c = color_of(ld(&g.word[nb]));
-            mark(g, nb, D_BITS | ((c == oldc || c == newc || c == 0) ? D_STATUS : 0u));
+    const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * WAVE) {
+        uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            nb[u] = j < e ? ld(&g.col[j]) : INVALID;
         }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++)
+            c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(ld(&g.word[nb[u]])) : 0;
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++)
+            if (nb[u] != INVALID && nb[u] < g.n)
+                mark(g, nb[u], D_BITS | ((c[u] == oldc || c[u] == newc || c[u] == 0) ? D_STATUS : 0u));
     }
 }
 
 // A/src/kernels/local_update.cu:42-56: the first slot holding nb becomes free.
-__device__ uint32_t row_remove(const GraphDev& g, uint32_t x, uint32_t nb) {
-    const uint32_t s = gp(g.row_ptr)[x], e = gp(g.row_ptr)[x + 1];
-    for (uint32_t base = s; base < e; base += WAVE) {
-        const uint32_t j = base + lane();
-        const bool in = j < e;
-        const uint32_t val = in ? ld(&g.col[j]) : INVALID;
-        uint64_t m = ballot(in && val == nb);
-        while (m) {
-            const uint32_t k = (uint32_t)__builtin_ctzll(m);
-            uint32_t old = 0;
-            if (lane() == 0) old = cas32(&g.col[base + k], nb, INVALID);
-            old = bcast0(old);
-            if (old == nb) return base + k - s + 1;
-            m &= m - 1;
+__device__ __forceinline__ uint32_t row_remove(const GraphDev& g, uint32_t x, uint32_t nb) {
+    const uint32_t s = gp(g.row_ptr)[x], e = s + ld(&g.fill[x]);
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * WAVE) {
+        uint32_t val[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            val[u] = j < e ? ld(&g.col[j]) : INVALID;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t b = base + u * WAVE;
+            uint64_t m = ballot(b + lane() < e && val[u] == nb);
+            while (m) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(m);
+                uint32_t old = 0;
+                if (lane() == 0) old = cas32(&g.col[b + k], nb, INVALID);
+                old = bcast0(old);
+                if (old == nb) return b + k - s + 1;
+                m &= m - 1;
+            }
         }
     }
     return e - s;
 }
 
 // A/src/kernels/recolor.cu:50-82: every live neighbour learns the new colour.
-__device__ uint32_t propagate(const GraphDev& g, uint32_t x, uint16_t oldc, uint16_t newc) {
+__device__ __forceinline__ uint32_t propagate(const GraphDev& g, uint32_t x, uint16_t oldc, uint16_t newc) {
     if (oldc == newc) return 0;
-    const uint32_t s = gp(g.row_ptr)[x], e = gp(g.row_ptr)[x + 1];
+    const uint32_t s = gp(g.row_ptr)[x], e = s + ld(&g.fill[x]);
     uint32_t live = 0;
-    for (uint32_t base = s; base < e; base += WAVE) {
-        const uint32_t j = base + lane();
-        const uint32_t nb = j < e ? ld(&g.col[j]) : INVALID;
-        if (nb != INVALID && nb < g.n) {
-            const bool bitmap_row = on_neighbor_recolor(g, nb, oldc, newc);
-            if (g.dirty) {
-                const uint16_t c = color_of(ld(&g.word[nb]));
-                mark(g, nb, (bitmap_row ? D_BITS : 0u) | ((c == oldc || c == newc || c == 0) ? D_STATUS : 0u));
-            }
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * WAVE) {
+        uint32_t nb[ROW_UNROLL], hd[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            nb[u] = j < e ? ld(&g.col[j]) : INVALID;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const bool ok = nb[u] != INVALID && nb[u] < g.n;
+            hd[u] = ok ? gp(g.hd_map)[nb[u]] : INVALID;
+            c[u] = (ok && g.dirty) ? color_of(ld(&g.word[nb[u]])) : 0;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            if (nb[u] == INVALID || nb[u] >= g.n) continue;
+            on_neighbor_recolor(g, nb[u], hd[u], oldc, newc);
+            mark(g, nb[u], (hd[u] == INVALID ? D_BITS : 0u) |
+                               ((c[u] == oldc || c[u] == newc || c[u] == 0) ? D_STATUS : 0u));
             live++;
         }
     }
@@ -375,7 +422,7 @@ __device__ uint32_t propagate(const GraphDev& g, uint32_t x, uint16_t oldc, uint
 
 // A/src/kernels/recolor.cu:87-141.  Select, then publish pack(c', version+1)
 // with one CAS on the packed word; on interference re-select and retry.
-__device__ bool recolor(const GraphDev& g, uint32_t x, WaveAcc& acc) {
+__device__ __forceinline__ bool recolor(const GraphDev& g, uint32_t x, WaveAcc& acc) {
     for (int attempt = 0; attempt < MAX_CAS_RETRIES; attempt++) {
         const uint32_t w = bcast0(ld(&g.word[x]));
         const uint16_t oldc = color_of(w);
@@ -388,7 +435,7 @@ __device__ bool recolor(const GraphDev& g, uint32_t x, WaveAcc& acc) {
         if (got == w) {
             uint32_t live = propagate(g, x, oldc, nc);
             acc.recolor++;
-            acc.rc_cap += gp(g.row_ptr)[x + 1] - gp(g.row_ptr)[x];
+            acc.rc_cap += ld(&g.fill[x]);
             acc.rc_live += live;
             return true;
         }
@@ -397,7 +444,7 @@ __device__ bool recolor(const GraphDev& g, uint32_t x, WaveAcc& acc) {
 }
 
 // A/src/kernels/recolor.cu:145-163
-__device__ void try_reduce(const GraphDev& g, uint32_t x, uint16_t freed, WaveAcc& acc) {
+__device__ __forceinline__ void try_reduce(const GraphDev& g, uint32_t x, uint16_t freed, WaveAcc& acc) {
     const uint16_t cur = color_of(bcast0(ld(&g.word[x])));
     if (cur == 0 || freed >= cur) return;
     if (find_color(g, x) < cur) recolor(g, x, acc);
@@ -412,7 +459,7 @@ __device__ __forceinline__ void color_fresh(const GraphDev& g, uint32_t x, uint3
 }
 
 // One update, A/src/kernels/local_update.cu:71-146.
-__device__ void apply_update(const GraphDev& g, uint32_t u, uint32_t v, uint32_t type, WaveAcc& acc) {
+__device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint32_t v, uint32_t type, WaveAcc& acc) {
     if (u >= g.n || v >= g.n) return;  // hash miss: skipped (:77)
     if (type <= 1 && lane() == 0) {
         // an insert fills the first free slot (the row stays packed); a delete leaves a hole

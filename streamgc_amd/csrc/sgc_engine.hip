@@ -8,7 +8,7 @@
 // launches as one large graph.  The single-graph C ABI (sgc_create ...) is a
 // pool of one.
 //
-// Per-graph layout: word[n] u32, deg0[n] u32, row_ptr[n+1] u32, col[S] u32,
+// Per-graph layout: word[n] u32, deg0[n] u32, row_ptr[n+1] u32, fill[n] u32, col[S] u32,
 // bitmap[n] u64, hd_map[n] u32, hubs[nhd] u32, hist[nhd*1024] u32,
 // overflow 2*cap u32, snapshot 2 x n u16, 5 mark arrays of n (rounded to 4) u8;
 // counters of all graphs in one array (one copy reads them all).
@@ -61,7 +61,7 @@ uint32_t mark_stride(uint32_t n) { return (n + 3u) & ~3u; }
 
 // Bytes one graph occupies on the device (admission control).
 uint64_t graph_bytes(uint32_t n, uint64_t S, uint32_t nhd, uint32_t ovf_cap) {
-    return 28ull * n + 4 + (uint64_t)M_ARRAYS * mark_stride(n) + 4 * S + 4ull * nhd + 4096ull * nhd +
+    return 32ull * n + 4 + (uint64_t)M_ARRAYS * mark_stride(n) + 4 * S + 4ull * nhd + 4096ull * nhd +
            8ull * (ovf_cap ? ovf_cap : 1);
 }
 
@@ -97,7 +97,7 @@ struct GraphState {
     uint32_t n = 0, nhd = 0;
     uint64_t S = 0;
     uint64_t bytes = 0;
-    uint32_t *word = nullptr, *deg0 = nullptr, *row_ptr = nullptr, *col = nullptr, *hd_map = nullptr;
+    uint32_t *word = nullptr, *deg0 = nullptr, *row_ptr = nullptr, *col = nullptr, *hd_map = nullptr, *fill = nullptr;
     uint32_t *hubs = nullptr, *hist = nullptr, *ovf = nullptr;
     uint64_t* bitmap = nullptr;
     uint16_t* snap = nullptr;  // 2 * n
@@ -112,11 +112,12 @@ struct GraphState {
     // Tier-1 loop state of the current repair
     uint32_t prev = 0, plateau = 0;
     bool iterating = false;
+    bool t1_clean = false;     // the last Tier-1 sweep found nothing to repair
     sgc_epoch_stats st{};
 
     void release() {
         for (void* p : {(void*)word, (void*)deg0, (void*)row_ptr, (void*)col, (void*)hd_map, (void*)hubs, (void*)hist,
-                        (void*)ovf, (void*)bitmap, (void*)snap, (void*)history, (void*)dirty})
+                        (void*)ovf, (void*)bitmap, (void*)snap, (void*)history, (void*)dirty, (void*)fill})
             if (p) (void)hipFree(p);
         for (int i = 0; i < 2; i++) {
             if (pub_ev[i]) (void)hipEventDestroy(pub_ev[i]);
@@ -163,6 +164,7 @@ struct Staged {
 
 struct sgc_pool {
     uint32_t max_graphs = 0, batch_cap = 0, flags = 0;
+    sgc_tuning tune{1024, 2048, 16384, 0};
     uint64_t budget = 0, used_bytes = 0;
     hipStream_t stream = nullptr, qstream = nullptr;
     std::vector<GraphState> gs;
@@ -215,6 +217,9 @@ struct sgc_pool {
         L.hbase = ng > 1 ? hbase.d : nullptr;
         L.H = H;
         L.totals = d_totals;
+        L.max_blocks = tune.apply_blocks;
+        L.sweep_blocks = tune.sweep_blocks;
+        L.long_row = tune.long_row;
         return L;
     }
 
@@ -319,7 +324,8 @@ struct sgc_pool {
         if (!tracking()) return hipSuccess;
         L.list = d_list;
         L.list_count = d_list_count;
-        return timed("build_list", [&] { return launch_build_list(L, kind, d_list, d_list_count); });
+        L.list_cap = list_cap;
+        return timed("build_list", [&] { return launch_build_list(L, kind, d_list, list_cap, d_list_count); });
     }
     // All sweeps must visit every vertex of graph gid (its state changed from the host).
     hipError_t mark_all(uint32_t gid) {
@@ -407,6 +413,7 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     A((void**)&G.deg0, (size_t)n * 4);
     A((void**)&G.row_ptr, ((size_t)n + 1) * 4);
     A((void**)&G.col, S * 4);
+    A((void**)&G.fill, (size_t)n * 4);
     A((void**)&G.bitmap, (size_t)n * 8);
     A((void**)&G.hd_map, (size_t)n * 4);
     A((void**)&G.hubs, (size_t)nhd * 4);
@@ -435,7 +442,7 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     A((void**)&col0, nnz * 4);
     if (e == hipSuccess) e = hipMemcpyAsync(rp0, row_ptr, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && nnz) e = hipMemcpyAsync(col0, col_idx, nnz * 4, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = launch_fill_rows(s, n, rp0, col0, G.row_ptr, G.col);
+    if (e == hipSuccess) e = launch_fill_rows(s, n, rp0, col0, G.row_ptr, G.col, G.fill);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (rp0) (void)hipFree(rp0);
     if (col0) (void)hipFree(col0);
@@ -450,6 +457,7 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     d.deg0 = G.deg0;
     d.row_ptr = G.row_ptr;
     d.col = G.col;
+    d.fill = G.fill;
     d.bitmap = G.bitmap;
     d.hd_map = G.hd_map;
     d.hubs = G.hubs;
@@ -594,6 +602,7 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
         gs[g].prev = 0xFFFFFFFFu;
         gs[g].plateau = 0;
         gs[g].iterating = gs[g].cfg.max_repair_iters > 0;
+        gs[g].t1_clean = false;
         gs[g].st.t1_iterations = 0;
         max_iters = std::max(max_iters, gs[g].cfg.max_repair_iters);
     }
@@ -606,8 +615,9 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
         if (e == hipSuccess) e = rebuild(L);
         if (e == hipSuccess) e = launch_zero_ctr(L, C_CONFLICTS);
         LaunchCtx L1 = L;
-        if (e == hipSuccess && !deterministic()) e = sweep(L1, M_CHECK);
-        if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, deterministic()); });
+        const bool serial = deterministic() || tune.serial_repair;
+        if (e == hipSuccess && !serial) e = sweep(L1, M_CHECK);
+        if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, serial); });
         if (e == hipSuccess) e = read_ctr();
         if (e != hipSuccess) return hip_fail(e, "tier-1");
         t1_sweeps++;
@@ -617,6 +627,7 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
             const uint32_t c = ctr(g, C_CONFLICTS);
             G.st.t1_iterations = it + 1;
             G.st.t1_conflicts_last = c;
+            G.t1_clean = c == 0;
             if (c == 0 || it + 1 >= G.cfg.max_repair_iters) {
                 G.iterating = false;
             } else if (c == G.prev) {
@@ -633,8 +644,20 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
 // cpu_serial_repair (A/src/streamgc.cu:1057-1240), run on the host only for
 // the graphs whose row check finds an uncoloured endpoint or a conflict
 // (elsewhere it would change nothing).
+//
+// A Tier-1 sweep that found nothing to repair proves the colouring proper
+// when every edge sits in both of its rows: a same-colour edge has a loser,
+// whose check would have recoloured it, and an uncoloured endpoint with a live
+// neighbour would have been coloured.  Those graphs skip the check (its count
+// is zero) unless an overflow ever left an edge in one row only.
 int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
-    hipError_t e = set_active(gids);
+    std::vector<uint32_t> check;
+    for (uint32_t g : gids) {
+        if (gs[g].t1_clean && ctr(g, C_ASYM) == 0) gs[g].st.t2_remaining = 0;
+        else check.push_back(g);
+    }
+    if (check.empty()) return SGC_OK;
+    hipError_t e = set_active(check);
     LaunchCtx L = ctx(true), Lc = L;
     if (e == hipSuccess) e = launch_zero_ctr(L, C_T2BAD);
     if (e == hipSuccess) e = sweep(Lc, M_T2);
@@ -642,7 +665,7 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
     if (e == hipSuccess) e = read_ctr();
     if (e != hipSuccess) return hip_fail(e, "tier-2 check");
     std::vector<uint32_t> ran;
-    for (uint32_t g : gids) {
+    for (uint32_t g : check) {
         GraphState& G = gs[g];
         G.st.t2_remaining = ctr(g, C_T2BAD);
         if (G.st.t2_remaining == 0) continue;
@@ -836,7 +859,7 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipHostMalloc(&P->h_totals, T_COUNT * 8);
     if (e == hipSuccess) e = hipMalloc(&P->d_batch, (size_t)batch_cap * sizeof(sgc_edge_update));
     if (e == hipSuccess) e = hipMalloc(&P->d_tmp16, 64);
-    if (e == hipSuccess) e = hipMalloc(&P->d_list_count, 4);
+    if (e == hipSuccess) e = hipMalloc(&P->d_list_count, 8);
     P->eg.size = 16;
     while (P->eg.size < 2u * batch_cap) P->eg.size <<= 1;
     P->eg.next_cap = batch_cap;
@@ -1069,6 +1092,19 @@ int sgc_destroy(sgc_engine* e) {
     delete e;
     return SGC_OK;
 }
+
+void sgc_default_tuning(sgc_tuning* t) {
+    if (t) *t = sgc_tuning{1024, 2048, 16384, 0};
+}
+static int set_tuning(sgc_pool* p, const sgc_tuning* t) {
+    if (!p || !t || t->sweep_blocks == 0 || t->apply_blocks == 0 || t->sweep_blocks > 65535 ||
+        t->apply_blocks > 65535)
+        return SGC_ERR_INVALID;
+    p->tune = *t;
+    return SGC_OK;
+}
+int sgc_set_tuning(sgc_engine* e, const sgc_tuning* t) { return set_tuning(e, t); }
+int sgc_pool_set_tuning(sgc_pool* p, const sgc_tuning* t) { return set_tuning(p, t); }
 
 int sgc_set_flags(sgc_engine* e, uint32_t flags) {
     if (!e || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))) return SGC_ERR_INVALID;

@@ -34,46 +34,94 @@ __device__ __forceinline__ void locate(const uint32_t* __restrict__ vbase, uint3
 }
 
 // ---------------------------------------------------------------- sweep sets
-// A maintenance sweep visits either every virtual vertex (list == null) or
-// the worklist a k_build_list launch left on the device (its length is read
-// by the sweep itself, so the host never waits for it).
-__device__ __forceinline__ uint32_t sweep_size(const uint32_t* list, const uint32_t* count, uint32_t V) {
-    return list ? *count : V;
-}
-__device__ __forceinline__ uint32_t sweep_item(const uint32_t* list, uint32_t k) { return list ? list[k] : k; }
+// A maintenance sweep visits either every virtual vertex (list == null, one
+// wavefront per row) or the worklist a k_build_list launch left on the device.
+// The worklist keeps rows of at most `long_row` filled slots at its front
+// (count[0] of them, one wavefront each) and longer rows at its back
+// (count[1], list[cap - 1 - k], one 256-thread block each), so a hub row
+// costs a few block-wide round trips instead of hundreds of wavefront ones.
+// Lengths are read on the device: the host never waits for them.
+constexpr uint32_t BLOCK = 256;
+
+struct SweepList {
+    const uint32_t* list;   // null: every vertex, wavefront per row
+    const uint32_t* count;  // [small, big]
+    uint32_t cap;
+};
+__device__ __forceinline__ uint32_t sweep_small(const SweepList& sl, uint32_t V) { return sl.list ? sl.count[0] : V; }
+__device__ __forceinline__ uint32_t sweep_big(const SweepList& sl) { return sl.list ? sl.count[1] : 0; }
+__device__ __forceinline__ uint32_t small_item(const SweepList& sl, uint32_t k) { return sl.list ? sl.list[k] : k; }
+__device__ __forceinline__ uint32_t big_item(const SweepList& sl, uint32_t k) { return sl.list[sl.cap - 1 - k]; }
 
 // Block-wide sum of per-wave values into one u64 total (one atomic per block).
 // Every thread of the block must call it.
-__device__ void block_add64(uint64_t* totals, uint32_t idx, uint64_t mine) {
-    __shared__ uint64_t part[16];
-    const uint32_t w = threadIdx.x / WAVE, nw = (blockDim.x + WAVE - 1) / WAVE;
+__device__ __forceinline__ void block_add64(uint64_t* totals, uint32_t idx, uint64_t mine) {
+    __shared__ uint64_t part[BLOCK / WAVE];
+    const uint32_t w = threadIdx.x / WAVE;
     if (lane() == 0) part[w] = mine;
     __syncthreads();
     if (threadIdx.x == 0 && totals) {
         uint64_t t = 0;
-        for (uint32_t k = 0; k < nw; k++) t += part[k];
+        for (uint32_t k = 0; k < BLOCK / WAVE; k++) t += part[k];
         if (t) add64(&totals[idx], t);
     }
     __syncthreads();
 }
 
+// Block-wide sum (every thread gets it).  Every thread must call it.
+__device__ __forceinline__ uint32_t block_sum32(uint32_t x) {
+    __shared__ uint32_t part[BLOCK / WAVE];
+    x = wave_sum32(x);
+    if (lane() == 0) part[threadIdx.x / WAVE] = x;
+    __syncthreads();
+    uint32_t t = 0;
+    for (uint32_t k = 0; k < BLOCK / WAVE; k++) t += part[k];
+    __syncthreads();
+    return t;
+}
+
+// Exclusive prefix of x over the block (thread order); *total = block sum.
+__device__ __forceinline__ uint32_t block_scan32(uint32_t x, uint32_t* total) {
+    __shared__ uint32_t part[BLOCK / WAVE];
+    uint32_t inc = x;
+    for (uint32_t d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d);
+        if (lane() >= d) inc += y;
+    }
+    const uint32_t w = threadIdx.x / WAVE;
+    if (lane() == WAVE - 1) part[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, t = 0;
+    for (uint32_t k = 0; k < BLOCK / WAVE; k++) {
+        before += k < w ? part[k] : 0;
+        t += part[k];
+    }
+    __syncthreads();
+    *total = t;
+    return before + inc - x;
+}
+
 // Worklist of the vertices marked in array `kind` (or sticky); their marks
 // are cleared.  One lane per vertex; each wavefront covers LIST_TILE chunks of
-// 64 vertices and appends all its hits with one atomic on the list counter.
+// 64 vertices, and each block appends its hits with one atomic per list end.
 constexpr uint32_t LIST_TILE = 16;
 __global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__ graphs, uint32_t ng,
                                                     const uint32_t* __restrict__ vbase, uint32_t V,
                                                     const uint32_t* __restrict__ active, uint32_t kind,
-                                                    uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+                                                    uint32_t long_row, uint32_t* __restrict__ list, uint32_t cap,
+                                                    uint32_t* __restrict__ count) {
+    __shared__ uint32_t wave_small[BLOCK / WAVE], wave_big[BLOCK / WAVE], block_at[2];
     const uint64_t lt_mask = (1ull << lane()) - 1ull;
-    const uint32_t step = wave_count() * WAVE * LIST_TILE;
-    for (uint32_t base = wave_id() * WAVE * LIST_TILE; base < V; base += step) {
-        uint64_t masks[LIST_TILE];
-        uint32_t total = 0;
+    const uint32_t wv = threadIdx.x / WAVE;
+    const uint32_t span = BLOCK * LIST_TILE;
+    for (uint32_t bbase = blockIdx.x * span; bbase < V; bbase += gridDim.x * span) {
+        const uint32_t base = bbase + wv * WAVE * LIST_TILE;
+        uint64_t small_m[LIST_TILE], big_m[LIST_TILE];
+        uint32_t nsmall = 0, nbig = 0;
 #pragma unroll
         for (uint32_t t = 0; t < LIST_TILE; t++) {
             const uint32_t i = base + t * WAVE + lane();
-            bool hit = false;
+            bool hit = false, big = false;
             if (i < V) {
                 uint32_t gid, v;
                 locate(vbase, ng, i, gid, v);
@@ -84,25 +132,50 @@ __global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__
                     const bool marked = d[kind * st] != 0;
                     hit = marked || d[M_STICKY * st] != 0;
                     if (marked) d[kind * st] = 0;
+                    if (hit) big = gp(g.fill)[v] > long_row;
                 }
             }
-            masks[t] = ballot(hit);
-            total += (uint32_t)__builtin_popcountll(masks[t]);
+            small_m[t] = ballot(hit && !big);
+            big_m[t] = ballot(big);
+            nsmall += (uint32_t)__builtin_popcountll(small_m[t]);
+            nbig += (uint32_t)__builtin_popcountll(big_m[t]);
         }
-        if (total == 0) continue;
-        uint32_t at = 0;
-        if (lane() == 0) at = add32(count, total);
-        at = bcast0(at);
+        if (lane() == 0) {
+            wave_small[wv] = nsmall;
+            wave_big[wv] = nbig;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t ts = 0, tb = 0;
+            for (uint32_t k = 0; k < BLOCK / WAVE; k++) {
+                ts += wave_small[k];
+                tb += wave_big[k];
+            }
+            block_at[0] = ts ? add32(&count[0], ts) : 0;
+            block_at[1] = tb ? add32(&count[1], tb) : 0;
+        }
+        __syncthreads();
+        uint32_t at = block_at[0], bt = block_at[1];
+        for (uint32_t k = 0; k < wv; k++) {
+            at += wave_small[k];
+            bt += wave_big[k];
+        }
+        __syncthreads();
 #pragma unroll
         for (uint32_t t = 0; t < LIST_TILE; t++) {
-            if ((masks[t] >> lane()) & 1ull)
-                list[at + (uint32_t)__builtin_popcountll(masks[t] & lt_mask)] = base + t * WAVE + lane();
-            at += (uint32_t)__builtin_popcountll(masks[t]);
+            const uint32_t i = base + t * WAVE + lane();
+            if ((small_m[t] >> lane()) & 1ull) list[at + (uint32_t)__builtin_popcountll(small_m[t] & lt_mask)] = i;
+            if ((big_m[t] >> lane()) & 1ull) list[cap - 1 - (bt + (uint32_t)__builtin_popcountll(big_m[t] & lt_mask))] = i;
+            at += (uint32_t)__builtin_popcountll(small_m[t]);
+            bt += (uint32_t)__builtin_popcountll(big_m[t]);
         }
     }
 }
 
-__global__ void k_zero_u32(uint32_t* p) { *p = 0; }
+__global__ void k_zero_list_count(uint32_t* p) {
+    p[0] = 0;
+    p[1] = 0;
+}
 
 // ---------------------------------------------------------------- per-edge ordering
 // Parallel waves apply a batch in no particular order, so two updates of one
@@ -197,26 +270,29 @@ __global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__
     WaveAcc acc;
     uint64_t grouped = 0;
     for (uint32_t i = wave_id(); i < count; i += wave_count()) {
-        if (!gkeys) {
-            apply_record(graphs, upd, i, multi, acc);
-            continue;
+        // the updates this wave applies for position i, in stream order: i
+        // alone, or the whole group of i's edge when this wave owns its list
+        uint32_t h = NONE;
+        if (gkeys) {
+            const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
+            const uint32_t s = key == KEY_EMPTY ? NONE : group_find(gkeys, key, gmask);
+            if (s != NONE) {
+                h = gp(ghead)[s];
+                if (h != i) continue;          // the list head's wave applies this group
+            }
         }
-        const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
-        const uint32_t s = key == KEY_EMPTY ? NONE : group_find(gkeys, key, gmask);
-        if (s == NONE) {                       // not an edge update of a live graph: apply alone
-            apply_record(graphs, upd, i, multi, acc);
-            continue;
-        }
-        const uint32_t h = gp(ghead)[s];
-        if (h != i) continue;                  // the list head's wave applies this group
-        // Apply the group in ascending stream position: repeatedly take the
-        // smallest position above the last one applied (groups are short).
+        // Take repeatedly the smallest position above the last one applied
+        // (groups are short).
         uint32_t last = 0;
         bool first = true;
         for (;;) {
             uint32_t best = NONE;
-            for (uint32_t k = h; k != NONE; k = gp(gnext)[k])
-                if ((first || k > last) && (best == NONE || k < best)) best = k;
+            if (h == NONE) {
+                best = first ? i : NONE;
+            } else {
+                for (uint32_t k = h; k != NONE; k = gp(gnext)[k])
+                    if ((first || k > last) && (best == NONE || k < best)) best = k;
+            }
             if (best == NONE) break;
             apply_record(graphs, upd, best, multi, acc);
             grouped += best != i;
@@ -294,37 +370,100 @@ __global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ g
 // changes: a live value moves to w + (live lanes before it); a slot whose
 // value moved and that lies past the chunk's new write front is freed (a
 // later chunk may refill it, in program order).
-__global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                 const uint32_t* __restrict__ vbase, uint32_t V,
-                                                 const uint32_t* __restrict__ active,
-                                                 const uint32_t* __restrict__ list, const uint32_t* count,
-                                                 uint64_t* totals) {
-    uint64_t live_total = 0, rows = 0, slots = 0;
+// One long row, all threads of the block: windows of 1024 slots (4
+// consecutive slots per thread), a block prefix of the live counts, then the
+// moves.  Writes of a window land below its end, reads of the next window
+// start at its end.  Returns the live count; the row's fill becomes it.
+__device__ __forceinline__ uint32_t compact_row_block(const GraphDev& g, uint32_t v) {
+    const uint32_t s = gp(g.row_ptr)[v], e = s + gp(g.fill)[v];
+    uint32_t w = s;
+    for (uint32_t base = s; base < e; base += 4 * BLOCK) {
+        const uint32_t j0 = base + 4 * threadIdx.x;
+        uint32_t val[4], cnt = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) {
+            val[q] = j0 + q < e ? gp(g.col)[j0 + q] : INVALID;
+            cnt += val[q] != INVALID;
+        }
+        uint32_t total = 0;
+        uint32_t pos = w + block_scan32(cnt, &total);
+        const uint32_t w_after = w + total;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) {
+            const uint32_t j = j0 + q;
+            if (val[q] == INVALID) continue;
+            if (pos != j) {
+                gp(g.col)[pos] = val[q];
+                if (j >= w_after) gp(g.col)[j] = INVALID;
+            }
+            pos++;
+        }
+        w = w_after;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gp(g.fill)[v] = w - s;
+    return w - s;
+}
+
+// One row, one wavefront: ROW_UNROLL chunks loaded, then packed in order.
+__device__ __forceinline__ uint32_t compact_row_wave(const GraphDev& g, uint32_t v) {
     const uint64_t lt_mask = (1ull << lane()) - 1ull;
-    const uint32_t total = sweep_size(list, count, V);
-    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
-        uint32_t gid, v;
-        locate(vbase, ng, sweep_item(list, k), gid, v);
-        if (active && !active[gid]) continue;
-        const GraphDev& g = graphs[gid];
-        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
-        uint32_t w = s;
-        for (uint32_t base = s; base < e; base += WAVE) {
-            const uint32_t j = base + lane();
-            const bool in = j < e;
-            const uint32_t val = in ? gp(g.col)[j] : INVALID;
-            const bool live = val != INVALID;
+    const uint32_t s = gp(g.row_ptr)[v], e = s + gp(g.fill)[v];
+    uint32_t w = s;
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * WAVE) {
+        uint32_t val[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            val[u] = j < e ? gp(g.col)[j] : INVALID;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            const bool live = val[u] != INVALID;
             const uint64_t m = ballot(live);
             const uint32_t pos = w + (uint32_t)__builtin_popcountll(m & lt_mask);
             const uint32_t w_after = w + (uint32_t)__builtin_popcountll(m);
-            if (live && pos != j) gp(g.col)[pos] = val;
+            if (live && pos != j) gp(g.col)[pos] = val[u];
             if (live && pos != j && j >= w_after) gp(g.col)[j] = INVALID;
             w = w_after;
         }
-        live_total += w - s;
-        rows++;
-        slots += e - s;
     }
+    if (lane() == 0) gp(g.fill)[v] = w - s;
+    return w - s;
+}
+
+__global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                 const uint32_t* __restrict__ vbase, uint32_t V,
+                                                 const uint32_t* __restrict__ active, SweepList sl,
+                                                 uint64_t* totals) {
+    uint64_t live_total = 0, rows = 0, slots = 0;
+    const uint32_t nbig = sweep_big(sl);
+    for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
+        uint32_t gid, v;
+        locate(vbase, ng, big_item(sl, k), gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t f = gp(g.fill)[v];
+        const uint32_t live = compact_row_block(g, v);
+        if (threadIdx.x == 0) {
+            live_total += live;
+            rows++;
+            slots += f;
+        }
+    }
+    const uint32_t total = sweep_small(sl, V);
+    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
+        uint32_t gid, v;
+        locate(vbase, ng, small_item(sl, k), gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t f = gp(g.fill)[v];
+        live_total += compact_row_wave(g, v);
+        rows++;
+        slots += f;
+    }
+    if (lane() != 0) live_total = rows = slots = 0;
     block_add64(totals, T_LIVE_SLOTS, live_total);
     block_add64(totals, T_SWEEP_ROWS + 0, rows);
     block_add64(totals, T_SWEEP_SLOTS + 0, slots);
@@ -388,64 +527,141 @@ __global__ __launch_bounds__(64) void k_drain_serial(const GraphDev* __restrict_
 // own 4 KiB of LDS, written out as 16 coalesced dwords per lane.  The
 // reference never rebuilds histograms (A/src/kernels/epoch_maintenance.cu:120-123),
 // so its counts drift; this mode keeps them exact at every rebuild.
+// A long hub row: the whole block counts into one shared 1024-bin tile.
+__device__ __forceinline__ void rebuild_hub_block(const GraphDev& g, uint32_t v, uint32_t hd, uint32_t* bins) {
+    for (uint32_t t = threadIdx.x; t < HIST_BINS; t += BLOCK) bins[t] = 0;
+    __syncthreads();
+    const uint32_t s = gp(g.row_ptr)[v], e = s + gp(g.fill)[v];
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * BLOCK) {
+        uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * BLOCK + threadIdx.x;
+            nb[u] = j < e ? gp(g.col)[j] : INVALID;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(gp(g.word)[nb[u]]) : 0;
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++)
+            if (c[u] != 0 && c[u] <= HIST_BINS) atomicAdd(&bins[c[u] - 1], 1u);
+    }
+    __syncthreads();
+    SGC_G uint32_t* dst = gp(g.hist) + (size_t)hd * HIST_BINS;
+    for (uint32_t t = threadIdx.x; t < HIST_BINS; t += BLOCK) dst[t] = bins[t];
+    __syncthreads();
+}
+
+// A bitmap row: the OR of its live neighbours' colours <= 64 (one wavefront,
+// or the whole block for a long row).
+template <bool kBlock>
+__device__ __forceinline__ void rebuild_bitmap(const GraphDev& g, uint32_t v) {
+    const uint32_t s = gp(g.row_ptr)[v], e = s + gp(g.fill)[v];
+    const uint32_t width = kBlock ? BLOCK : WAVE, me = kBlock ? threadIdx.x : lane();
+    uint64_t bm = 0;
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * width) {
+        uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * width + me;
+            nb[u] = j < e ? gp(g.col)[j] : INVALID;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(gp(g.word)[nb[u]]) : 0;
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++)
+            if (c[u] != 0 && c[u] <= 64) bm |= 1ull << (c[u] - 1);
+    }
+    bm = wave_or64(bm);
+    if (kBlock) {
+        __shared__ uint64_t part[BLOCK / WAVE];
+        if (lane() == 0) part[threadIdx.x / WAVE] = bm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t t = 0;
+            for (uint32_t k = 0; k < BLOCK / WAVE; k++) t |= part[k];
+            gp(g.bitmap)[v] = t;
+        }
+        __syncthreads();
+    } else if (lane() == 0) {
+        gp(g.bitmap)[v] = bm;
+    }
+}
+
+// A/include/color_select.cuh:213-251, A/src/kernels/epoch_maintenance.cu:113-128:
+// bitmap-mode rows get the exact OR of their live neighbours' colours <= 64.
+// Exact-histogram mode also recounts every listed hub's 1024 counters from
+// its live row (colours 1..1024) in LDS, then writes them out coalesced.  The
+// reference never rebuilds histograms (A/src/kernels/epoch_maintenance.cu:120-123),
+// so its counts drift; this mode keeps them exact at every rebuild.
 __global__ __launch_bounds__(256) void k_rebuild(const GraphDev* __restrict__ graphs, uint32_t ng,
                                                  const uint32_t* __restrict__ vbase, uint32_t V,
-                                                 const uint32_t* __restrict__ active,
-                                                 const uint32_t* __restrict__ list, const uint32_t* count,
+                                                 const uint32_t* __restrict__ active, SweepList sl,
                                                  int exact_hist, uint64_t* totals) {
-    __shared__ uint32_t bins[4][HIST_BINS];
-    uint32_t* b = bins[threadIdx.x / WAVE];
+    __shared__ uint32_t bins[BLOCK / WAVE][HIST_BINS];
     uint64_t rows = 0, slots = 0;
-    const uint32_t total = sweep_size(list, count, V);
-    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
+    const uint32_t nbig = sweep_big(sl);
+    for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
         uint32_t gid, v;
-        locate(vbase, ng, sweep_item(list, k), gid, v);
+        locate(vbase, ng, big_item(sl, k), gid, v);
         if (active && !active[gid]) continue;
         const GraphDev& g = graphs[gid];
         const uint32_t hd = gp(g.hd_map)[v];
         if (hd != INVALID && !exact_hist) continue;
-        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+        if (threadIdx.x == 0) {
+            rows++;
+            slots += gp(g.fill)[v];
+        }
+        if (hd == INVALID) rebuild_bitmap<true>(g, v);
+        else rebuild_hub_block(g, v, hd, &bins[0][0]);
+    }
+    uint32_t* b = bins[threadIdx.x / WAVE];
+    const uint32_t total = sweep_small(sl, V);
+    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
+        uint32_t gid, v;
+        locate(vbase, ng, small_item(sl, k), gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t hd = gp(g.hd_map)[v];
+        if (hd != INVALID && !exact_hist) continue;
+        const uint32_t s = gp(g.row_ptr)[v], e = s + gp(g.fill)[v];
         rows++;
         slots += e - s;
         if (hd == INVALID) {
-            uint64_t bm = 0;
-            for (uint32_t base = s; base < e; base += WAVE) {
-                const uint32_t j = base + lane();
-                const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
-                if (nb != INVALID && nb < g.n) {
-                    const uint32_t c = color_of(gp(g.word)[nb]);
-                    if (c != 0 && c <= 64) bm |= 1ull << (c - 1);
-                }
-            }
-            bm = wave_or64(bm);
-            if (lane() == 0) gp(g.bitmap)[v] = bm;
+            rebuild_bitmap<false>(g, v);
             continue;
         }
         for (uint32_t t = lane(); t < HIST_BINS; t += WAVE) b[t] = 0;
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t base = s; base < e; base += WAVE) {
-            const uint32_t j = base + lane();
-            const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
-            if (nb != INVALID && nb < g.n) {
-                const uint32_t c = color_of(gp(g.word)[nb]);
-                if (c != 0 && c <= HIST_BINS) atomicAdd(&b[c - 1], 1u);
+        for (uint32_t base = s; base < e; base += ROW_UNROLL * WAVE) {
+            uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+                const uint32_t j = base + u * WAVE + lane();
+                nb[u] = j < e ? gp(g.col)[j] : INVALID;
             }
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++)
+                c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(gp(g.word)[nb[u]]) : 0;
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++)
+                if (c[u] != 0 && c[u] <= HIST_BINS) atomicAdd(&b[c[u] - 1], 1u);
         }
         __builtin_amdgcn_wave_barrier();
         SGC_G uint32_t* dst = gp(g.hist) + (size_t)hd * HIST_BINS;
         for (uint32_t t = lane(); t < HIST_BINS; t += WAVE) dst[t] = b[t];
         __builtin_amdgcn_wave_barrier();
     }
+    if (lane() != 0) rows = slots = 0;
     block_add64(totals, T_SWEEP_ROWS + 1, rows);
     block_add64(totals, T_SWEEP_SLOTS + 1, slots);
 }
 
 // ---------------------------------------------------------------- tier-1 repair
 // A/src/kernels/conflict_repair.cu:10-114 for one vertex; returns 1 if counted.
-__device__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) {
+__device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) {
     const uint32_t w = bcast0(ld(&g.word[v]));
     const uint16_t mine = color_of(w);
-    const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+    const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
     if (mine == 0) {
         bool has_nb = false;
         for (uint32_t base = s; base < e && !has_nb; base += WAVE) {
@@ -463,13 +679,23 @@ __device__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) {
     }
     const uint32_t my_deg = gp(g.deg0)[v];
     bool need = false;
-    for (uint32_t base = s; base < e && !need; base += WAVE) {
-        const uint32_t j = base + lane();
-        const uint32_t nb = j < e ? ld(&g.col[j]) : INVALID;
+    for (uint32_t base = s; base < e && !need; base += ROW_UNROLL * WAVE) {
+        uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            nb[u] = j < e ? ld(&g.col[j]) : INVALID;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++)
+            c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(ld(&g.word[nb[u]])) : 0;
         bool hit = false;
-        if (nb != INVALID && nb < g.n && color_of(ld(&g.word[nb])) == mine) {
-            const uint32_t d = gp(g.deg0)[nb];
-            hit = (my_deg < d) || (my_deg == d && v < nb);  // pi(v) < pi(nb)
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            if (c[u] == mine) {
+                const uint32_t d = gp(g.deg0)[nb[u]];
+                hit |= (my_deg < d) || (my_deg == d && v < nb[u]);  // pi(v) < pi(nb)
+            }
         }
         need = ballot(hit) != 0;
     }
@@ -484,17 +710,95 @@ __device__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) {
     return 1;
 }
 
+// The same for a long row, all threads of the block: the verdict is a block
+// OR; the first wavefront selects and stores; the block marks the neighbours.
+__device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32_t v) {
+    __shared__ uint32_t pick;
+    const uint32_t w = ld(&g.word[v]);
+    const uint16_t mine = color_of(w);
+    const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
+    const uint32_t my_deg = gp(g.deg0)[v];
+    uint32_t hit = 0;
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * BLOCK) {
+        uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * BLOCK + threadIdx.x;
+            nb[u] = j < e ? ld(&g.col[j]) : INVALID;
+        }
+        if (mine == 0) {
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++) hit |= nb[u] != INVALID;
+            continue;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++)
+            c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(ld(&g.word[nb[u]])) : 0;
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            if (c[u] == mine) {
+                const uint32_t d = gp(g.deg0)[nb[u]];
+                hit |= (my_deg < d) || (my_deg == d && v < nb[u]);
+            }
+        }
+    }
+    if (block_sum32(hit) == 0) return 0;
+    if (threadIdx.x < WAVE) {
+        if (mine != 0 && mine <= 64 && lane() == 0) or64(&g.bitmap[v], 1ull << (mine - 1));
+        const uint16_t c = find_color(g, v);
+        if (lane() == 0) {
+            st(&g.word[v], bump(w, c));
+            mark(g, v, D_BITS | D_STATUS);
+            pick = c;
+        }
+    }
+    __syncthreads();
+    const uint16_t c = (uint16_t)pick;
+    __syncthreads();
+    if (g.dirty) {
+        for (uint32_t base = s; base < e; base += ROW_UNROLL * BLOCK) {
+            uint32_t nb[ROW_UNROLL], cn[ROW_UNROLL];
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+                const uint32_t j = base + u * BLOCK + threadIdx.x;
+                nb[u] = j < e ? ld(&g.col[j]) : INVALID;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++)
+                cn[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(ld(&g.word[nb[u]])) : 0;
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++)
+                if (nb[u] != INVALID && nb[u] < g.n)
+                    mark(g, nb[u], D_BITS | ((cn[u] == mine || cn[u] == c || cn[u] == 0) ? D_STATUS : 0u));
+        }
+    }
+    return 1;
+}
+
 __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__ graphs, uint32_t ng,
                                                     const uint32_t* __restrict__ vbase, uint32_t V,
-                                                    const uint32_t* __restrict__ active,
-                                                    const uint32_t* __restrict__ list, const uint32_t* count,
+                                                    const uint32_t* __restrict__ active, SweepList sl,
                                                     uint64_t* totals) {
-    uint32_t cnt = 0, cur_gid = 0xFFFFFFFFu;
     uint64_t rows = 0, slots = 0;
-    const uint32_t total = sweep_size(list, count, V);
+    const uint32_t nbig = sweep_big(sl);
+    for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
+        uint32_t gid, v;
+        locate(vbase, ng, big_item(sl, k), gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t f = gp(g.fill)[v];
+        const uint32_t r = tier1_vertex_block(g, v);
+        if (threadIdx.x == 0) {
+            if (r) add32(&g.ctr[C_CONFLICTS], 1u);
+            rows++;
+            slots += f;
+        }
+    }
+    uint32_t cnt = 0, cur_gid = 0xFFFFFFFFu;
+    const uint32_t total = sweep_small(sl, V);
     for (uint32_t k = wave_id(); k < total; k += wave_count()) {
         uint32_t gid, v;
-        locate(vbase, ng, sweep_item(list, k), gid, v);
+        locate(vbase, ng, small_item(sl, k), gid, v);
         if (active && !active[gid]) continue;
         if (gid != cur_gid) {
             if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
@@ -503,10 +807,11 @@ __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__
         }
         const GraphDev& g = graphs[gid];
         rows++;
-        slots += gp(g.row_ptr)[v + 1] - gp(g.row_ptr)[v];
+        slots += gp(g.fill)[v];
         cnt += tier1_vertex(g, v);
     }
     if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
+    if (lane() != 0) rows = slots = 0;
     block_add64(totals, T_SWEEP_ROWS + 2, rows);
     block_add64(totals, T_SWEEP_SLOTS + 2, slots);
 }
@@ -558,39 +863,71 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
 // ---------------------------------------------------------------- checks
 // Tier-2 has work iff some row slot joins an uncoloured endpoint or two
 // endpoints of one colour (A/src/streamgc.cu:1190-1218 then change nothing).
-// A vertex found with a bad slot keeps its check bit, so the next check
-// counts it again until a repair changes it.
-__global__ __launch_bounds__(256) void k_t2_check(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                  const uint32_t* __restrict__ vbase, uint32_t V,
-                                                  const uint32_t* __restrict__ active,
-                                                  const uint32_t* __restrict__ list, const uint32_t* count,
-                                                  uint64_t* totals) {
-    uint64_t rows = 0, slots = 0;
-    const uint32_t total = sweep_size(list, count, V);
-    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
-        uint32_t gid, v;
-        locate(vbase, ng, sweep_item(list, k), gid, v);
-        if (active && !active[gid]) continue;
-        const GraphDev& g = graphs[gid];
-        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
-        const uint16_t cv = color_of(gp(g.word)[v]);
-        uint32_t bad = 0;
-        for (uint32_t base = s; base < e; base += WAVE) {
-            const uint32_t j = base + lane();
-            const uint32_t nb = j < e ? gp(g.col)[j] : INVALID;
-            if (nb != INVALID && nb < g.n && nb != v) {
-                const uint16_t cn = color_of(gp(g.word)[nb]);
+// Bad slots of row v: an uncoloured endpoint or two equal colours.
+template <bool kBlock>
+__device__ __forceinline__ uint32_t t2_row(const GraphDev& g, uint32_t v) {
+    const uint32_t s = gp(g.row_ptr)[v], e = s + gp(g.fill)[v];
+    const uint32_t width = kBlock ? BLOCK : WAVE, me = kBlock ? threadIdx.x : lane();
+    const uint16_t cv = color_of(gp(g.word)[v]);
+    uint32_t bad = 0;
+    for (uint32_t base = s; base < e; base += ROW_UNROLL * width) {
+        uint32_t nb[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * width + me;
+            nb[u] = j < e ? gp(g.col)[j] : INVALID;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            if (nb[u] != INVALID && nb[u] < g.n && nb[u] != v) {
+                const uint16_t cn = color_of(gp(g.word)[nb[u]]);
                 bad += (cv == 0 || cn == 0 || cv == cn);
             }
         }
-        bad = wave_sum32(bad);
+    }
+    return kBlock ? block_sum32(bad) : wave_sum32(bad);
+}
+
+// Tier-2 has work iff some row slot joins an uncoloured endpoint or two
+// endpoints of one colour (A/src/streamgc.cu:1190-1218 then change nothing).
+// A vertex found with a bad slot keeps its mark, so the next check counts it
+// again until a repair changes it.
+__global__ __launch_bounds__(256) void k_t2_check(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                  const uint32_t* __restrict__ vbase, uint32_t V,
+                                                  const uint32_t* __restrict__ active, SweepList sl,
+                                                  uint64_t* totals) {
+    uint64_t rows = 0, slots = 0;
+    const uint32_t nbig = sweep_big(sl);
+    for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
+        uint32_t gid, v;
+        locate(vbase, ng, big_item(sl, k), gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t bad = t2_row<true>(g, v);
+        if (threadIdx.x == 0) {
+            rows++;
+            slots += gp(g.fill)[v];
+            if (bad) {
+                add32(&g.ctr[C_T2BAD], bad);
+                if (g.dirty) gp(g.dirty)[(size_t)M_T2 * g.dstride + v] = 1;
+            }
+        }
+    }
+    const uint32_t total = sweep_small(sl, V);
+    for (uint32_t k = wave_id(); k < total; k += wave_count()) {
+        uint32_t gid, v;
+        locate(vbase, ng, small_item(sl, k), gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t bad = t2_row<false>(g, v);
         rows++;
-        slots += e - s;
+        slots += gp(g.fill)[v];
         if (bad && lane() == 0) {
             add32(&g.ctr[C_T2BAD], bad);
             if (g.dirty) gp(g.dirty)[(size_t)M_T2 * g.dstride + v] = 1;
         }
     }
+    if (lane() != 0) rows = slots = 0;
     block_add64(totals, T_SWEEP_ROWS + 3, rows);
     block_add64(totals, T_SWEEP_SLOTS + 3, slots);
 }
@@ -602,7 +939,7 @@ __global__ __launch_bounds__(256) void k_validate(const GraphDev* __restrict__ g
     const GraphDev& g = graphs[gid];
     uint32_t viol = 0, unc = 0, mx = 0;
     for (uint32_t v = wave_id(); v < g.n; v += wave_count()) {
-        const uint32_t s = gp(g.row_ptr)[v], e = gp(g.row_ptr)[v + 1];
+        const uint32_t s = gp(g.row_ptr)[v], e = s + gp(g.fill)[v];
         const uint16_t cv = colors ? colors[v] : color_of(gp(g.word)[v]);
         uint32_t bad = 0, has = 0;
         for (uint32_t base = s; base < e; base += WAVE) {
@@ -660,10 +997,12 @@ __global__ __launch_bounds__(256) void k_validate_history(const uint16_t* __rest
 // the front of its slack row, free slots after it.
 __global__ __launch_bounds__(256) void k_fill_rows(uint32_t n, const uint32_t* __restrict__ rp0,
                                                    const uint32_t* __restrict__ col0,
-                                                   const uint32_t* __restrict__ rp, uint32_t* __restrict__ col) {
+                                                   const uint32_t* __restrict__ rp, uint32_t* __restrict__ col,
+                                                   uint32_t* __restrict__ fill) {
     for (uint32_t v = wave_id(); v < n; v += wave_count()) {
         const uint32_t s0 = rp0[v], d = rp0[v + 1] - s0;
         const uint32_t s = rp[v], e = rp[v + 1];
+        if (lane() == 0) fill[v] = d < e - s ? d : e - s;
         for (uint32_t j = s + lane(); j < e; j += WAVE) {
             const uint32_t k = j - s;
             col[j] = k < d ? col0[s0 + k] : INVALID;
@@ -726,21 +1065,23 @@ hipError_t launch_snapshot(const LaunchCtx& L) {
     k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
     return hipGetLastError();
 }
-// Sweep grids: one wavefront per row, grid-stride; a worklist sweep takes the
-// grid of a full sweep capped at 8192 wavefronts (its length is on the device).
+// Sweep grids: one wavefront per row (blocks first take the long rows of a
+// worklist), grid-stride; a worklist sweep takes the grid of a full sweep
+// capped at sweep_blocks (its length is on the device).
 static inline uint32_t sweep_grid(const LaunchCtx& L) {
-    return grid_for_waves(L.V, 4, L.list ? 2048 : L.max_blocks);
+    return grid_for_waves(L.V, 4, L.list ? L.sweep_blocks : L.max_blocks);
 }
-hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t* count) {
-    if (kind >= M_STICKY) return hipErrorInvalidValue;
-    k_zero_u32<<<1, 1, 0, L.stream>>>(count);
-    uint32_t grid = grid_for_waves((L.V + WAVE * LIST_TILE - 1) / (WAVE * LIST_TILE), 4, 4096);
-    k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, kind, list, count);
+static inline SweepList sweep_list(const LaunchCtx& L) { return SweepList{L.list, L.list_count, L.list_cap}; }
+hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count) {
+    if (kind >= M_STICKY || cap < L.V) return hipErrorInvalidValue;
+    k_zero_list_count<<<1, 1, 0, L.stream>>>(count);
+    uint32_t grid = grid_for_waves((L.V + WAVE * LIST_TILE - 1) / (WAVE * LIST_TILE), BLOCK / WAVE, 4096);
+    k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, kind, L.long_row, list, cap,
+                                             count);
     return hipGetLastError();
 }
 hipError_t launch_compact(const LaunchCtx& L) {
-    k_compact<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.list, L.list_count,
-                                                   L.totals);
+    k_compact<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, sweep_list(L), L.totals);
     return hipGetLastError();
 }
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
@@ -754,7 +1095,7 @@ hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
     return hipGetLastError();
 }
 hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist) {
-    k_rebuild<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.list, L.list_count,
+    k_rebuild<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, sweep_list(L),
                                                    exact_hist ? 1 : 0, L.totals);
     return hipGetLastError();
 }
@@ -762,13 +1103,13 @@ hipError_t launch_tier1(const LaunchCtx& L, bool serial) {
     if (serial) {
         k_tier1_serial<<<L.ng < 1024 ? L.ng : 1024, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, L.totals);
     } else {
-        k_tier1_elem<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.list,
-                                                          L.list_count, L.totals);
+        k_tier1_elem<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, sweep_list(L),
+                                                          L.totals);
     }
     return hipGetLastError();
 }
 hipError_t launch_t2_check(const LaunchCtx& L) {
-    k_t2_check<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, L.list, L.list_count,
+    k_t2_check<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, sweep_list(L),
                                                     L.totals);
     return hipGetLastError();
 }
@@ -791,9 +1132,9 @@ hipError_t launch_validate_history(hipStream_t s, const uint16_t* snaps, uint32_
     return hipGetLastError();
 }
 hipError_t launch_fill_rows(hipStream_t s, uint32_t n, const uint32_t* rp0, const uint32_t* col0, const uint32_t* rp,
-                            uint32_t* col) {
+                            uint32_t* col, uint32_t* fill) {
     uint32_t grid = grid_for_waves(n, 4, 16384);
-    k_fill_rows<<<grid, 256, 0, s>>>(n, rp0, col0, rp, col);
+    k_fill_rows<<<grid, 256, 0, s>>>(n, rp0, col0, rp, col, fill);
     return hipGetLastError();
 }
 

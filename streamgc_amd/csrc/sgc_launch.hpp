@@ -20,9 +20,12 @@ struct LaunchCtx {
     const uint32_t* hbase = nullptr;   // device, ng + 1 prefix of hub counts (null when ng == 1)
     uint32_t H = 0;                    // total hubs
     uint64_t* totals = nullptr;        // device u64 counters
-    uint32_t max_blocks = 16384;
+    uint32_t max_blocks = 16384;       // grid cap of per-update and full-sweep launches
+    uint32_t sweep_blocks = 2048;      // grid cap of worklist sweeps
+    uint32_t long_row = 1024;          // worklist rows with more filled slots go to a whole block
     const uint32_t* list = nullptr;    // device worklist of virtual vertices (null = sweep all)
-    const uint32_t* list_count = nullptr;
+    const uint32_t* list_count = nullptr;  // device [rows at the front, long rows at the back]
+    uint32_t list_cap = 0;
 };
 
 // Per-edge grouping of a parallel batch: hash table (size: power of two >=
@@ -43,7 +46,7 @@ hipError_t launch_max_color(const LaunchCtx& L);
 hipError_t launch_snapshot(const LaunchCtx& L);
 hipError_t launch_compact(const LaunchCtx& L);
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
-hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t* count);
+hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count);
 hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist);
 hipError_t launch_tier1(const LaunchCtx& L, bool serial);
 hipError_t launch_t2_check(const LaunchCtx& L);
@@ -53,6 +56,6 @@ hipError_t launch_validate_history(hipStream_t s, const uint16_t* snaps, uint32_
                                    const uint32_t* ev, const uint64_t* born, const uint64_t* died, uint64_t m,
                                    const uint64_t* pos, uint32_t count, unsigned long long* bad_out);
 hipError_t launch_fill_rows(hipStream_t s, uint32_t n, const uint32_t* rp0, const uint32_t* col0, const uint32_t* rp,
-                            uint32_t* col);
+                            uint32_t* col, uint32_t* fill);
 
 }  // namespace sgc
