@@ -35,6 +35,10 @@ WORKLOADS = {
     "c2": dict(name="Chung-Lu com-LiveJournal shape: 3,997,962 vertices, 34,681,189 edges, gamma 2.3, "
                     "max degree 15K, permuted (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
                kind="chung_lu", n=3997962, m=34681189, gamma=2.3, max_degree=15000.0),
+    "c5": dict(name="served: 64 tenants (Delaunay 2^10-2^14, R-MAT s10-14 ef 8-40, Chung-Lu 2^12-2^15, "
+                    "R-MAT s12 ef 48) + com-Orkut-shape Chung-Lu (3,072,441 vertices, 117M edges) resident in one "
+                    "pool; churn batches of U[1,8192] updates to uniform tenants; E_g = 4,096",
+               kind="served"),
     "c1": dict(name="Delaunay 2^17 uniform points (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
                kind="delaunay", points=1 << 17),
 }
@@ -113,6 +117,9 @@ def main():
     ap.add_argument("--long-row", type=int, default=None, help="tuning: worklist rows longer than this go to a block")
     ap.add_argument("--sweep-blocks", type=int, default=None, help="tuning: grid cap of worklist sweeps")
     ap.add_argument("--serial-repair", type=int, default=None, help="tuning: 1 = Tier-1 in vertex order")
+    ap.add_argument("--requests", type=int, default=2000, help="served workload: requests per phase")
+    ap.add_argument("--load", type=float, default=0.5, help="served workload: offered load / served capacity")
+    ap.add_argument("--max-delay-us", type=int, default=2000, help="served workload: publish deadline")
     ap.add_argument("--init-batch", type=int, default=256,
                     help="apply the (untimed) init stream in launches of this many updates, epochs still every E")
     args = ap.parse_args()
@@ -126,6 +133,8 @@ def main():
         sys.exit(1)
 
     w = WORKLOADS[args.workload]
+    if w["kind"] == "served":
+        return run_served(S, args, w)
     E = 32768
     K, W = args.steps, args.warmup
     t = time.perf_counter()
@@ -246,6 +255,169 @@ def main():
     }
     print(json.dumps(out))
 
+
+
+# ---------------------------------------------------------------- served (C5)
+def make_tenants(S, count, seed=42):
+    """Tenant graphs of the C1-C4 shapes at tenant scale (seed 42 + index)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        kind = ("delaunay", "rmat", "chung_lu", "rmat_dense")[i % 4]
+        if kind == "delaunay":
+            g = S.gen_delaunay(int(2 ** rng.uniform(10, 14)), seed=seed + i)
+        elif kind == "rmat":
+            g = S.gen_rmat(int(rng.integers(10, 15)), float(rng.uniform(8, 40)), seed=seed + i)
+        elif kind == "chung_lu":
+            n = int(2 ** rng.uniform(12, 15))
+            g = S.gen_chung_lu(n, n * 8, 2.3, 4.0 * n ** 0.5, seed=seed + i, permute=True)
+        else:
+            g = S.gen_rmat(12, 48.0, seed=seed + i)
+        out.append((kind, g))
+    return out
+
+
+def pct(a, q):
+    return round(float(np.percentile(a, q)), 3) if len(a) else None
+
+
+def run_served(S, args, w):
+    """Served vs plain on the mixed-tenant workload.
+
+    plain: the per-launch engine -- every request is its own launch
+    (sgc_pool_apply of that request), and a tenant's epoch runs on its own as
+    soon as it is due.  served: the request loop packs every request that has
+    arrived into one launch and runs one shared epoch step for the due
+    tenants.  Both start from the same resident state; plain takes the first
+    half of every tenant's churn, served the second half (same distribution).
+    Throughput is measured at saturation (every request available at once);
+    latency at an offered Poisson load below the served capacity.
+    """
+    E = 4096
+    R = args.requests
+    t = time.perf_counter()
+    tenants = make_tenants(S, 64)
+    orkut = S.gen_chung_lu(3072441, 117185083, 2.3, 33313.0, seed=42, permute=True)
+    tenants.append(("orkut", orkut))
+    log(f"tenants: 64 + Orkut-shape ({orkut.n} vertices, {orkut.num_edges // 2} edges) "
+        f"({time.perf_counter() - t:.1f}s)")
+    # the untimed init streams are applied without intermediate epochs, then
+    # repaired once; the timed phases cut every tenant's epochs at E
+    cfg = S.StreamGCConfig(epoch_size=1 << 30, batch_size=1 << 16, exact_histogram=True, repair_every_epoch=True)
+    pool = S.GraphPool(max_graphs=len(tenants), batch_capacity=1 << 16, repair_every_epoch=True,
+                       exact_histogram=True)
+    gids = [pool.add(g, cfg) for _, g in tenants]
+    used, budget, ng = pool.usage()
+    t = time.perf_counter()
+    for gid, (_, g) in zip(gids, tenants):
+        pool.submit(S.tag_graph(S.generate_init_stream(g), gid), max_stats=0)
+    pool.finish()
+    log(f"init of {ng} tenants ({used / 2**30:.2f} GiB resident): {time.perf_counter() - t:.1f}s")
+
+    # requests: uniform tenant, U[1, 8192] updates, each tenant's churn in order
+    rng = np.random.default_rng(7)
+    picks = rng.integers(0, len(tenants), 3 * R)
+    sizes = rng.integers(1, 8193, 3 * R)
+    need = np.bincount(picks, weights=sizes, minlength=len(tenants)).astype(np.int64)
+    churn = [S.generate_bench_stream_edges(g, int(need[k]) + 1, int(rng.integers(30, 71)), 42 + k)
+             for k, (_, g) in enumerate(tenants)]
+    head = [0] * len(tenants)
+    reqs = []
+    for k, n in zip(picks, sizes):
+        reqs.append((int(k), S.tag_graph(churn[k][head[k]:head[k] + n], gids[k])))
+        head[k] += int(n)
+    plain_reqs, served_reqs, lat_reqs = reqs[:R], reqs[R:2 * R], reqs[2 * R:]
+
+    # plain: one launch per request, per-tenant epochs when due
+    since = [0] * len(tenants)
+    pool.kernel_timing(True)
+    tot0 = pool.totals()
+    t0 = time.perf_counter()
+    for k, recs in plain_reqs:
+        pool.apply(recs)
+        since[k] += len(recs)
+        if since[k] >= E:
+            pool.epoch([gids[k]])
+            since[k] = 0
+    pool.epoch([gids[k] for k in range(len(tenants)) if since[k]])
+    plain_s = time.perf_counter() - t0
+    plain_n = sum(len(r) for _, r in plain_reqs)
+    plain_rate = plain_n / plain_s
+    plain_k = pool.kernel_times()
+    tot1 = pool.totals()
+    plain_tot = {k: tot1[k] - tot0[k] for k in ("t1_sweeps", "t2_runs", "overflow_events")}
+    pool.kernel_timing(False)
+    pool.kernel_timing(True)
+    log(f"plain: {len(plain_reqs)} requests, {plain_n} updates in {plain_s:.2f}s = {plain_rate / 1e6:.2f} M updates/s")
+
+    # served at saturation
+    srv = S.GraphServer(pool, max_batch=1 << 16, max_delay_us=args.max_delay_us)
+    for k, gid in enumerate(gids):
+        srv.attach(gid, E)
+    t0 = time.perf_counter()
+    for k, recs in served_reqs:
+        srv.submit(gids[k], recs)
+    srv.flush()
+    served_s = time.perf_counter() - t0
+    served_n = sum(len(r) for _, r in served_reqs)
+    served_rate = served_n / served_s
+    st = srv.stats()
+    served_k = pool.kernel_times()
+    tot2 = pool.totals()
+    served_tot = {k: tot2[k] - tot1[k] for k in ("t1_sweeps", "t2_runs", "overflow_events")}
+    pool.kernel_timing(False)
+    log(f"served: {served_n} updates in {served_s:.2f}s = {served_rate / 1e6:.2f} M updates/s "
+        f"({st['batches']} launches, {st['epoch_steps']} epoch steps for {st['graph_epochs']} graph epochs)")
+
+    # latency at an offered Poisson load of args.load x the served capacity
+    srv.reset_stats()
+    rate = args.load * served_rate
+    gaps = rng.exponential(1.0, len(lat_reqs)) * (np.mean([len(r) for _, r in lat_reqs]) / rate)
+    t0 = time.perf_counter()
+    due = t0
+    qids = [rng.integers(0, tenants[k][1].n, int(rng.integers(1, 1025))).astype(np.uint32) for k, _ in lat_reqs]
+    for j, (k, recs) in enumerate(lat_reqs):
+        due += gaps[j]
+        now = time.perf_counter()
+        if due > now:
+            time.sleep(due - now)
+        srv.submit(gids[k], recs)
+        if j % 10 == 9:
+            srv.query(gids[k], qids[j])
+    srv.flush()
+    lat_s = time.perf_counter() - t0
+    lat = srv.latencies("update") / 1e3
+    qlat = srv.latencies("query") / 1e3
+    lat_n = sum(len(r) for _, r in lat_reqs)
+    epoch_period_ms = E / (rate / len(tenants)) * 1e3
+    log(f"at {rate / 1e6:.1f} M updates/s offered: p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms; "
+        f"query p50 {pct(qlat, 50)} ms p99 {pct(qlat, 99)} ms; one tenant epoch = {epoch_period_ms:.1f} ms")
+    srv.close()
+    proper = all(pool.validate(gid, snapshot=True)[:2] == (0, 0) for gid in gids)
+    max_colors = {kind: 0 for kind, _ in tenants}
+    for gid, (kind, g) in zip(gids, tenants):
+        max_colors[kind] = max(max_colors[kind], pool.validate(gid)[2])
+    out = {
+        "metric": METRIC, "value": round(served_rate, 1), "unit": "updates/s", "n_gpus": 1, "steps": R,
+        "warmup": 0, "ms_per_step": round(served_s * 1e3 / R, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": w["name"], "tenants": len(tenants), "resident_gib": round(used / 2**30, 2),
+                   "requests_per_phase": R, "epoch_size": E, "max_delay_us": args.max_delay_us},
+        "plain": {"value": round(plain_rate, 1), "seconds": round(plain_s, 3), "updates": plain_n,
+                  "totals": plain_tot, "kernels_ms": {k: round(v[0], 2) for k, v in plain_k.items()}},
+        "served_vs_plain": round(served_rate / plain_rate, 2),
+        "served": {"seconds": round(served_s, 3), "updates": served_n, **{k: st[k] for k in
+                   ("batches", "epoch_steps", "graph_epochs")}, "totals": served_tot,
+                   "kernels_ms": {k: round(v[0], 2) for k, v in served_k.items()}},
+        "latency_ms": {"offered_updates_per_s": round(rate, 1), "achieved_updates_per_s": round(lat_n / lat_s, 1),
+                       "update_to_snapshot_p50": pct(lat, 50), "update_to_snapshot_p99": pct(lat, 99),
+                       "query_p50": pct(qlat, 50), "query_p99": pct(qlat, 99),
+                       "tenant_epoch_period_ms": round(epoch_period_ms, 2), "requests": int(len(lat)),
+                       "queries": int(len(qlat))},
+        "parity": {"all_snapshots_proper": proper, "max_color_by_shape": max_colors},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
 
 if __name__ == "__main__":
     main()

@@ -264,6 +264,59 @@ int sgc_pool_kernel_timing_enable(sgc_pool* p, int enable);
 int sgc_pool_kernel_times(sgc_pool* p, char* names, uint32_t name_stride, double* total_ms, uint64_t* launches,
                           uint32_t cap, uint32_t* count);
 
+/* ---- served pattern: a persistent request loop over a pool ----
+ * No reference counterpart.  Any thread submits update batches and queries
+ * for resident graphs as they arrive; one loop thread packs every arrived
+ * request into one batch per launch (each graph filled at most to its own
+ * epoch boundary, so each graph sees the epoch boundaries it would see
+ * alone), then runs one shared epoch step for the graphs that are due (epoch
+ * full, or oldest unpublished update older than max_delay_us).  A request
+ * completes when the snapshot holding its last update is published; its
+ * update-to-snapshot latency is kept raw.  Admissions wait in arrival order
+ * until the pool's HBM budget has room.  While a server runs, its pool is
+ * driven only by the server. */
+typedef struct sgc_server sgc_server;
+typedef struct sgc_server_config {
+    uint32_t max_batch;     /* updates per launch over all tenants (default 32768) */
+    uint32_t max_delay_us;  /* epoch for a graph whose oldest unpublished update waited this long (0: full epochs only) */
+    uint32_t reserved[2];
+} sgc_server_config;
+typedef struct sgc_server_stats {
+    uint64_t updates_applied;
+    uint64_t batches;         /* apply launches (each over all tenants with work) */
+    uint64_t epoch_steps;     /* shared epoch steps */
+    uint64_t graph_epochs;    /* graph epochs inside them */
+    uint64_t requests_done;
+    uint64_t queries;
+    uint64_t queued_updates;  /* submitted, not yet applied */
+    double span_us;           /* first submit to last publish */
+    double busy_us;           /* loop time in turns that did work */
+} sgc_server_stats;
+void sgc_default_server_config(sgc_server_config* c);
+int sgc_server_create(sgc_pool* pool, const sgc_server_config* cfg, sgc_server** out);
+int sgc_server_destroy(sgc_server* s);   /* publishes everything pending, then stops */
+const char* sgc_server_last_error(const sgc_server* s);
+/* Queue a graph for admission; sgc_server_admitted waits for its pool id. */
+int sgc_server_admit(sgc_server* s, const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr,
+                     const uint32_t* col_idx, const uint32_t* deg0, uint64_t* ticket);
+int sgc_server_admitted(sgc_server* s, uint64_t ticket, uint32_t timeout_ms, uint32_t* gid);
+/* Serve a graph already in the pool (its epochs every epoch_size updates); wait with sgc_server_admitted. */
+int sgc_server_attach(sgc_server* s, uint32_t gid, uint32_t epoch_size, uint64_t* ticket);
+/* Remove a graph once its pending updates are published. */
+int sgc_server_remove(sgc_server* s, uint32_t gid);
+/* Copy and queue `count` updates for graph gid (graph id bytes are stamped). */
+int sgc_server_submit(sgc_server* s, uint32_t gid, const sgc_edge_update* updates, uint32_t count, uint64_t* ticket);
+/* Wait until the ticket's updates are in a published snapshot (SGC_ERR_STATE on timeout). */
+int sgc_server_wait(sgc_server* s, uint64_t ticket, uint32_t timeout_ms);
+/* Publish every pending update now (partial epochs) and wait for it. */
+int sgc_server_flush(sgc_server* s, uint32_t timeout_ms);
+/* Colours of the published snapshot; never waits for the loop. */
+int sgc_server_query(sgc_server* s, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors);
+int sgc_server_get_stats(sgc_server* s, sgc_server_stats* out);
+/* Raw latencies in microseconds: which 0 = update-to-snapshot per request, 1 = query. */
+int sgc_server_latencies(sgc_server* s, int which, double* out, uint64_t cap, uint64_t* count);
+int sgc_server_reset_stats(sgc_server* s);
+
 /* ---- host pipeline (C++17) ---- */
 
 /* Graph container (host CSR, symmetric, sorted rows, no self-loops). */

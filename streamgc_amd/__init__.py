@@ -20,7 +20,7 @@ __all__ = ["StreamGCConfig", "default_config", "CSRGraph", "GPUPartition", "Devi
            "load_graph_csr", "csr_from_edges", "csr_from_edge_arrays", "gen_chung_lu", "gen_rmat",
            "gen_delaunay", "core_numbers", "dense_growth_stream", "generate_init_stream", "generate_bench_stream_edges",
            "static_greedy_coloring", "check_coloring_valid", "edge_intervals", "make_updates", "SgcError",
-           "SgcNoMemory", "device_count", "GraphPool", "tag_graph"]
+           "SgcNoMemory", "device_count", "GraphPool", "GraphServer", "tag_graph"]
 
 from ._lib import SgcNoMemory  # noqa: E402
 
@@ -624,3 +624,112 @@ def tag_graph(records, gid):
 def check_coloring_valid(partition, snapshot=False):
     """= check_coloring_valid (A/src/streamgc.cu:448-556)."""
     return partition.validate(snapshot)
+
+
+class GraphServer:
+    """The served pattern over a GraphPool: a persistent loop thread takes
+    update batches and queries for many resident graphs as they arrive,
+    packs all arrived updates into one launch (continuous batching) and runs
+    shared epoch steps for the graphs that are due.  While it runs, the pool
+    is driven only by the server."""
+
+    def __init__(self, pool, max_batch=32768, max_delay_us=0):
+        cfg = _lib.ServerConfig()
+        lib().sgc_default_server_config(C.byref(cfg))
+        cfg.max_batch = max_batch
+        cfg.max_delay_us = max_delay_us
+        h = C.c_void_p()
+        check(lib().sgc_server_create(pool.h, C.byref(cfg), C.byref(h)))
+        self.h = h.value
+        self.pool = pool
+        self.graphs = {}
+
+    def _ck(self, rc):
+        if rc != 0:
+            msg = (lib().sgc_server_last_error(self.h) or b"").decode()
+            raise (SgcNoMemory if rc == -3 else SgcError)(f"streamgc server error {rc}: {msg}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            rc = lib().sgc_server_destroy(self.h)
+            self.h = None
+            if rc:
+                raise SgcError(f"streamgc server loop ended with error {rc}")
+
+    def __del__(self):
+        try:
+            self.close()
+        except SgcError:
+            pass
+
+    def admit(self, g, config=None):
+        """Queue a graph for admission; returns a ticket for admitted()."""
+        cfg = (config or StreamGCConfig()).to_c()
+        t = C.c_uint64()
+        col = g.col_idx if g.num_edges else np.zeros(1, np.uint32)
+        self._ck(lib().sgc_server_admit(self.h, C.byref(cfg), g.n, g.row_ptr, col, g.degree, C.byref(t)))
+        self._pending_graph = getattr(self, "_pending_graph", {})
+        self._pending_graph[t.value] = g
+        return t.value
+
+    def admitted(self, ticket, timeout_ms=60000):
+        gid = C.c_uint32()
+        self._ck(lib().sgc_server_admitted(self.h, ticket, timeout_ms, C.byref(gid)))
+        g = self._pending_graph.pop(ticket, None)
+        self.graphs[gid.value] = g
+        if g is not None:
+            self.pool.graphs[gid.value] = g
+        return gid.value
+
+    def add(self, g, config=None, timeout_ms=60000):
+        return self.admitted(self.admit(g, config), timeout_ms)
+
+    def attach(self, gid, epoch_size, timeout_ms=60000):
+        """Serve a graph already in the pool (added and initialised through it)."""
+        t = C.c_uint64()
+        self._ck(lib().sgc_server_attach(self.h, gid, epoch_size, C.byref(t)))
+        self._pending_graph = getattr(self, "_pending_graph", {})
+        self._pending_graph[t.value] = self.pool.graphs.get(gid)
+        return self.admitted(t.value, timeout_ms)
+
+    def remove(self, gid):
+        """Remove a graph once its pending updates are published."""
+        self._ck(lib().sgc_server_remove(self.h, gid))
+        self.graphs.pop(gid, None)
+        self.pool.graphs.pop(gid, None)
+
+    def submit(self, gid, records):
+        records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
+        t = C.c_uint64()
+        self._ck(lib().sgc_server_submit(self.h, gid, records.ctypes.data, len(records), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout_ms=60000):
+        self._ck(lib().sgc_server_wait(self.h, ticket, timeout_ms))
+
+    def flush(self, timeout_ms=120000):
+        self._ck(lib().sgc_server_flush(self.h, timeout_ms))
+
+    def query(self, gid, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        out = np.zeros(len(ids), np.uint16)
+        self._ck(lib().sgc_server_query(self.h, gid, ids, len(ids), out))
+        return out
+
+    def stats(self):
+        st = _lib.ServerStats()
+        self._ck(lib().sgc_server_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def latencies(self, which="update"):
+        """Raw latencies in microseconds ("update": update-to-snapshot per request, "query")."""
+        k = 0 if which == "update" else 1
+        n = C.c_uint64()
+        self._ck(lib().sgc_server_latencies(self.h, k, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.float64)
+        self._ck(lib().sgc_server_latencies(self.h, k, out.ctypes.data, n.value, C.byref(n)))
+        return out[:n.value]
+
+    def reset_stats(self):
+        self._ck(lib().sgc_server_reset_stats(self.h))

@@ -60,6 +60,19 @@ class Tuning(C.Structure):
                 ("serial_repair", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_uint32), ("max_delay_us", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("updates_applied", "batches", "epoch_steps", "graph_epochs",
+                                          "requests_done", "queries", "queued_updates")] + \
+               [("span_us", C.c_double), ("busy_us", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class PoolConfig(C.Structure):
     _fields_ = [("max_graphs", C.c_uint32), ("batch_capacity", C.c_uint32), ("flags", C.c_uint32),
                 ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64)]
@@ -138,6 +151,21 @@ SIGNATURES = {
     "sgc_pool_kernel_timing_enable": [_vp, C.c_int],
     "sgc_pool_kernel_times": [_vp, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                               C.c_uint32, C.POINTER(C.c_uint32)],
+    "sgc_default_server_config": [C.POINTER(ServerConfig)],
+    "sgc_server_create": [_vp, C.POINTER(ServerConfig), C.POINTER(_vp)],
+    "sgc_server_destroy": [_vp],
+    "sgc_server_last_error": [_vp],
+    "sgc_server_admit": [_vp, C.POINTER(Config), C.c_uint32, _u32p, _u32p, _u32p, C.POINTER(C.c_uint64)],
+    "sgc_server_admitted": [_vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_uint32)],
+    "sgc_server_attach": [_vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)],
+    "sgc_server_remove": [_vp, C.c_uint32],
+    "sgc_server_submit": [_vp, C.c_uint32, _vp, C.c_uint32, C.POINTER(C.c_uint64)],
+    "sgc_server_wait": [_vp, C.c_uint64, C.c_uint32],
+    "sgc_server_flush": [_vp, C.c_uint32],
+    "sgc_server_query": [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p],
+    "sgc_server_get_stats": [_vp, C.POINTER(ServerStats)],
+    "sgc_server_latencies": [_vp, C.c_int, _vp, C.c_uint64, C.POINTER(C.c_uint64)],
+    "sgc_server_reset_stats": [_vp],
     "sgc_csr_load_mtx": [C.c_char_p, C.POINTER(_vp)],
     "sgc_csr_from_edges": [C.c_uint32, _u32p, _u32p, C.c_uint64, C.POINTER(_vp)],
     "sgc_csr_info": [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
@@ -154,7 +182,8 @@ SIGNATURES = {
     "sgc_edge_intervals": [C.c_uint32, _u32p, _u32p, _vp, C.c_uint64, C.POINTER(C.c_uint64), _vp, _vp, _vp, _vp],
 }
 _RESTYPE = {"sgc_default_config": None, "sgc_default_tuning": None, "sgc_last_error": C.c_char_p, "sgc_version": C.c_char_p,
-            "sgc_pool_last_error": C.c_char_p}
+            "sgc_pool_last_error": C.c_char_p, "sgc_server_last_error": C.c_char_p,
+            "sgc_default_server_config": None}
 
 _lib = None
 

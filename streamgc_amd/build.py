@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libstreamgc_amd.so")
 CLI = os.path.join(HERE, "streaming_gc")
 
 HIP_SOURCES = ["sgc_kernels.hip", "sgc_engine.hip"]
-CXX_SOURCES = ["sgc_host.cpp", "sgc_gen.cpp"]
+CXX_SOURCES = ["sgc_host.cpp", "sgc_gen.cpp", "sgc_server.cpp"]
 CLI_SOURCES = ["sgc_cli.cpp"]
 
 ARCH = os.environ.get("SGC_OFFLOAD_ARCH", "gfx950")
@@ -55,10 +55,10 @@ def build(verbose=False):
         if not os.path.exists(src):
             continue
         o = os.path.join(objdir, s + ".o")
-        _run([hipcc, *COMMON, "-fopenmp", "-x", "c++", "-c", src, "-o", o])
+        _run([hipcc, *COMMON, "-fopenmp", "-pthread", "-x", "c++", "-c", src, "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fopenmp", "-o", tmp, *objs])
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fopenmp", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     cli_src = [os.path.join(CSRC, s) for s in CLI_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     if cli_src:

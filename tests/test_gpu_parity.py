@@ -321,13 +321,17 @@ def test_tracked_sweeps_visit_fewer_rows(graph_path):
     assert full["rows_t2_check"] == 0 and tracked["rows_t2_check"] == 0
 
 
+@pytest.mark.parametrize("overflow", [False, True])
 @pytest.mark.parametrize("name", ["er2k", "delaunay1k", "rmat_s10", "chunglu4k"])
-def test_long_rows_swept_by_blocks_stay_bit_exact(name, graph_path):
+def test_long_rows_swept_by_blocks_stay_bit_exact(name, overflow, graph_path):
     # long_row=16 sends most rows of these graphs down the block-per-row
-    # paths of compaction, rebuild and the Tier-2 check
+    # paths of compaction and rebuild; rows that overflow keep the Tier-2
+    # check running (a clean Tier-1 proves nothing once an edge sits in one
+    # row only), which exercises its block path too
     g = S.load_graph_csr(graph_path(name))
+    extra = dict(slack_pct=0, min_row_capacity=2, overflow_capacity=4096) if overflow else {}
     gpu, orc, og = _pair(g, deterministic=True, epoch_size=120, batch_size=120, repair_every_epoch=True,
-                         exact_histogram=True, histogram_threshold=8)
+                         exact_histogram=True, histogram_threshold=8, **extra)
     gpu.set_tuning(long_row=16, sweep_blocks=7)
     init = S.generate_init_stream(g)
     assert gpu.process_stream(init) == orc.submit(init["u"], init["v"], init["type"])
@@ -338,7 +342,9 @@ def test_long_rows_swept_by_blocks_stay_bit_exact(name, graph_path):
     assert gpu.finish() == orc.finish()
     assert np.array_equal(gpu.snapshot(), orc.snapshot())
     t = gpu.totals()
-    assert t["rows_compact"] > 0 and t["rows_t2_check"] > 0
+    assert t["rows_compact"] > 0
+    if overflow:
+        assert gpu.overflow_count() >= 0 and t["rows_t2_check"] > 0
 
 
 @pytest.mark.parametrize("long_row", [16, 1 << 30])
