@@ -37,7 +37,7 @@ WORKLOADS = {
                kind="chung_lu", n=3997962, m=34681189, gamma=2.3, max_degree=15000.0),
     "c5": dict(name="served: 64 tenants (Delaunay 2^10-2^14, R-MAT s10-14 ef 8-40, Chung-Lu 2^12-2^15, "
                     "R-MAT s12 ef 48) + com-Orkut-shape Chung-Lu (3,072,441 vertices, 117M edges) resident in one "
-                    "pool; churn batches of U[1,8192] updates to uniform tenants; E_g = 4,096",
+                    "pool; churn (50% insert) batches of U[1,8192] updates to uniform tenants; E_g = 4,096",
                kind="served"),
     "c1": dict(name="Delaunay 2^17 uniform points (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
                kind="delaunay", points=1 << 17),
@@ -116,7 +116,8 @@ def main():
                     help="maintenance sweeps visit every vertex instead of the dirty worklists")
     ap.add_argument("--long-row", type=int, default=None, help="tuning: worklist rows longer than this go to a block")
     ap.add_argument("--sweep-blocks", type=int, default=None, help="tuning: grid cap of worklist sweeps")
-    ap.add_argument("--serial-repair", type=int, default=None, help="tuning: 1 = Tier-1 in vertex order")
+    ap.add_argument("--serial-repair", type=int, default=None,
+                    help="tuning: graphs of at most this many vertices run Tier-1 in vertex order")
     ap.add_argument("--requests", type=int, default=2000, help="served workload: requests per phase")
     ap.add_argument("--load", type=float, default=0.5, help="served workload: offered load / served capacity")
     ap.add_argument("--max-delay-us", type=int, default=2000, help="served workload: publish deadline")
@@ -319,8 +320,9 @@ def run_served(S, args, w):
     picks = rng.integers(0, len(tenants), 3 * R)
     sizes = rng.integers(1, 8193, 3 * R)
     need = np.bincount(picks, weights=sizes, minlength=len(tenants)).astype(np.int64)
-    churn = [S.generate_bench_stream_edges(g, int(need[k]) + 1, int(rng.integers(30, 71)), 42 + k)
-             for k, (_, g) in enumerate(tenants)]
+    # 50% inserts (restoring deleted edges) keep every tenant near its loaded
+    # size however many requests it receives
+    churn = [S.generate_bench_stream_edges(g, int(need[k]) + 1, 50, 42 + k) for k, (_, g) in enumerate(tenants)]
     head = [0] * len(tenants)
     reqs = []
     for k, n in zip(picks, sizes):

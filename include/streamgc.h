@@ -115,7 +115,10 @@ typedef struct sgc_tuning {
     uint32_t long_row;      /* worklist rows with more filled slots are swept by a 256-thread block (default 1024) */
     uint32_t sweep_blocks;  /* grid cap of worklist sweeps, in 256-thread blocks (default 2048) */
     uint32_t apply_blocks;  /* grid cap of the per-update launch and of full sweeps (default 16384) */
-    uint32_t serial_repair; /* 1: Tier-1 sweeps in vertex order on one wavefront per graph even in parallel mode */
+    uint32_t serial_repair; /* a graph of at most this many vertices whose last Tier-1 sweep found at most
+                               serial_conflicts conflicts sweeps next in vertex order on one wavefront, in
+                               parallel mode too (default 65536; deterministic mode: every sweep) */
+    uint32_t serial_conflicts; /* default 64 */
 } sgc_tuning;
 void sgc_default_tuning(sgc_tuning* t);
 const char* sgc_last_error(const sgc_engine* e); /* NULL engine: last global error */

@@ -3,9 +3,10 @@
 // Every graph has its own slack CSR, packed vertex words, bitmaps, hub
 // histograms, overflow buffer and double-buffered snapshot (layout below).
 // Every phase of the hot path is ONE launch over all graphs taking part: the
-// kernels walk a virtual vertex space (vbase = prefix of vertex counts) with
-// per-graph active flags, so 256 small graphs cost the same number of
-// launches as one large graph.  The single-graph C ABI (sgc_create ...) is a
+// kernels walk the vertices of the graphs taking part laid end to end (a
+// prefix of their vertex counts and their ids), so 256 small graphs cost the
+// same number of launches as one large graph, and a phase for a few graphs of
+// a large pool touches only their vertices.  The single-graph C ABI (sgc_create ...) is a
 // pool of one.
 //
 // Per-graph layout: word[n] u32, deg0[n] u32, row_ptr[n+1] u32, fill[n] u32, col[S] u32,
@@ -164,12 +165,13 @@ struct Staged {
 
 struct sgc_pool {
     uint32_t max_graphs = 0, batch_cap = 0, flags = 0;
-    sgc_tuning tune{1024, 2048, 16384, 0};
+    sgc_tuning tune{1024, 2048, 16384, 65536, 64};
     uint64_t budget = 0, used_bytes = 0;
     hipStream_t stream = nullptr, qstream = nullptr;
     std::vector<GraphState> gs;
     Staged<GraphDev> graphs;              // descriptors
-    Staged<uint32_t> vbase, hbase, gbase, active;
+    Staged<uint32_t> vbase, gbase, active;   // all graphs: vertex prefix, edge-key offsets, active flags
+    Staged<uint32_t> sbase, sgid;            // the active graphs: vertex prefix and ids
     Staged<uint32_t> perm, seg;           // deterministic per-graph replay
     uint32_t* d_ctr = nullptr;            // max_graphs * C_COUNT
     uint32_t* h_ctr = nullptr;            // pinned copy
@@ -180,7 +182,8 @@ struct sgc_pool {
     uint32_t* d_list = nullptr;           // sweep worklist (virtual vertex ids)
     uint32_t* d_list_count = nullptr;
     uint32_t list_cap = 0;
-    uint32_t ng = 0, V = 0, H = 0;        // slots in use, vertices, hubs
+    uint32_t ng = 0, V = 0;               // slots in use, vertices
+    uint32_t act_n = 0, act_V = 0;        // active graphs and their vertices
     uint16_t* d_tmp16 = nullptr;
     uint32_t* d_qids = nullptr;
     uint16_t* d_qout = nullptr;
@@ -210,12 +213,19 @@ struct sgc_pool {
         L.stream = stream;
         L.graphs = graphs.d;
         L.ng = ng ? ng : 1;
-        L.vbase = ng > 1 ? vbase.d : nullptr;
-        L.V = V;
         L.active = use_active ? active.d : nullptr;
+        if (use_active && act_n != ng) {
+            L.sbase = sbase.d;
+            L.sgid = sgid.d;
+            L.ns = act_n;
+            L.V = act_V;
+        } else {
+            L.sbase = ng > 1 ? vbase.d : nullptr;
+            L.sgid = nullptr;
+            L.ns = ng ? ng : 1;
+            L.V = V;
+        }
         L.gbase = gbase.d;
-        L.hbase = ng > 1 ? hbase.d : nullptr;
-        L.H = H;
         L.totals = d_totals;
         L.max_blocks = tune.apply_blocks;
         L.sweep_blocks = tune.sweep_blocks;
@@ -266,46 +276,57 @@ struct sgc_pool {
     }
     uint32_t ctr(uint32_t gid, uint32_t idx) const { return h_ctr[(size_t)gid * C_COUNT + idx]; }
 
-    // Active flags for a set of graphs (uploaded in stream order).
+    // The graphs taking part in the next launches: active flags and their
+    // vertex space (uploaded in stream order).  A LaunchCtx made with
+    // ctx(true) describes the set current when it was made.
     hipError_t set_active(const std::vector<uint32_t>& gids) {
         hipError_t e = active.writable();
+        if (e == hipSuccess) e = sbase.writable();
+        if (e == hipSuccess) e = sgid.writable();
         if (e != hipSuccess) return e;
         memset(active.h, 0, ng * 4);
         for (uint32_t g : gids) active.h[g] = 1;
-        return active.upload(stream, 0, ng);
+        uint64_t v = 0;
+        act_n = 0;
+        for (uint32_t g = 0; g < ng; g++) {
+            if (!active.h[g]) continue;
+            sbase.h[act_n] = (uint32_t)v;
+            sgid.h[act_n] = g;
+            act_n++;
+            v += gs[g].n;
+        }
+        sbase.h[act_n] = (uint32_t)v;
+        act_V = (uint32_t)v;
+        e = active.upload(stream, 0, ng);
+        if (e == hipSuccess && act_n) e = sbase.upload(stream, 0, act_n + 1);
+        if (e == hipSuccess && act_n) e = sgid.upload(stream, 0, act_n);
+        return e;
     }
     hipError_t set_active_iterating(const std::vector<uint32_t>& gids) {
-        hipError_t e = active.writable();
-        if (e != hipSuccess) return e;
-        memset(active.h, 0, ng * 4);
-        for (uint32_t g : gids) active.h[g] = gs[g].iterating ? 1 : 0;
-        return active.upload(stream, 0, ng);
+        std::vector<uint32_t> it;
+        for (uint32_t g : gids)
+            if (gs[g].iterating) it.push_back(g);
+        return set_active(it);
     }
 
     hipError_t refresh_layout() {
         hipError_t e = vbase.writable();
-        if (e == hipSuccess) e = hbase.writable();
         if (e == hipSuccess) e = gbase.writable();
         if (e != hipSuccess) return e;
         ng = 0;
         for (uint32_t g = 0; g < max_graphs; g++)
             if (gs[g].used) ng = g + 1;
-        uint64_t v = 0, h = 0;
+        uint64_t v = 0;
         for (uint32_t g = 0; g < ng; g++) {
             vbase.h[g] = (uint32_t)v;
-            hbase.h[g] = (uint32_t)h;
             gbase.h[g] = (uint32_t)v;
-            if (gs[g].used) {
-                v += gs[g].n;
-                h += gs[g].nhd;
-            }
+            if (gs[g].used) v += gs[g].n;
         }
         vbase.h[ng] = (uint32_t)v;
-        hbase.h[ng] = (uint32_t)h;
         V = (uint32_t)v;
-        H = (uint32_t)h;
+        act_n = ng;   // every graph until the next set_active
+        act_V = V;
         e = vbase.upload(stream, 0, ng + 1);
-        if (e == hipSuccess) e = hbase.upload(stream, 0, ng + 1);
         if (e == hipSuccess) e = gbase.upload(stream, 0, ng + 1);
         if (e == hipSuccess && V > list_cap) {
             e = hipStreamSynchronize(stream);
@@ -614,10 +635,29 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
         LaunchCtx L = ctx(true);
         if (e == hipSuccess) e = rebuild(L);
         if (e == hipSuccess) e = launch_zero_ctr(L, C_CONFLICTS);
-        LaunchCtx L1 = L;
-        const bool serial = deterministic() || tune.serial_repair;
-        if (e == hipSuccess && !serial) e = sweep(L1, M_CHECK);
-        if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, serial); });
+        // A graph sweeps in vertex order on one wavefront (Gauss-Seidel) when
+        // deterministic, or when it is small and its last sweep left few
+        // conflicts (a racy sweep over a small conflict set can chase the same
+        // conflicts for many iterations); otherwise one wavefront per listed row.
+        std::vector<uint32_t> small, large;
+        for (uint32_t g : gids) {
+            const GraphState& G = gs[g];
+            if (!G.iterating) continue;
+            const bool serial = deterministic() || (G.n <= tune.serial_repair && G.prev <= tune.serial_conflicts);
+            (serial ? small : large).push_back(g);
+        }
+        if (e == hipSuccess && !large.empty()) {
+            if (!small.empty()) e = set_active(large);
+            LaunchCtx L1 = ctx(true);
+            if (e == hipSuccess) e = sweep(L1, M_CHECK);
+            if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, false, true); });
+        }
+        if (e == hipSuccess && !small.empty()) {
+            if (!large.empty()) e = set_active(small);
+            LaunchCtx L1 = ctx(true);
+            if (e == hipSuccess)
+                e = timed("tier1_serial", [&] { return launch_tier1(L1, true, !deterministic()); });
+        }
         if (e == hipSuccess) e = read_ctr();
         if (e != hipSuccess) return hip_fail(e, "tier-1");
         t1_sweeps++;
@@ -688,6 +728,7 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
     }
     if (ran.empty()) return SGC_OK;
     e = set_active(ran);
+    L = ctx(true);
     Lc = L;
     if (e == hipSuccess) e = launch_zero_ctr(L, C_T2BAD);
     if (e == hipSuccess) e = sweep(Lc, M_T2);
@@ -765,6 +806,7 @@ int sgc_pool::epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<
     if (!repair) {
         if ((r = publish(gids))) return r;
         e = set_active(gids);
+        L = ctx(true);
     }
     LaunchCtx Lc = L;
     if (e == hipSuccess) e = sweep(Lc, M_ROW);
@@ -801,7 +843,8 @@ void sgc_pool::release() {
     for (auto& G : gs) G.release();
     graphs.release();
     vbase.release();
-    hbase.release();
+    sbase.release();
+    sgid.release();
     gbase.release();
     active.release();
     perm.release();
@@ -846,7 +889,8 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&P->qstream, hipStreamNonBlocking);
     if (e == hipSuccess) e = P->graphs.alloc(max_graphs);
     if (e == hipSuccess) e = P->vbase.alloc(max_graphs + 1);
-    if (e == hipSuccess) e = P->hbase.alloc(max_graphs + 1);
+    if (e == hipSuccess) e = P->sbase.alloc(max_graphs + 1);
+    if (e == hipSuccess) e = P->sgid.alloc(max_graphs);
     if (e == hipSuccess) e = P->gbase.alloc(max_graphs + 1);
     if (e == hipSuccess) e = P->active.alloc(max_graphs);
     if (e == hipSuccess) e = P->perm.alloc(batch_cap);
@@ -1094,7 +1138,7 @@ int sgc_destroy(sgc_engine* e) {
 }
 
 void sgc_default_tuning(sgc_tuning* t) {
-    if (t) *t = sgc_tuning{1024, 2048, 16384, 0};
+    if (t) *t = sgc_tuning{1024, 2048, 16384, 65536, 64};
 }
 static int set_tuning(sgc_pool* p, const sgc_tuning* t) {
     if (!p || !t || t->sweep_blocks == 0 || t->apply_blocks == 0 || t->sweep_blocks > 65535 ||

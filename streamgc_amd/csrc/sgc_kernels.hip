@@ -2,8 +2,8 @@
 //
 // Work decomposition: updates are one wavefront each (grid-stride), or one
 // wavefront per segment replayed in order (deterministic mode).  Epoch sweeps
-// run one wavefront per vertex row over a virtual vertex space that spans all
-// graphs of a pool (vbase = prefix of vertex counts); rows are streamed 64
+// run one wavefront per vertex row over the vertex space of the graphs taking
+// part (their vertex counts laid end to end); rows are streamed 64
 // slots per load.  The deterministic repair sweep runs one wavefront per
 // graph in vertex order.
 #include <hip/hip_runtime.h>
@@ -16,22 +16,28 @@ namespace sgc {
 __device__ __forceinline__ uint32_t wave_id() { return (blockIdx.x * blockDim.x + threadIdx.x) / WAVE; }
 __device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) / WAVE; }
 
-__device__ __forceinline__ void locate(const uint32_t* __restrict__ vbase, uint32_t ng, uint32_t i, uint32_t& gid,
-                                       uint32_t& v) {
-    if (ng == 1) {
-        gid = 0;
-        v = i;
-        return;
+// The vertex space of a launch: the vertices of the graphs taking part, one
+// graph after another.  A launch over a few graphs of a large pool touches
+// only their vertices.
+struct Space {
+    const uint32_t* base;  // ns + 1 prefix of vertex counts (null: a single graph)
+    const uint32_t* gid;   // graph id of each part (null: part k is graph k)
+    uint32_t ns;
+    uint32_t total;
+    __device__ __forceinline__ void locate(uint32_t i, uint32_t& g, uint32_t& v) const {
+        uint32_t lo = 0;
+        if (base && ns > 1) {
+            uint32_t hi = ns;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (base[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+        }
+        g = gid ? gid[lo] : lo;
+        v = base ? i - base[lo] : i;
     }
-    uint32_t lo = 0, hi = ng;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (vbase[mid] <= i) lo = mid;
-        else hi = mid;
-    }
-    gid = lo;
-    v = i - vbase[lo];
-}
+};
 
 // ---------------------------------------------------------------- sweep sets
 // A maintenance sweep visits either every virtual vertex (list == null, one
@@ -105,16 +111,14 @@ __device__ __forceinline__ uint32_t block_scan32(uint32_t x, uint32_t* total) {
 // are cleared.  One lane per vertex; each wavefront covers LIST_TILE chunks of
 // 64 vertices, and each block appends its hits with one atomic per list end.
 constexpr uint32_t LIST_TILE = 16;
-__global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                    const uint32_t* __restrict__ vbase, uint32_t V,
-                                                    const uint32_t* __restrict__ active, uint32_t kind,
+__global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__ graphs, Space sp, uint32_t kind,
                                                     uint32_t long_row, uint32_t* __restrict__ list, uint32_t cap,
                                                     uint32_t* __restrict__ count) {
     __shared__ uint32_t wave_small[BLOCK / WAVE], wave_big[BLOCK / WAVE], block_at[2];
     const uint64_t lt_mask = (1ull << lane()) - 1ull;
     const uint32_t wv = threadIdx.x / WAVE;
     const uint32_t span = BLOCK * LIST_TILE;
-    for (uint32_t bbase = blockIdx.x * span; bbase < V; bbase += gridDim.x * span) {
+    for (uint32_t bbase = blockIdx.x * span; bbase < sp.total; bbase += gridDim.x * span) {
         const uint32_t base = bbase + wv * WAVE * LIST_TILE;
         uint64_t small_m[LIST_TILE], big_m[LIST_TILE];
         uint32_t nsmall = 0, nbig = 0;
@@ -122,18 +126,16 @@ __global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__
         for (uint32_t t = 0; t < LIST_TILE; t++) {
             const uint32_t i = base + t * WAVE + lane();
             bool hit = false, big = false;
-            if (i < V) {
+            if (i < sp.total) {
                 uint32_t gid, v;
-                locate(vbase, ng, i, gid, v);
-                if (!active || active[gid]) {
-                    const GraphDev& g = graphs[gid];
-                    SGC_G uint8_t* d = gp(g.dirty) + v;
-                    const size_t st = g.dstride;
-                    const bool marked = d[kind * st] != 0;
-                    hit = marked || d[M_STICKY * st] != 0;
-                    if (marked) d[kind * st] = 0;
-                    if (hit) big = gp(g.fill)[v] > long_row;
-                }
+                sp.locate(i, gid, v);
+                const GraphDev& g = graphs[gid];
+                SGC_G uint8_t* d = gp(g.dirty) + v;
+                const size_t st = g.dstride;
+                const bool marked = d[kind * st] != 0;
+                hit = marked || d[M_STICKY * st] != 0;
+                if (marked) d[kind * st] = 0;
+                if (hit) big = gp(g.fill)[v] > long_row;
             }
             small_m[t] = ballot(hit && !big);
             big_m[t] = ballot(big);
@@ -325,14 +327,11 @@ __global__ void k_zero_ctr(const GraphDev* __restrict__ graphs, uint32_t ng, con
 }
 
 // Maximum colour per graph into ctr[C_MAXC] (A/src/streamgc.cu:549-553).
-__global__ __launch_bounds__(256) void k_max_color(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                   const uint32_t* __restrict__ vbase, uint32_t V,
-                                                   const uint32_t* __restrict__ active) {
+__global__ __launch_bounds__(256) void k_max_color(const GraphDev* __restrict__ graphs, Space sp) {
     uint32_t cur = NONE, mx = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sp.total; i += gridDim.x * blockDim.x) {
         uint32_t gid, v;
-        locate(vbase, ng, i, gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(i, gid, v);
         if (gid != cur) {
             if (cur != NONE && mx) max32(&graphs[cur].ctr[C_MAXC], mx);
             cur = gid;
@@ -352,13 +351,10 @@ __global__ __launch_bounds__(256) void k_max_color(const GraphDev* __restrict__ 
 
 // ---------------------------------------------------------------- snapshot
 // A/src/kernels/epoch_maintenance.cu:5-16, into the back buffer.
-__global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                  const uint32_t* __restrict__ vbase, uint32_t V,
-                                                  const uint32_t* __restrict__ active) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ graphs, Space sp) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sp.total; i += gridDim.x * blockDim.x) {
         uint32_t gid, v;
-        locate(vbase, ng, i, gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(i, gid, v);
         const GraphDev& g = graphs[gid];
         gp(g.snap_write)[v] = color_of(gp(g.word)[v]);
     }
@@ -433,16 +429,13 @@ __device__ __forceinline__ uint32_t compact_row_wave(const GraphDev& g, uint32_t
     return w - s;
 }
 
-__global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                 const uint32_t* __restrict__ vbase, uint32_t V,
-                                                 const uint32_t* __restrict__ active, SweepList sl,
+__global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ graphs, Space sp, SweepList sl,
                                                  uint64_t* totals) {
     uint64_t live_total = 0, rows = 0, slots = 0;
     const uint32_t nbig = sweep_big(sl);
     for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
         uint32_t gid, v;
-        locate(vbase, ng, big_item(sl, k), gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(big_item(sl, k), gid, v);
         const GraphDev& g = graphs[gid];
         const uint32_t f = gp(g.fill)[v];
         const uint32_t live = compact_row_block(g, v);
@@ -452,11 +445,10 @@ __global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ gr
             slots += f;
         }
     }
-    const uint32_t total = sweep_small(sl, V);
+    const uint32_t total = sweep_small(sl, sp.total);
     for (uint32_t k = wave_id(); k < total; k += wave_count()) {
         uint32_t gid, v;
-        locate(vbase, ng, small_item(sl, k), gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(small_item(sl, k), gid, v);
         const GraphDev& g = graphs[gid];
         const uint32_t f = gp(g.fill)[v];
         live_total += compact_row_wave(g, v);
@@ -593,17 +585,14 @@ __device__ __forceinline__ void rebuild_bitmap(const GraphDev& g, uint32_t v) {
 // its live row (colours 1..1024) in LDS, then writes them out coalesced.  The
 // reference never rebuilds histograms (A/src/kernels/epoch_maintenance.cu:120-123),
 // so its counts drift; this mode keeps them exact at every rebuild.
-__global__ __launch_bounds__(256) void k_rebuild(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                 const uint32_t* __restrict__ vbase, uint32_t V,
-                                                 const uint32_t* __restrict__ active, SweepList sl,
+__global__ __launch_bounds__(256) void k_rebuild(const GraphDev* __restrict__ graphs, Space sp, SweepList sl,
                                                  int exact_hist, uint64_t* totals) {
     __shared__ uint32_t bins[BLOCK / WAVE][HIST_BINS];
     uint64_t rows = 0, slots = 0;
     const uint32_t nbig = sweep_big(sl);
     for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
         uint32_t gid, v;
-        locate(vbase, ng, big_item(sl, k), gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(big_item(sl, k), gid, v);
         const GraphDev& g = graphs[gid];
         const uint32_t hd = gp(g.hd_map)[v];
         if (hd != INVALID && !exact_hist) continue;
@@ -615,11 +604,10 @@ __global__ __launch_bounds__(256) void k_rebuild(const GraphDev* __restrict__ gr
         else rebuild_hub_block(g, v, hd, &bins[0][0]);
     }
     uint32_t* b = bins[threadIdx.x / WAVE];
-    const uint32_t total = sweep_small(sl, V);
+    const uint32_t total = sweep_small(sl, sp.total);
     for (uint32_t k = wave_id(); k < total; k += wave_count()) {
         uint32_t gid, v;
-        locate(vbase, ng, small_item(sl, k), gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(small_item(sl, k), gid, v);
         const GraphDev& g = graphs[gid];
         const uint32_t hd = gp(g.hd_map)[v];
         if (hd != INVALID && !exact_hist) continue;
@@ -657,19 +645,106 @@ __global__ __launch_bounds__(256) void k_rebuild(const GraphDev* __restrict__ gr
 }
 
 // ---------------------------------------------------------------- tier-1 repair
+// Colours 1..255 present among a row's live neighbours, bit (c & 63) of
+// m[c >> 6].
+struct ColourSet {
+    uint64_t m[4] = {0, 0, 0, 0};
+    __device__ __forceinline__ void add(uint32_t c) {
+        if (c == 0 || c >= 256) return;
+        const uint64_t b = 1ull << (c & 63);
+        const uint32_t q = c >> 6;
+        m[0] |= q == 0 ? b : 0;
+        m[1] |= q == 1 ? b : 0;
+        m[2] |= q == 2 ? b : 0;
+        m[3] |= q == 3 ? b : 0;
+    }
+    __device__ __forceinline__ void wave_reduce() {
+        for (int k = 0; k < 4; k++) m[k] = wave_or64(m[k]);
+    }
+    // find_available_color_bitmap on an exact bitmap: the lowest colour in
+    // 1..255 not present, else 256 (A/include/color_select.cuh:20-52).
+    __device__ __forceinline__ uint16_t lowest_free() const {
+        const uint64_t m0 = m[0] | 1ull;   // colour 0 is not a colour
+        if (~m0) return (uint16_t)__builtin_ctzll(~m0);
+        for (int k = 1; k < 4; k++)
+            if (~m[k]) return (uint16_t)(64 * k + __builtin_ctzll(~m[k]));
+        return 256;
+    }
+};
+
+// Colours 1..1024 present among a hub's live neighbours: 32 words of LDS
+// per wavefront (or per block).  The lowest absent colour is what
+// find_available_color_histogram picks from an exact histogram
+// (A/include/color_select.cuh:59-89); with all 1024 present it falls back to
+// the histogram's minimum.
+constexpr uint32_t PRESENCE_WORDS = HIST_BINS / 32;
+__device__ __forceinline__ void presence_add(uint32_t* p, uint32_t c) {
+    if (c != 0 && c <= HIST_BINS) atomicOr(&p[(c - 1) >> 5], 1u << ((c - 1) & 31));
+}
+// One wavefront reads the 1024 bits: the lowest clear bit, or 0 if none.
+__device__ __forceinline__ uint16_t presence_lowest_free(const uint32_t* p) {
+    const uint32_t word = lane() < PRESENCE_WORDS ? p[lane()] : 0xFFFFFFFFu;
+    const uint64_t open = ballot(word != 0xFFFFFFFFu);
+    if (!open) return 0;
+    const uint32_t k = (uint32_t)__builtin_ctzll(open);
+    const uint32_t wk = (uint32_t)__shfl((int)word, (int)k);
+    return (uint16_t)(32 * k + __builtin_ctz(~wk) + 1);
+}
+
+__device__ __forceinline__ uint16_t fresh_pick(const GraphDev& g, uint32_t v, bool hub, ColourSet& cs,
+                                               const uint32_t* pres) {
+    if (!hub) {
+        cs.wave_reduce();
+        return cs.lowest_free();
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint16_t c = presence_lowest_free(pres);
+    return c ? c : fas_hist(g, gp(g.hd_map)[v]);
+}
+
 // A/src/kernels/conflict_repair.cu:10-114 for one vertex; returns 1 if counted.
-__device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) {
+// kFresh (parallel mode): a loser selects from the exact colours of its
+// neighbours, gathered by the same row scan that found the conflict -- the
+// bitmap or histogram a rebuild at this instant would leave -- instead of
+// the ones rebuilt before the sweep, which miss the recolours made earlier in
+// the same sweep (the reference does not propagate them,
+// A/src/kernels/conflict_repair.cu).  Deterministic mode keeps the
+// reference's selection.  pres: this wavefront's PRESENCE_WORDS of LDS.
+template <bool kFresh>
+__device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v, uint32_t* pres) {
     const uint32_t w = bcast0(ld(&g.word[v]));
     const uint16_t mine = color_of(w);
     const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
+    const bool hub = gp(g.hd_map)[v] != INVALID;
+    const bool fresh = kFresh;
+    ColourSet cs;
+    if (fresh && hub) {
+        if (lane() < PRESENCE_WORDS) pres[lane()] = 0;
+        __builtin_amdgcn_wave_barrier();
+    }
     if (mine == 0) {
         bool has_nb = false;
-        for (uint32_t base = s; base < e && !has_nb; base += WAVE) {
-            const uint32_t j = base + lane();
-            has_nb = ballot(j < e && ld(&g.col[j]) != INVALID) != 0;
+        for (uint32_t base = s; base < e && (fresh || !has_nb); base += ROW_UNROLL * WAVE) {
+            uint32_t nb[ROW_UNROLL];
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+                const uint32_t j = base + u * WAVE + lane();
+                nb[u] = j < e ? ld(&g.col[j]) : INVALID;
+            }
+            bool any = false;
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+                any |= nb[u] != INVALID;
+                if (fresh && nb[u] != INVALID && nb[u] < g.n) {
+                    const uint32_t cn = color_of(ld(&g.word[nb[u]]));
+                    if (hub) presence_add(pres, cn);
+                    else cs.add(cn);
+                }
+            }
+            has_nb |= ballot(any) != 0;
         }
         if (!has_nb) return 0;
-        const uint16_t c = find_color(g, v);
+        const uint16_t c = fresh ? fresh_pick(g, v, hub, cs, pres) : find_color(g, v);
         if (lane() == 0) {
             st(&g.word[v], bump(w, c));
             mark(g, v, D_BITS | D_STATUS);
@@ -679,7 +754,7 @@ __device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) 
     }
     const uint32_t my_deg = gp(g.deg0)[v];
     bool need = false;
-    for (uint32_t base = s; base < e && !need; base += ROW_UNROLL * WAVE) {
+    for (uint32_t base = s; base < e && (fresh || !need); base += ROW_UNROLL * WAVE) {
         uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
 #pragma unroll
         for (uint32_t u = 0; u < ROW_UNROLL; u++) {
@@ -692,16 +767,24 @@ __device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) 
         bool hit = false;
 #pragma unroll
         for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            if (fresh) {
+                if (hub) presence_add(pres, c[u]);
+                else cs.add(c[u]);
+            }
             if (c[u] == mine) {
                 const uint32_t d = gp(g.deg0)[nb[u]];
                 hit |= (my_deg < d) || (my_deg == d && v < nb[u]);  // pi(v) < pi(nb)
             }
         }
-        need = ballot(hit) != 0;
+        need |= ballot(hit) != 0;
     }
     if (!need) return 0;
     if (mine <= 64 && lane() == 0) or64(&g.bitmap[v], 1ull << (mine - 1));
-    const uint16_t c = find_color(g, v);
+    if (fresh && !hub) {   // own colour OR-ed in (A/src/kernels/conflict_repair.cu:105-107)
+        if (mine < 64) cs.m[0] |= 1ull << mine;
+        if (mine == 64) cs.m[1] |= 1ull;
+    }
+    const uint16_t c = fresh ? fresh_pick(g, v, hub, cs, pres) : find_color(g, v);
     if (lane() == 0) {
         st(&g.word[v], bump(w, c));
         mark(g, v, D_BITS | D_STATUS);   // its own colour bit was OR-ed in
@@ -714,11 +797,19 @@ __device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v) 
 // OR; the first wavefront selects and stores; the block marks the neighbours.
 __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32_t v) {
     __shared__ uint32_t pick;
+    __shared__ uint64_t sets[BLOCK / WAVE][4];
+    __shared__ uint32_t pres[PRESENCE_WORDS];
     const uint32_t w = ld(&g.word[v]);
     const uint16_t mine = color_of(w);
     const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
     const uint32_t my_deg = gp(g.deg0)[v];
+    const bool hub = gp(g.hd_map)[v] != INVALID;   // parallel mode only: fresh selection, see tier1_vertex
+    ColourSet cs;
     uint32_t hit = 0;
+    if (hub) {
+        if (threadIdx.x < PRESENCE_WORDS) pres[threadIdx.x] = 0;
+        __syncthreads();
+    }
     for (uint32_t base = s; base < e; base += ROW_UNROLL * BLOCK) {
         uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
 #pragma unroll
@@ -726,26 +817,42 @@ __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32
             const uint32_t j = base + u * BLOCK + threadIdx.x;
             nb[u] = j < e ? ld(&g.col[j]) : INVALID;
         }
-        if (mine == 0) {
-#pragma unroll
-            for (uint32_t u = 0; u < ROW_UNROLL; u++) hit |= nb[u] != INVALID;
-            continue;
-        }
 #pragma unroll
         for (uint32_t u = 0; u < ROW_UNROLL; u++)
             c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(ld(&g.word[nb[u]])) : 0;
 #pragma unroll
         for (uint32_t u = 0; u < ROW_UNROLL; u++) {
-            if (c[u] == mine) {
+            if (hub) presence_add(pres, c[u]);
+            else cs.add(c[u]);
+            if (mine == 0) {
+                hit |= nb[u] != INVALID;
+            } else if (c[u] == mine) {
                 const uint32_t d = gp(g.deg0)[nb[u]];
                 hit |= (my_deg < d) || (my_deg == d && v < nb[u]);
             }
         }
     }
-    if (block_sum32(hit) == 0) return 0;
+    if (block_sum32(hit) == 0) return 0;   // (its barrier also completes the presence bits)
+    if (!hub) {
+        cs.wave_reduce();
+        if (lane() == 0)
+            for (int k = 0; k < 4; k++) sets[threadIdx.x / WAVE][k] = cs.m[k];
+        __syncthreads();
+    }
     if (threadIdx.x < WAVE) {
         if (mine != 0 && mine <= 64 && lane() == 0) or64(&g.bitmap[v], 1ull << (mine - 1));
-        const uint16_t c = find_color(g, v);
+        uint16_t c;
+        if (!hub) {
+            ColourSet all;
+            for (uint32_t wv = 0; wv < BLOCK / WAVE; wv++)
+                for (int k = 0; k < 4; k++) all.m[k] |= sets[wv][k];
+            if (mine != 0 && mine < 64) all.m[0] |= 1ull << mine;
+            if (mine == 64) all.m[1] |= 1ull;
+            c = all.lowest_free();
+        } else {
+            c = presence_lowest_free(pres);
+            if (!c) c = fas_hist(g, gp(g.hd_map)[v]);
+        }
         if (lane() == 0) {
             st(&g.word[v], bump(w, c));
             mark(g, v, D_BITS | D_STATUS);
@@ -775,16 +882,14 @@ __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32
     return 1;
 }
 
-__global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                    const uint32_t* __restrict__ vbase, uint32_t V,
-                                                    const uint32_t* __restrict__ active, SweepList sl,
+__global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__ graphs, Space sp, SweepList sl,
                                                     uint64_t* totals) {
+    __shared__ uint32_t presence[BLOCK / WAVE][PRESENCE_WORDS];
     uint64_t rows = 0, slots = 0;
     const uint32_t nbig = sweep_big(sl);
     for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
         uint32_t gid, v;
-        locate(vbase, ng, big_item(sl, k), gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(big_item(sl, k), gid, v);
         const GraphDev& g = graphs[gid];
         const uint32_t f = gp(g.fill)[v];
         const uint32_t r = tier1_vertex_block(g, v);
@@ -795,11 +900,10 @@ __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__
         }
     }
     uint32_t cnt = 0, cur_gid = 0xFFFFFFFFu;
-    const uint32_t total = sweep_small(sl, V);
+    const uint32_t total = sweep_small(sl, sp.total);
     for (uint32_t k = wave_id(); k < total; k += wave_count()) {
         uint32_t gid, v;
-        locate(vbase, ng, small_item(sl, k), gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(small_item(sl, k), gid, v);
         if (gid != cur_gid) {
             if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
             cnt = 0;
@@ -808,7 +912,7 @@ __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__
         const GraphDev& g = graphs[gid];
         rows++;
         slots += gp(g.fill)[v];
-        cnt += tier1_vertex(g, v);
+        cnt += tier1_vertex<true>(g, v, presence[threadIdx.x / WAVE]);
     }
     if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
     if (lane() != 0) rows = slots = 0;
@@ -822,36 +926,40 @@ __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__
 // check, since a recolour marks later vertices of the same 64-vertex chunk.
 // A vertex whose bit is clear would be found conflict-free, so the sweep
 // changes exactly what the full sweep changes.
-__device__ __forceinline__ uint32_t mark_byte(const GraphDev& g, uint32_t kind, uint32_t v) {
-    const uint32_t w = ld((const uint32_t*)(g.dirty + (size_t)kind * g.dstride + (v & ~3u)));
-    return (w >> (8 * (v & 3u))) & 0xFFu;
-}
-
+template <bool kFresh>
 __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict__ graphs, uint32_t ng,
                                                      const uint32_t* __restrict__ active, uint64_t* totals) {
+    __shared__ uint32_t presence[PRESENCE_WORDS];
     uint64_t rows = 0;
     for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
         if (active && !active[gid]) continue;
         const GraphDev g = graphs[gid];
         uint32_t cnt = 0;
         if (!g.dirty) {
-            for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex(g, v);
+            for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex<kFresh>(g, v, presence);
             rows += g.n;
         } else {
-            for (uint32_t base = 0; base < g.n; base += WAVE) {
-                const uint32_t v = base + lane();
-                const bool sticky = v < g.n && gp(g.dirty)[(size_t)M_STICKY * g.dstride + v];
-                uint32_t from = 0;
+            // 256 vertices per step: each lane reads the marks of 4 as one dword
+            const size_t cst = (size_t)M_CHECK * g.dstride, sst = (size_t)M_STICKY * g.dstride;
+            for (uint32_t base = 0; base < g.n; base += 4 * WAVE) {
+                const uint32_t v0 = base + 4 * lane();
+                const uint32_t sticky = v0 < g.n ? *gp((const uint32_t*)(g.dirty + sst + v0)) : 0u;
+                uint32_t from = 0;   // chunk offset of the next vertex the sweep may visit
                 for (;;) {
-                    const bool hit = lane() >= from && v < g.n && (sticky || mark_byte(g, M_CHECK, v));
-                    const uint64_t m = ballot(hit);
-                    if (!m) break;
-                    const uint32_t k = (uint32_t)__builtin_ctzll(m);
-                    const uint32_t u = base + k;
-                    if (lane() == k) gp(g.dirty)[(size_t)M_CHECK * g.dstride + u] = 0;
-                    cnt += tier1_vertex(g, u);
+                    const uint32_t w = (v0 < g.n ? ld((const uint32_t*)(g.dirty + cst + v0)) : 0u) | sticky;
+                    uint32_t m4 = 0;
+#pragma unroll
+                    for (uint32_t q = 0; q < 4; q++)
+                        if (((w >> (8 * q)) & 0xFFu) && v0 + q < g.n && 4 * lane() + q >= from) m4 |= 1u << q;
+                    const uint64_t any = ballot(m4 != 0);
+                    if (!any) break;
+                    const uint32_t k = (uint32_t)__builtin_ctzll(any);
+                    const uint32_t q = (uint32_t)__builtin_ctz((uint32_t)__shfl((int)m4, (int)k));
+                    const uint32_t u = base + 4 * k + q;
+                    if (lane() == 0) gp(g.dirty)[cst + u] = 0;
+                    cnt += tier1_vertex<kFresh>(g, u, presence);
                     rows++;
-                    from = k + 1;
+                    from = 4 * k + q + 1;
                 }
             }
         }
@@ -892,16 +1000,13 @@ __device__ __forceinline__ uint32_t t2_row(const GraphDev& g, uint32_t v) {
 // endpoints of one colour (A/src/streamgc.cu:1190-1218 then change nothing).
 // A vertex found with a bad slot keeps its mark, so the next check counts it
 // again until a repair changes it.
-__global__ __launch_bounds__(256) void k_t2_check(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                  const uint32_t* __restrict__ vbase, uint32_t V,
-                                                  const uint32_t* __restrict__ active, SweepList sl,
+__global__ __launch_bounds__(256) void k_t2_check(const GraphDev* __restrict__ graphs, Space sp, SweepList sl,
                                                   uint64_t* totals) {
     uint64_t rows = 0, slots = 0;
     const uint32_t nbig = sweep_big(sl);
     for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
         uint32_t gid, v;
-        locate(vbase, ng, big_item(sl, k), gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(big_item(sl, k), gid, v);
         const GraphDev& g = graphs[gid];
         const uint32_t bad = t2_row<true>(g, v);
         if (threadIdx.x == 0) {
@@ -913,11 +1018,10 @@ __global__ __launch_bounds__(256) void k_t2_check(const GraphDev* __restrict__ g
             }
         }
     }
-    const uint32_t total = sweep_small(sl, V);
+    const uint32_t total = sweep_small(sl, sp.total);
     for (uint32_t k = wave_id(); k < total; k += wave_count()) {
         uint32_t gid, v;
-        locate(vbase, ng, small_item(sl, k), gid, v);
-        if (active && !active[gid]) continue;
+        sp.locate(small_item(sl, k), gid, v);
         const GraphDev& g = graphs[gid];
         const uint32_t bad = t2_row<false>(g, v);
         rows++;
@@ -1015,6 +1119,8 @@ __global__ __launch_bounds__(256) void k_fill_rows(uint32_t n, const uint32_t* _
 // ---------------------------------------------------------------- launchers
 namespace sgc {
 
+static inline Space space(const LaunchCtx& L) { return Space{L.sbase, L.sgid, L.ns, L.V}; }
+
 static inline uint32_t grid_for_waves(uint64_t waves, uint32_t waves_per_block, uint32_t cap) {
     uint64_t b = (waves + waves_per_block - 1) / waves_per_block;
     if (b < 1) b = 1;
@@ -1057,12 +1163,12 @@ hipError_t launch_zero_ctr(const LaunchCtx& L, uint32_t idx) {
 }
 hipError_t launch_max_color(const LaunchCtx& L) {
     uint32_t grid = grid_for_waves(L.V, 4 * WAVE, 1024);
-    k_max_color<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
+    k_max_color<<<grid, 256, 0, L.stream>>>(L.graphs, space(L));
     return hipGetLastError();
 }
 hipError_t launch_snapshot(const LaunchCtx& L) {
     uint32_t grid = grid_for_waves(L.V, 4 * WAVE, L.max_blocks);
-    k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active);
+    k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, space(L));
     return hipGetLastError();
 }
 // Sweep grids: one wavefront per row (blocks first take the long rows of a
@@ -1076,12 +1182,12 @@ hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, 
     if (kind >= M_STICKY || cap < L.V) return hipErrorInvalidValue;
     k_zero_list_count<<<1, 1, 0, L.stream>>>(count);
     uint32_t grid = grid_for_waves((L.V + WAVE * LIST_TILE - 1) / (WAVE * LIST_TILE), BLOCK / WAVE, 4096);
-    k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, kind, L.long_row, list, cap,
+    k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, list, cap,
                                              count);
     return hipGetLastError();
 }
 hipError_t launch_compact(const LaunchCtx& L) {
-    k_compact<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, sweep_list(L), L.totals);
+    k_compact<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L), L.totals);
     return hipGetLastError();
 }
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
@@ -1095,21 +1201,23 @@ hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
     return hipGetLastError();
 }
 hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist) {
-    k_rebuild<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, sweep_list(L),
+    k_rebuild<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L),
                                                    exact_hist ? 1 : 0, L.totals);
     return hipGetLastError();
 }
-hipError_t launch_tier1(const LaunchCtx& L, bool serial) {
+hipError_t launch_tier1(const LaunchCtx& L, bool serial, bool fresh) {
     if (serial) {
-        k_tier1_serial<<<L.ng < 1024 ? L.ng : 1024, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, L.totals);
+        const uint32_t grid = L.ng < 1024 ? L.ng : 1024;
+        if (fresh) k_tier1_serial<true><<<grid, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, L.totals);
+        else k_tier1_serial<false><<<grid, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, L.totals);
     } else {
-        k_tier1_elem<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, sweep_list(L),
+        k_tier1_elem<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L),
                                                           L.totals);
     }
     return hipGetLastError();
 }
 hipError_t launch_t2_check(const LaunchCtx& L) {
-    k_t2_check<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, L.ng, L.vbase, L.V, L.active, sweep_list(L),
+    k_t2_check<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L),
                                                     L.totals);
     return hipGetLastError();
 }

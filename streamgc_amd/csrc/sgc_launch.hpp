@@ -7,18 +7,19 @@ namespace sgc {
 
 struct GraphDev;
 
-// One launch context: the graph descriptors (device array), the virtual
-// vertex space of the pool, optional per-graph active flags and totals.
+// One launch context: the graph descriptors (device array), per-graph
+// active flags (graph-indexed kernels), the vertex space of the graphs taking
+// part (vertex-indexed kernels) and totals.
 struct LaunchCtx {
     hipStream_t stream = nullptr;
     const GraphDev* graphs = nullptr;  // device array of ng descriptors
     uint32_t ng = 1;
-    const uint32_t* vbase = nullptr;   // device, ng + 1 prefix of vertex counts (null when ng == 1)
-    uint32_t V = 0;                    // total vertices
     const uint32_t* active = nullptr;  // device, ng flags (null = all)
+    const uint32_t* sbase = nullptr;   // device, ns + 1 prefix of vertex counts of the graphs taking part
+    const uint32_t* sgid = nullptr;    // device, their graph ids (null = 0..ns-1)
+    uint32_t ns = 1;
+    uint32_t V = 0;                    // vertices of the graphs taking part
     const uint32_t* gbase = nullptr;   // device, ng global-vertex offsets (pool edge keys)
-    const uint32_t* hbase = nullptr;   // device, ng + 1 prefix of hub counts (null when ng == 1)
-    uint32_t H = 0;                    // total hubs
     uint64_t* totals = nullptr;        // device u64 counters
     uint32_t max_blocks = 16384;       // grid cap of per-update and full-sweep launches
     uint32_t sweep_blocks = 2048;      // grid cap of worklist sweeps
@@ -48,7 +49,9 @@ hipError_t launch_compact(const LaunchCtx& L);
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
 hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count);
 hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist);
-hipError_t launch_tier1(const LaunchCtx& L, bool serial);
+// serial: vertex order, one wavefront per graph; fresh: parallel-mode colour
+// selection from the exact neighbour colours (see tier1_vertex).
+hipError_t launch_tier1(const LaunchCtx& L, bool serial, bool fresh);
 hipError_t launch_t2_check(const LaunchCtx& L);
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors);
 hipError_t launch_find_color(const LaunchCtx& L, uint32_t gid, uint32_t v, uint16_t* out);

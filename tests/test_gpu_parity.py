@@ -174,15 +174,19 @@ def _proper_against_live(og, colors, u=None, v=None, t=None):
     return O.validate_edges(og.n, lu, lv, colors)
 
 
+@pytest.mark.parametrize("t1", ["parallel", "serial"])
 @pytest.mark.parametrize("repair", [False, True])
 @pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("name", ["er2k", "chunglu4k", "rmat_s10", "delaunay1k"])
-def test_parallel_repair_is_proper_and_close(name, exact, repair, graph_path):
+def test_parallel_repair_is_proper_and_close(name, exact, repair, t1, graph_path):
     # batches of 64 on graphs of 1K-4K vertices: 3-12% of the vertices are
     # endpoints of one batch, 2-8x the endpoint density of a 32K batch on the
     # 4M-vertex power-law graph
     g = S.load_graph_csr(graph_path(name))
     gpu, orc, og = _pair(g, epoch_size=256, batch_size=64, exact_histogram=exact, repair_every_epoch=repair)
+    # small graphs repair in vertex order by default; "parallel" forces the racy sweep
+    gpu.set_tuning(serial_repair=0 if t1 == "parallel" else 1 << 20,
+                   serial_conflicts=0 if t1 == "parallel" else 0xFFFFFFFF)
     init = S.generate_init_stream(g)
     gpu.process_stream(init)
     orc.submit(init["u"], init["v"], init["type"])
@@ -352,7 +356,7 @@ def test_parallel_long_row_repair_is_proper(long_row, graph_path):
     g = S.load_graph_csr(graph_path("chunglu4k"))
     gpu, orc, og = _pair(g, epoch_size=256, batch_size=256, exact_histogram=True, repair_every_epoch=True,
                          histogram_threshold=8)
-    gpu.set_tuning(long_row=long_row)
+    gpu.set_tuning(long_row=long_row, serial_repair=0)
     init = S.generate_init_stream(g)
     gpu.process_stream(init)
     gpu.finish()
