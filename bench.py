@@ -83,6 +83,21 @@ def kernel_bytes(name, launches, n, tot, live_frac):
     return 0
 
 
+ROCPROF_NAMES = {"apply": "k_apply_elem", "group": "k_group_link", "rebuild": "k_rebuild", "tier1": "k_tier1_elem",
+                 "compact": "k_compact", "build_list": "k_build_list", "snapshot": "k_snapshot"}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-kernel calibrated), if any."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    t = json.load(open(path))
+    b = t["bytes_per_launch"].get(ROCPROF_NAMES.get(kernel, kernel))
+    return (int(b), t["source"]) if b else (None, None)
+
+
 def cpu_baseline(g, state, bench, E, batches, threads):
     """The oracle (sequential updates, OpenMP sweeps) on the same batches from the same state."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -116,14 +131,19 @@ def main():
                     help="maintenance sweeps visit every vertex instead of the dirty worklists")
     ap.add_argument("--long-row", type=int, default=None, help="tuning: worklist rows longer than this go to a block")
     ap.add_argument("--sweep-blocks", type=int, default=None, help="tuning: grid cap of worklist sweeps")
+    ap.add_argument("--fresh-repair", type=int, default=None, help="tuning: 0/1/2 exact-colour Tier-1 selection")
     ap.add_argument("--serial-repair", type=int, default=None,
                     help="tuning: graphs of at most this many vertices run Tier-1 in vertex order")
+    ap.add_argument("--profile-summary", default=None, metavar="DIR",
+                    help="summarise the rocprofv3 output under DIR (timed window = the last --steps steps) and exit")
     ap.add_argument("--requests", type=int, default=2000, help="served workload: requests per phase")
     ap.add_argument("--load", type=float, default=0.5, help="served workload: offered load / served capacity")
     ap.add_argument("--max-delay-us", type=int, default=2000, help="served workload: publish deadline")
     ap.add_argument("--init-batch", type=int, default=256,
                     help="apply the (untimed) init stream in launches of this many updates, epochs still every E")
     args = ap.parse_args()
+    if args.profile_summary:
+        return profile_summary(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this "
               f"build", file=sys.stderr)
@@ -149,7 +169,8 @@ def main():
     eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True,
                                              full_sweeps=args.full_sweeps))
     knobs = {k: v for k, v in (("long_row", args.long_row), ("sweep_blocks", args.sweep_blocks),
-                               ("serial_repair", args.serial_repair)) if v is not None}
+                               ("serial_repair", args.serial_repair), ("fresh_repair", args.fresh_repair))
+             if v is not None}
     if knobs:
         eng.set_tuning(**knobs)
     t = time.perf_counter()
@@ -203,9 +224,10 @@ def main():
                                 gbs=round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 and b else None)
     dom = max(per_kernel, key=lambda k: per_kernel[k]["total_ms"])
     pk = per_kernel[dom]
+    traffic, tsrc = measured_traffic(args.workload, dom)
     roofline = dict(bound="hbm", kernel=dom, achieved=pk["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(pk["gbs"] / HBM_PEAK_GBS, 4) if pk["gbs"] else None, traffic=None,
-                    bytes_per_launch=pk["bytes_per_launch"], avg_launch_us=pk["avg_us"])
+                    frac=round(pk["gbs"] / HBM_PEAK_GBS, 4) if pk["gbs"] else None, traffic=traffic,
+                    traffic_source=tsrc, bytes_per_launch=pk["bytes_per_launch"], avg_launch_us=pk["avg_us"])
 
     parity = {"epochs_timed": len(epochs), "t2_remaining_max": max(e["t2_remaining"] for e in epochs),
               "max_color_final": epochs[-1]["max_color"], "t1_iterations_mean":
@@ -255,6 +277,57 @@ def main():
         "tuning": knobs,
     }
     print(json.dumps(out))
+
+
+
+
+# ---------------------------------------------------------------- profile summaries
+def _short(name):
+    return name.split("(")[0].replace("sgc::", "").replace("void ", "").strip()
+
+
+def _timed_window(rows, steps):
+    """Rows (dicts with Kernel_Name, ordered by dispatch) of the last `steps`
+    steps: from the steps-th last per-update launch to the end of the run."""
+    idx = [i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == "k_apply_elem"]
+    return rows[idx[-steps]:] if len(idx) >= steps else rows
+
+
+def profile_summary(args):
+    """Summaries of the rocprofv3 output of a bench run: the kernel-trace
+    statistics over the timed steps, and per-dispatch FETCH_SIZE /
+    WRITE_SIZE (KiB) over the same window scaled by what the counters read
+    for a copy kernel of known size."""
+    import csv
+    out = {}
+    d = args.profile_summary
+    trace = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("kernel_trace.csv")]
+    if trace:
+        rows = sorted(csv.DictReader(open(trace[0])), key=lambda r: int(r["Start_Timestamp"]))
+        win = _timed_window(rows, args.steps)
+        per = {}
+        for r in win:
+            k = _short(r["Kernel_Name"])
+            per.setdefault(k, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        out["timed_window"] = {k: {"calls": len(v), "total_us": round(sum(v), 2), "avg_us": round(sum(v) / len(v), 2),
+                                   "min_us": round(min(v), 2), "max_us": round(max(v), 2)}
+                               for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))}
+        out["dispatches_in_trace"] = len(rows)
+    for tag in ("FETCH_SIZE", "WRITE_SIZE"):
+        files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs
+                 if f.endswith("counter_collection.csv") and tag.lower().split("_")[0] in r.lower()]
+        for f in files:
+            rows = list(csv.DictReader(open(f)))
+            rows = [r for r in rows if r.get("Counter_Name") == tag]
+            rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+            kind = "copy" if "copy" in f else "bench"
+            win = rows if kind == "copy" else _timed_window(rows, args.steps)
+            per = {}
+            for r in win:
+                per.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+            out.setdefault(kind, {})[tag] = {k: {"dispatches": len(v), "kib_avg": round(sum(v) / len(v), 1)}
+                                             for k, v in per.items()}
+    print(json.dumps(out, indent=1))
 
 
 

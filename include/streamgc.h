@@ -119,6 +119,9 @@ typedef struct sgc_tuning {
                                serial_conflicts conflicts sweeps next in vertex order on one wavefront, in
                                parallel mode too (default 65536; deterministic mode: every sweep) */
     uint32_t serial_conflicts; /* default 64 */
+    uint32_t fresh_repair;  /* parallel-mode Tier-1 selects from the exact colours of the loser's neighbours
+                               gathered by its conflict scan: 0 never (the reference's bitmap/histogram),
+                               1 bitmap-mode rows, 2 all rows (default 2) */
 } sgc_tuning;
 void sgc_default_tuning(sgc_tuning* t);
 const char* sgc_last_error(const sgc_engine* e); /* NULL engine: last global error */

@@ -60,7 +60,7 @@ def make_tuning(**knobs):
     t = _lib.Tuning()
     lib().sgc_default_tuning(C.byref(t))
     for k, v in knobs.items():
-        if k not in ("long_row", "sweep_blocks", "apply_blocks", "serial_repair", "serial_conflicts"):
+        if k not in ("long_row", "sweep_blocks", "apply_blocks", "serial_repair", "serial_conflicts", "fresh_repair"):
             raise TypeError(f"unknown tuning knob {k}")
         setattr(t, k, int(v))
     return t

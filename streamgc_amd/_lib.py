@@ -57,7 +57,7 @@ class Totals(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("long_row", C.c_uint32), ("sweep_blocks", C.c_uint32), ("apply_blocks", C.c_uint32),
-                ("serial_repair", C.c_uint32), ("serial_conflicts", C.c_uint32)]
+                ("serial_repair", C.c_uint32), ("serial_conflicts", C.c_uint32), ("fresh_repair", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

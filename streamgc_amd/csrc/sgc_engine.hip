@@ -165,7 +165,7 @@ struct Staged {
 
 struct sgc_pool {
     uint32_t max_graphs = 0, batch_cap = 0, flags = 0;
-    sgc_tuning tune{1024, 2048, 16384, 65536, 64};
+    sgc_tuning tune{1024, 2048, 16384, 65536, 64, 2};
     uint64_t budget = 0, used_bytes = 0;
     hipStream_t stream = nullptr, qstream = nullptr;
     std::vector<GraphState> gs;
@@ -201,6 +201,7 @@ struct sgc_pool {
     bool exact_hist() const { return (flags & SGC_EXACT_HISTOGRAM) != 0; }
     bool repair_every_epoch() const { return (flags & SGC_REPAIR_EVERY_EPOCH) != 0; }
     bool tracking() const { return (flags & SGC_FULL_SWEEPS) == 0; }
+    bool rebuild_each_sweep() const { return deterministic() || tune.fresh_repair < 2; }
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -590,7 +591,8 @@ int sgc_pool::apply(const sgc_edge_update* upd, uint32_t count, bool on_device, 
         if (e == hipSuccess)
             e = timed("apply", [&] { return launch_apply_seg(L, src, seg.d, nseg, multi ? perm.d : nullptr, multi); });
     } else {
-        e = timed("apply", [&] { return launch_apply(L, src, count, multi, &eg); });
+        e = timed("group", [&] { return launch_group(L, src, count, multi, &eg); });
+        if (e == hipSuccess) e = timed("apply", [&] { return launch_apply(L, src, count, multi, &eg); });
     }
     if (e != hipSuccess) return hip_fail(e, "apply launch");
     if (!on_device) {
@@ -633,7 +635,11 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
         if (!any) break;
         hipError_t e = set_active_iterating(gids);
         LaunchCtx L = ctx(true);
-        if (e == hipSuccess) e = rebuild(L);
+        // The reference rebuilds before every sweep (A/src/streamgc.cu:1317-1393).
+        // A parallel-mode sweep that selects from exact neighbour colours for
+        // every row (fresh_repair 2) reads no bitmap or histogram, so there the
+        // rebuild runs once, after the repair.
+        if (e == hipSuccess && rebuild_each_sweep()) e = rebuild(L);
         if (e == hipSuccess) e = launch_zero_ctr(L, C_CONFLICTS);
         // A graph sweeps in vertex order on one wavefront (Gauss-Seidel) when
         // deterministic, or when it is small and its last sweep left few
@@ -650,13 +656,14 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
             if (!small.empty()) e = set_active(large);
             LaunchCtx L1 = ctx(true);
             if (e == hipSuccess) e = sweep(L1, M_CHECK);
-            if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, false, true); });
+            if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, false, (int)tune.fresh_repair); });
         }
         if (e == hipSuccess && !small.empty()) {
             if (!large.empty()) e = set_active(small);
             LaunchCtx L1 = ctx(true);
             if (e == hipSuccess)
-                e = timed("tier1_serial", [&] { return launch_tier1(L1, true, !deterministic()); });
+                e = timed("tier1_serial",
+                          [&] { return launch_tier1(L1, true, deterministic() ? 0 : (int)tune.fresh_repair); });
         }
         if (e == hipSuccess) e = read_ctr();
         if (e != hipSuccess) return hip_fail(e, "tier-1");
@@ -818,12 +825,18 @@ int sgc_pool::epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<
     if (repair || out)
         if ((r = read_drain(gids))) return r;
     if (repair) {
-        if (finish) {
+        if (finish && rebuild_each_sweep()) {
             e = rebuild(L);
             if (e != hipSuccess) return hip_fail(e, "rebuild");
         }
         if ((r = tier1(gids))) return r;
         if ((r = tier2(gids))) return r;
+        if (!deterministic()) {   // bitmaps and histograms exact again for the next updates
+            e = set_active(gids);
+            L = ctx(true);
+            if (e == hipSuccess) e = rebuild(L);
+            if (e != hipSuccess) return hip_fail(e, "rebuild");
+        }
         if ((r = publish(gids))) return r;
     } else {
         e = rebuild(L);
@@ -1138,11 +1151,11 @@ int sgc_destroy(sgc_engine* e) {
 }
 
 void sgc_default_tuning(sgc_tuning* t) {
-    if (t) *t = sgc_tuning{1024, 2048, 16384, 65536, 64};
+    if (t) *t = sgc_tuning{1024, 2048, 16384, 65536, 64, 2};
 }
 static int set_tuning(sgc_pool* p, const sgc_tuning* t) {
     if (!p || !t || t->sweep_blocks == 0 || t->apply_blocks == 0 || t->sweep_blocks > 65535 ||
-        t->apply_blocks > 65535)
+        t->apply_blocks > 65535 || t->fresh_repair > 2)
         return SGC_ERR_INVALID;
     p->tune = *t;
     return SGC_OK;

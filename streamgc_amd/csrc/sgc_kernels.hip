@@ -703,20 +703,21 @@ __device__ __forceinline__ uint16_t fresh_pick(const GraphDev& g, uint32_t v, bo
 }
 
 // A/src/kernels/conflict_repair.cu:10-114 for one vertex; returns 1 if counted.
-// kFresh (parallel mode): a loser selects from the exact colours of its
+// Fresh selection (parallel mode): a loser selects from the exact colours of its
 // neighbours, gathered by the same row scan that found the conflict -- the
 // bitmap or histogram a rebuild at this instant would leave -- instead of
 // the ones rebuilt before the sweep, which miss the recolours made earlier in
 // the same sweep (the reference does not propagate them,
 // A/src/kernels/conflict_repair.cu).  Deterministic mode keeps the
 // reference's selection.  pres: this wavefront's PRESENCE_WORDS of LDS.
-template <bool kFresh>
-__device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v, uint32_t* pres) {
+// fresh: 0 = the reference's selection, 1 = exact colours for bitmap-mode
+// rows, 2 = for histogram-mode rows too.
+__device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v, uint32_t* pres, int fresh_level) {
     const uint32_t w = bcast0(ld(&g.word[v]));
     const uint16_t mine = color_of(w);
     const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
     const bool hub = gp(g.hd_map)[v] != INVALID;
-    const bool fresh = kFresh;
+    const bool fresh = fresh_level >= (hub ? 2 : 1);
     ColourSet cs;
     if (fresh && hub) {
         if (lane() < PRESENCE_WORDS) pres[lane()] = 0;
@@ -795,7 +796,7 @@ __device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v, 
 
 // The same for a long row, all threads of the block: the verdict is a block
 // OR; the first wavefront selects and stores; the block marks the neighbours.
-__device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32_t v) {
+__device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32_t v, int fresh_level) {
     __shared__ uint32_t pick;
     __shared__ uint64_t sets[BLOCK / WAVE][4];
     __shared__ uint32_t pres[PRESENCE_WORDS];
@@ -803,10 +804,11 @@ __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32
     const uint16_t mine = color_of(w);
     const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
     const uint32_t my_deg = gp(g.deg0)[v];
-    const bool hub = gp(g.hd_map)[v] != INVALID;   // parallel mode only: fresh selection, see tier1_vertex
+    const bool hub = gp(g.hd_map)[v] != INVALID;
+    const bool fresh = fresh_level >= (hub ? 2 : 1);   // see tier1_vertex
     ColourSet cs;
     uint32_t hit = 0;
-    if (hub) {
+    if (fresh && hub) {
         if (threadIdx.x < PRESENCE_WORDS) pres[threadIdx.x] = 0;
         __syncthreads();
     }
@@ -822,8 +824,8 @@ __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32
             c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(ld(&g.word[nb[u]])) : 0;
 #pragma unroll
         for (uint32_t u = 0; u < ROW_UNROLL; u++) {
-            if (hub) presence_add(pres, c[u]);
-            else cs.add(c[u]);
+            if (fresh && hub) presence_add(pres, c[u]);
+            else if (fresh) cs.add(c[u]);
             if (mine == 0) {
                 hit |= nb[u] != INVALID;
             } else if (c[u] == mine) {
@@ -833,7 +835,7 @@ __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32
         }
     }
     if (block_sum32(hit) == 0) return 0;   // (its barrier also completes the presence bits)
-    if (!hub) {
+    if (fresh && !hub) {
         cs.wave_reduce();
         if (lane() == 0)
             for (int k = 0; k < 4; k++) sets[threadIdx.x / WAVE][k] = cs.m[k];
@@ -842,7 +844,9 @@ __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32
     if (threadIdx.x < WAVE) {
         if (mine != 0 && mine <= 64 && lane() == 0) or64(&g.bitmap[v], 1ull << (mine - 1));
         uint16_t c;
-        if (!hub) {
+        if (!fresh) {
+            c = find_color(g, v);
+        } else if (!hub) {
             ColourSet all;
             for (uint32_t wv = 0; wv < BLOCK / WAVE; wv++)
                 for (int k = 0; k < 4; k++) all.m[k] |= sets[wv][k];
@@ -883,7 +887,7 @@ __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32
 }
 
 __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__ graphs, Space sp, SweepList sl,
-                                                    uint64_t* totals) {
+                                                    int fresh, uint64_t* totals) {
     __shared__ uint32_t presence[BLOCK / WAVE][PRESENCE_WORDS];
     uint64_t rows = 0, slots = 0;
     const uint32_t nbig = sweep_big(sl);
@@ -892,7 +896,7 @@ __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__
         sp.locate(big_item(sl, k), gid, v);
         const GraphDev& g = graphs[gid];
         const uint32_t f = gp(g.fill)[v];
-        const uint32_t r = tier1_vertex_block(g, v);
+        const uint32_t r = tier1_vertex_block(g, v, fresh);
         if (threadIdx.x == 0) {
             if (r) add32(&g.ctr[C_CONFLICTS], 1u);
             rows++;
@@ -912,7 +916,7 @@ __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__
         const GraphDev& g = graphs[gid];
         rows++;
         slots += gp(g.fill)[v];
-        cnt += tier1_vertex<true>(g, v, presence[threadIdx.x / WAVE]);
+        cnt += tier1_vertex(g, v, presence[threadIdx.x / WAVE], fresh);
     }
     if (cnt && lane() == 0) add32(&graphs[cur_gid].ctr[C_CONFLICTS], cnt);
     if (lane() != 0) rows = slots = 0;
@@ -926,9 +930,9 @@ __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__
 // check, since a recolour marks later vertices of the same 64-vertex chunk.
 // A vertex whose bit is clear would be found conflict-free, so the sweep
 // changes exactly what the full sweep changes.
-template <bool kFresh>
 __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                     const uint32_t* __restrict__ active, uint64_t* totals) {
+                                                     const uint32_t* __restrict__ active, int fresh,
+                                                     uint64_t* totals) {
     __shared__ uint32_t presence[PRESENCE_WORDS];
     uint64_t rows = 0;
     for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
@@ -936,7 +940,7 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
         const GraphDev g = graphs[gid];
         uint32_t cnt = 0;
         if (!g.dirty) {
-            for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex<kFresh>(g, v, presence);
+            for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex(g, v, presence, fresh);
             rows += g.n;
         } else {
             // 256 vertices per step: each lane reads the marks of 4 as one dword
@@ -957,7 +961,7 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
                     const uint32_t q = (uint32_t)__builtin_ctz((uint32_t)__shfl((int)m4, (int)k));
                     const uint32_t u = base + 4 * k + q;
                     if (lane() == 0) gp(g.dirty)[cst + u] = 0;
-                    cnt += tier1_vertex<kFresh>(g, u, presence);
+                    cnt += tier1_vertex(g, u, presence, fresh);
                     rows++;
                     from = 4 * k + q + 1;
                 }
@@ -1139,15 +1143,21 @@ hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int
         head = eg->head;
         next = eg->next;
         mask = eg->size - 1;
-        uint32_t cg = grid_for_waves(eg->size, 4 * WAVE, 1024);
-        k_group_clear<<<cg, 256, 0, L.stream>>>(eg->keys, eg->head, eg->size);
-        uint32_t mg = grid_for_waves(count, 4 * WAVE, 1024);
-        k_group_link<<<mg, 256, 0, L.stream>>>(L.graphs, L.gbase, (const uint4*)upd, count, multi, eg->keys,
-                                               eg->head, eg->next, mask);
     }
     uint32_t grid = grid_for_waves(count, 4, L.max_blocks);
     k_apply_elem<<<grid, 256, 0, L.stream>>>(L.graphs, (const uint4*)upd, count, multi, L.gbase, keys, head, next,
                                               mask, L.totals);
+    return hipGetLastError();
+}
+hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg) {
+    if (count <= 1 || !eg || !eg->keys) return hipSuccess;
+    if ((uint64_t)eg->size < 2ull * count || eg->next_cap < count) return hipErrorInvalidValue;
+    const uint32_t mask = eg->size - 1;
+    uint32_t cg = grid_for_waves(eg->size, 4 * WAVE, 1024);
+    k_group_clear<<<cg, 256, 0, L.stream>>>(eg->keys, eg->head, eg->size);
+    uint32_t mg = grid_for_waves(count, 4 * WAVE, 1024);
+    k_group_link<<<mg, 256, 0, L.stream>>>(L.graphs, L.gbase, (const uint4*)upd, count, multi, eg->keys, eg->head,
+                                           eg->next, mask);
     return hipGetLastError();
 }
 hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg,
@@ -1205,13 +1215,12 @@ hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist) {
                                                    exact_hist ? 1 : 0, L.totals);
     return hipGetLastError();
 }
-hipError_t launch_tier1(const LaunchCtx& L, bool serial, bool fresh) {
+hipError_t launch_tier1(const LaunchCtx& L, bool serial, int fresh) {
     if (serial) {
         const uint32_t grid = L.ng < 1024 ? L.ng : 1024;
-        if (fresh) k_tier1_serial<true><<<grid, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, L.totals);
-        else k_tier1_serial<false><<<grid, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, L.totals);
+        k_tier1_serial<<<grid, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, fresh, L.totals);
     } else {
-        k_tier1_elem<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L),
+        k_tier1_elem<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L), fresh,
                                                           L.totals);
     }
     return hipGetLastError();

@@ -39,6 +39,8 @@ struct EdgeGroups {
     uint32_t next_cap = 0;
 };
 
+// Per-edge grouping of a parallel batch (before launch_apply with the same eg).
+hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg);
 hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg);
 hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg,
                             const uint32_t* perm, int multi);
@@ -50,8 +52,9 @@ hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
 hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count);
 hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist);
 // serial: vertex order, one wavefront per graph; fresh: parallel-mode colour
-// selection from the exact neighbour colours (see tier1_vertex).
-hipError_t launch_tier1(const LaunchCtx& L, bool serial, bool fresh);
+// selection from the exact neighbour colours (0 none, 1 bitmap-mode rows,
+// 2 all rows; see tier1_vertex).
+hipError_t launch_tier1(const LaunchCtx& L, bool serial, int fresh);
 hipError_t launch_t2_check(const LaunchCtx& L);
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors);
 hipError_t launch_find_color(const LaunchCtx& L, uint32_t gid, uint32_t v, uint16_t* out);

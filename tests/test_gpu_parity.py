@@ -368,3 +368,25 @@ def test_parallel_long_row_repair_is_proper(long_row, graph_path):
     bad = gpu.validate_history(iv, [256 * (k + 1) for k in range(gpu.history_count())])
     assert int(bad.sum()) == 0
     assert gpu.validate()[:2] == (0, 0)
+
+
+@pytest.mark.parametrize("fresh", [0, 1, 2])
+@pytest.mark.parametrize("name", ["rmat_s10", "chunglu4k"])
+def test_parallel_selection_modes_repair_to_proper(name, fresh, graph_path):
+    # fresh 0: the reference's selection (bitmaps rebuilt before every sweep);
+    # 1 / 2: exact neighbour colours for bitmap rows / all rows
+    g = S.load_graph_csr(graph_path(name))
+    gpu, orc, og = _pair(g, epoch_size=256, batch_size=128, exact_histogram=True, repair_every_epoch=True,
+                         histogram_threshold=16)
+    gpu.set_tuning(fresh_repair=fresh, serial_repair=0)
+    init = S.generate_init_stream(g)
+    gpu.process_stream(init)
+    gpu.finish()
+    bench = S.generate_bench_stream_edges(g, 256 * 16, 50, 21)
+    epochs = gpu.process_stream(bench)
+    assert all(e["t2_remaining"] == 0 for e in epochs)
+    assert _proper_against_live(og, gpu.snapshot(), bench["u"], bench["v"], bench["type"]) == (0, 0)
+    orc.submit(init["u"], init["v"], init["type"])
+    orc.finish()
+    orc.submit(bench["u"], bench["v"], bench["type"])
+    assert gpu.validate()[2] <= orc.finish()["max_color"] * 1.25 + 2
