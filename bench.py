@@ -39,6 +39,9 @@ WORKLOADS = {
                     "R-MAT s12 ef 48) + com-Orkut-shape Chung-Lu (3,072,441 vertices, 117M edges) resident in one "
                     "pool; churn (50% insert) batches of U[1,8192] updates to uniform tenants; E_g = 4,096",
                kind="served"),
+    "c3": dict(name="R-MAT scale 20 (a,b,c = .57,.19,.19), edge factor 48 (seed 42); dense-growth stream inside the "
+                    "90th-percentile k-core, 2 inserts per delete (seed 42); E=32,768; plain engine",
+               kind="rmat", scale=20, edge_factor=48.0),
     "c1": dict(name="Delaunay 2^17 uniform points (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
                kind="delaunay", points=1 << 17),
 }
@@ -51,7 +54,15 @@ def log(*a):
 def make_graph(S, w):
     if w["kind"] == "chung_lu":
         return S.gen_chung_lu(w["n"], w["m"], w["gamma"], w["max_degree"], seed=42, permute=True)
+    if w["kind"] == "rmat":
+        return S.gen_rmat(w["scale"], w["edge_factor"], seed=42)
     return S.gen_delaunay(w["points"], seed=42)
+
+
+def make_stream(S, w, g, size):
+    if w["kind"] == "rmat":   # dense growth inside the dense core
+        return S.dense_growth_stream(g, size, core_pct=90, ins_per_del=2, seed=42)
+    return S.generate_bench_stream_edges(g, size, 50, 42)
 
 
 SWEEP_ROW_BYTES = {"compact": 8, "rebuild": 16, "tier1": 12, "t2_check": 12}
@@ -163,7 +174,7 @@ def main():
     log(f"graph {g.n} vertices {g.num_edges // 2} edges max deg {int(g.degree.max())} ({time.perf_counter() - t:.1f}s)")
     t = time.perf_counter()
     init = S.generate_init_stream(g)
-    bench = S.generate_bench_stream_edges(g, (W + K) * E, 50, 42)
+    bench = make_stream(S, w, g, (W + K) * E)
     log(f"streams: init {len(init)}, churn {len(bench)} ({time.perf_counter() - t:.1f}s)")
 
     eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True,

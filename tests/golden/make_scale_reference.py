@@ -6,7 +6,7 @@ publish every epoch -- and records the maximum colour (the reference's colour
 count, A/src/streamgc.cu:549-553) after the init repair and after every epoch.
 The GPU engine's parallel results are compared against these numbers.
 
-Usage: python tests/golden/make_scale_reference.py c1 [c2 ...]   (CPU; c2 takes minutes)
+Usage: python tests/golden/make_scale_reference.py c1 [c2 c3 ...]   (CPU; c2 and c3 take minutes)
 """
 import json
 import os
@@ -30,7 +30,15 @@ def graph(name):
         return S.gen_delaunay(1 << 17, seed=42), 100000
     if name == "c2":
         return S.gen_chung_lu(3997962, 34681189, 2.3, 15000.0, seed=42, permute=True), 31 * E
+    if name == "c3":
+        return S.gen_rmat(20, 48.0, seed=42), 31 * E
     raise SystemExit(f"unknown workload {name}")
+
+
+def stream(name, g, size):
+    if name == "c3":   # dense growth inside the 90th-percentile k-core, 2 inserts per delete
+        return S.dense_growth_stream(g, size, core_pct=90, ins_per_del=2, seed=42)
+    return S.generate_bench_stream_edges(g, size, 50, 42)
 
 
 def run(name, exact):
@@ -44,7 +52,7 @@ def run(name, exact):
     fin = eng.finish()
     t_init = time.time() - t
     eng.set_flags(flags | O.FLAG_REPAIR_EVERY_EPOCH)
-    bench = S.generate_bench_stream_edges(g, size, 50, 42)
+    bench = stream(name, g, size)
     t = time.time()
     eps = eng.submit(bench["u"], bench["v"], bench["type"])
     last = eng.finish()
