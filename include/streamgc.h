@@ -129,6 +129,10 @@ const char* sgc_version(void);
 
 /* Device probe: number of HIP devices (0 when none); never fails. */
 int sgc_device_count(void);
+/* The device's marketing name (cudaGetDeviceProperties().name, which the
+ * reference uses as its hardware tag, A/src/streamgc.cu:691-699); the ISA
+ * name ("gfx950") when the runtime reports no marketing name. */
+int sgc_device_name(int device, char* name, uint32_t cap);
 
 /* = allocate_gpu_partition (A/src/streamgc.cu:144-444) + the priority fill
  * (:755-767).  Input: symmetric CSR with sorted rows (as the loader makes),
@@ -322,6 +326,36 @@ int sgc_server_get_stats(sgc_server* s, sgc_server_stats* out);
 /* Raw latencies in microseconds: which 0 = update-to-snapshot per request, 1 = query. */
 int sgc_server_latencies(sgc_server* s, int which, double* out, uint64_t cap, uint64_t* count);
 int sgc_server_reset_stats(sgc_server* s);
+
+/* ---- the reference's result files (A/src/streamgc.cu:595-642,1469-1573) ----
+ * sgc_write_coloring writes <path> (raw u16 per vertex) and <path>.txt (the
+ * "# vertices / # colors / # algorithm" header, then "id colour" lines).
+ * sgc_write_meta writes the .coloring.meta JSON with the reference's keys in
+ * its order; throughput and colour ratio are derived as the reference derives
+ * them, multi-GPU stage timings are 0. */
+typedef struct sgc_run_meta {
+    const char* graph;          /* graph name (file name without directory and extension) */
+    const char* hw_tag;
+    uint32_t num_vertices;
+    uint32_t num_colors_used;
+    uint64_t num_edges;         /* CSR entries, as the reference's loader counts them */
+    uint32_t static_greedy_colors;
+    uint32_t insert_pct;
+    uint64_t stream_size;
+    uint64_t seed;
+    double init_time_ms;
+    double bench_time_ms;       /* stream + final repair */
+    double bench_stream_ms;
+    double repair_time_ms;
+    double local_kernel_ms;     /* per-update kernels */
+    double gpu_sync_ms;
+    double epoch_maintenance_ms;
+    double memcpy_ms;
+    double query_latency_us;
+    uint64_t query_count;
+} sgc_run_meta;
+int sgc_write_coloring(const char* path, const uint16_t* colors, uint32_t n);
+int sgc_write_meta(const char* path, const sgc_run_meta* m);
 
 /* ---- host pipeline (C++17) ---- */
 

@@ -9,6 +9,7 @@ load_graph_csr (A/src/host/graph_loader.cpp:29), generate_init_stream and
 generate_bench_stream_edges (A/src/host/stream_gen.cpp:30,65).
 """
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -20,7 +21,8 @@ __all__ = ["StreamGCConfig", "default_config", "CSRGraph", "GPUPartition", "Devi
            "load_graph_csr", "csr_from_edges", "csr_from_edge_arrays", "gen_chung_lu", "gen_rmat",
            "gen_delaunay", "core_numbers", "dense_growth_stream", "generate_init_stream", "generate_bench_stream_edges",
            "static_greedy_coloring", "check_coloring_valid", "edge_intervals", "make_updates", "SgcError",
-           "SgcNoMemory", "device_count", "GraphPool", "GraphServer", "tag_graph"]
+           "SgcNoMemory", "device_count", "GraphPool", "GraphServer", "tag_graph", "write_coloring",
+           "write_coloring_meta", "read_coloring"]
 
 from ._lib import SgcNoMemory  # noqa: E402
 
@@ -733,3 +735,23 @@ class GraphServer:
 
     def reset_stats(self):
         self._ck(lib().sgc_server_reset_stats(self.h))
+
+
+def write_coloring(path, colors):
+    """<path> (raw u16 per vertex) and <path>.txt, as the reference writes them."""
+    colors = np.ascontiguousarray(colors, dtype=np.uint16)
+    check(lib().sgc_write_coloring(os.fsencode(path), colors, len(colors)))
+
+
+def read_coloring(path):
+    return np.fromfile(path, dtype="<u2")
+
+
+def write_coloring_meta(path, **fields):
+    """<path> (.coloring.meta JSON with the reference's keys); fields of sgc_run_meta."""
+    m = _lib.RunMeta()
+    for k, v in fields.items():
+        if k in ("graph", "hw_tag"):
+            v = str(v).encode()
+        setattr(m, k, v)
+    check(lib().sgc_write_meta(os.fsencode(path), C.byref(m)))

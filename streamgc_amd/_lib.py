@@ -73,6 +73,15 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RunMeta(C.Structure):
+    _fields_ = [("graph", C.c_char_p), ("hw_tag", C.c_char_p), ("num_vertices", C.c_uint32),
+                ("num_colors_used", C.c_uint32), ("num_edges", C.c_uint64), ("static_greedy_colors", C.c_uint32),
+                ("insert_pct", C.c_uint32), ("stream_size", C.c_uint64), ("seed", C.c_uint64)] + \
+               [(k, C.c_double) for k in ("init_time_ms", "bench_time_ms", "bench_stream_ms", "repair_time_ms",
+                                          "local_kernel_ms", "gpu_sync_ms", "epoch_maintenance_ms", "memcpy_ms",
+                                          "query_latency_us")] + [("query_count", C.c_uint64)]
+
+
 class PoolConfig(C.Structure):
     _fields_ = [("max_graphs", C.c_uint32), ("batch_capacity", C.c_uint32), ("flags", C.c_uint32),
                 ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64)]
@@ -93,6 +102,7 @@ SIGNATURES = {
     "sgc_last_error": [_vp],
     "sgc_version": [],
     "sgc_device_count": [],
+    "sgc_device_name": [C.c_int, C.c_char_p, C.c_uint32],
     "sgc_create": [C.POINTER(Config), C.c_uint32, _u32p, _u32p, _u32p, C.POINTER(_vp)],
     "sgc_destroy": [_vp],
     "sgc_submit": [_vp, _vp, C.c_uint64, C.c_int, C.POINTER(EpochStats), C.c_uint32, C.POINTER(C.c_uint32)],
@@ -166,6 +176,8 @@ SIGNATURES = {
     "sgc_server_get_stats": [_vp, C.POINTER(ServerStats)],
     "sgc_server_latencies": [_vp, C.c_int, _vp, C.c_uint64, C.POINTER(C.c_uint64)],
     "sgc_server_reset_stats": [_vp],
+    "sgc_write_coloring": [C.c_char_p, _u16p, C.c_uint32],
+    "sgc_write_meta": [C.c_char_p, C.POINTER(RunMeta)],
     "sgc_csr_load_mtx": [C.c_char_p, C.POINTER(_vp)],
     "sgc_csr_from_edges": [C.c_uint32, _u32p, _u32p, C.c_uint64, C.POINTER(_vp)],
     "sgc_csr_info": [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],

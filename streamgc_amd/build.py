@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libstreamgc_amd.so")
 CLI = os.path.join(HERE, "streaming_gc")
 
 HIP_SOURCES = ["sgc_kernels.hip", "sgc_engine.hip"]
-CXX_SOURCES = ["sgc_host.cpp", "sgc_gen.cpp", "sgc_server.cpp"]
+CXX_SOURCES = ["sgc_host.cpp", "sgc_gen.cpp", "sgc_server.cpp", "sgc_output.cpp"]
 CLI_SOURCES = ["sgc_cli.cpp"]
 
 ARCH = os.environ.get("SGC_OFFLOAD_ARCH", "gfx950")

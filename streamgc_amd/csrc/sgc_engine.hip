@@ -1124,6 +1124,20 @@ int sgc_device_count(void) {
     return c;
 }
 
+int sgc_device_name(int device, char* name, uint32_t cap) {
+    if (!name || cap == 0) return SGC_ERR_INVALID;
+    hipDeviceProp_t prop;
+    hipError_t r = hipGetDeviceProperties(&prop, device);
+    if (r != hipSuccess) return global_fail(SGC_ERR_HIP, hipGetErrorString(r));
+    if (prop.name[0]) {
+        snprintf(name, cap, "%s", prop.name);
+    } else {   // some runtimes leave the marketing name empty: fall back to the ISA name
+        std::string arch(prop.gcnArchName);
+        snprintf(name, cap, "%s", arch.substr(0, arch.find(':')).c_str());
+    }
+    return SGC_OK;
+}
+
 // ---------------------------------------------------------------- single graph
 int sgc_create(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
                const uint32_t* deg0, sgc_engine** out) {
