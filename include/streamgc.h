@@ -261,6 +261,7 @@ int sgc_pool_submit(sgc_pool* p, const sgc_edge_update* updates, uint64_t count,
                     sgc_epoch_stats* stats_out, uint32_t max_stats, uint32_t* nstats);
 int sgc_pool_query(sgc_pool* p, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors);
 int sgc_pool_download_words(sgc_pool* p, uint32_t gid, uint32_t* words);
+int sgc_pool_upload_words(sgc_pool* p, uint32_t gid, const uint32_t* words);  /* every sweep then visits every vertex */
 int sgc_pool_download_snapshot(sgc_pool* p, uint32_t gid, uint16_t* colors);
 int sgc_pool_validate(sgc_pool* p, uint32_t gid, int snapshot, uint64_t* violations, uint64_t* uncolored_with_edges,
                       uint32_t* max_color);
@@ -273,6 +274,48 @@ int sgc_pool_totals(sgc_pool* p, sgc_totals* out);
 int sgc_pool_kernel_timing_enable(sgc_pool* p, int enable);
 int sgc_pool_kernel_times(sgc_pool* p, char* names, uint32_t name_stride, double* total_ms, uint64_t* launches,
                           uint32_t cap, uint32_t* count);
+/* Per-graph epoch size E_g (no reference counterpart; the reference has one
+ * --epoch-size for its single graph).  Takes effect at the graph's next batch. */
+int sgc_pool_set_epoch_size(sgc_pool* p, uint32_t gid, uint32_t epoch_size);
+
+/* ---- batched pattern: many graphs, one resident stream, captured rounds ----
+ * No reference counterpart.  sgc_pool_batch_prepare copies a mixed stream
+ * (records tagged with their graph id, stream order kept per graph) to the
+ * device and cuts it into rounds: round r gives every graph with records left
+ * its next min(E_g - pending, left) records, then every graph of the round
+ * takes an epoch step; a graph's last partial epoch is repaired and published
+ * like a full one.  Each graph thus sees exactly the batches, epochs and
+ * final repair it sees alone (batch = epoch = E_g, then sgc_finish).
+ * sgc_pool_batch_run runs the rounds: either one captured hipGraph replayed per
+ * round (every per-round parameter, the Tier-1 loop control and the snapshot
+ * flip live on the device; the host steps in only for a round that needs a
+ * host Tier-2 repair or more Tier-1 sweeps than were captured), or host-driven
+ * launches.  The autotuner times each launch-shape candidate on interleaved
+ * rounds and keeps the fastest per update (the pool keeps it afterwards).
+ * Needs SGC_REPAIR_EVERY_EPOCH and dirty tracking.  Queries of the pool wait
+ * while a run is in progress. */
+typedef struct sgc_batch_config {
+    uint32_t use_graph;        /* 1: replay a captured hipGraph per round; 0: host-driven launches */
+    uint32_t t1_sweeps;        /* Tier-1 sweeps captured per round (0: the graphs' max_repair_iters) */
+    uint32_t autotune_rounds;  /* > 0: rounds given to each launch-shape candidate (hipGraph mode) */
+    uint32_t reserved;
+} sgc_batch_config;
+#define SGC_BATCH_CANDIDATES 4
+typedef struct sgc_batch_stats {
+    uint32_t rounds;           /* rounds in the plan */
+    uint32_t host_rounds;      /* rounds the host finished */
+    uint32_t candidates;       /* launch-shape candidates timed (0: no tuning) */
+    uint32_t chosen;           /* the candidate kept */
+    uint64_t updates;
+    double ms;                 /* wall time of the run, first launch to the last snapshot published */
+    double candidate_ns_per_update[SGC_BATCH_CANDIDATES];
+    sgc_tuning candidate[SGC_BATCH_CANDIDATES];
+} sgc_batch_stats;
+void sgc_default_batch_config(sgc_batch_config* c);
+int sgc_pool_batch_prepare(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, const sgc_batch_config* cfg,
+                           uint32_t* rounds);
+int sgc_pool_batch_run(sgc_pool* p, sgc_batch_stats* out);
+int sgc_pool_batch_release(sgc_pool* p);
 
 /* ---- served pattern: a persistent request loop over a pool ----
  * No reference counterpart.  Any thread submits update batches and queries

@@ -538,6 +538,35 @@ class GraphPool:
                                        max_stats, C.byref(k)))
         return [st[i].as_dict() for i in range(min(k.value, max_stats))]
 
+    def set_epoch_size(self, gid, epoch_size):
+        """E_g of one graph (takes effect at its next batch)."""
+        self._ck(lib().sgc_pool_set_epoch_size(self.h, gid, epoch_size))
+
+    def batch_prepare(self, records, use_graph=True, t1_sweeps=0, autotune_rounds=0):
+        """Batched pattern: make the round plan of a mixed stream and keep it on the device.
+
+        Round r gives every graph its next min(E_g, left) records, then every
+        graph of the round takes an epoch step (its last partial epoch too);
+        use_graph replays one captured hipGraph per round.  Returns the rounds.
+        """
+        records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
+        cfg = _lib.BatchConfig()
+        lib().sgc_default_batch_config(C.byref(cfg))
+        cfg.use_graph = 1 if use_graph else 0
+        cfg.t1_sweeps = t1_sweeps
+        cfg.autotune_rounds = autotune_rounds
+        r = C.c_uint32()
+        self._ck(lib().sgc_pool_batch_prepare(self.h, records.ctypes.data, len(records), C.byref(cfg), C.byref(r)))
+        return r.value
+
+    def batch_run(self):
+        st = _lib.BatchStats()
+        self._ck(lib().sgc_pool_batch_run(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def batch_release(self):
+        self._ck(lib().sgc_pool_batch_release(self.h))
+
     def due(self):
         cap = max(len(self.graphs), 1)
         out = np.zeros(cap, np.uint32)
@@ -571,6 +600,9 @@ class GraphPool:
 
     def colors(self, gid):
         return (self.words(gid) & 0xFFFF).astype(np.uint16)
+
+    def set_words(self, gid, words):
+        self._ck(lib().sgc_pool_upload_words(self.h, gid, np.ascontiguousarray(words, np.uint32)))
 
     def snapshot(self, gid):
         s = np.zeros(self.graphs[gid].n, np.uint16)

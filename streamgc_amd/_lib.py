@@ -82,6 +82,28 @@ class RunMeta(C.Structure):
                                           "query_latency_us")] + [("query_count", C.c_uint64)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("use_graph", C.c_uint32), ("t1_sweeps", C.c_uint32), ("autotune_rounds", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+BATCH_CANDIDATES = 4
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("rounds", C.c_uint32), ("host_rounds", C.c_uint32), ("candidates", C.c_uint32),
+                ("chosen", C.c_uint32), ("updates", C.c_uint64), ("ms", C.c_double),
+                ("candidate_ns_per_update", C.c_double * BATCH_CANDIDATES),
+                ("candidate", Tuning * BATCH_CANDIDATES)]
+
+    def as_dict(self):
+        k = max(self.candidates, 1)
+        return {"rounds": self.rounds, "host_rounds": self.host_rounds, "candidates": self.candidates,
+                "chosen": self.chosen, "updates": self.updates, "ms": self.ms,
+                "candidate_ns_per_update": [self.candidate_ns_per_update[i] for i in range(k)],
+                "candidate": [{f: getattr(self.candidate[i], f) for f, _ in Tuning._fields_} for i in range(k)]}
+
+
 class PoolConfig(C.Structure):
     _fields_ = [("max_graphs", C.c_uint32), ("batch_capacity", C.c_uint32), ("flags", C.c_uint32),
                 ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64)]
@@ -150,6 +172,7 @@ SIGNATURES = {
     "sgc_pool_submit": [_vp, _vp, C.c_uint64, C.c_int, C.POINTER(EpochStats), C.c_uint32, C.POINTER(C.c_uint32)],
     "sgc_pool_query": [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p],
     "sgc_pool_download_words": [_vp, C.c_uint32, _u32p],
+    "sgc_pool_upload_words": [_vp, C.c_uint32, _u32p],
     "sgc_pool_download_snapshot": [_vp, C.c_uint32, _u16p],
     "sgc_pool_validate": [_vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                           C.POINTER(C.c_uint32)],
@@ -159,6 +182,11 @@ SIGNATURES = {
                                   _u64p],
     "sgc_pool_totals": [_vp, C.POINTER(Totals)],
     "sgc_pool_kernel_timing_enable": [_vp, C.c_int],
+    "sgc_pool_set_epoch_size": [_vp, C.c_uint32, C.c_uint32],
+    "sgc_default_batch_config": [C.POINTER(BatchConfig)],
+    "sgc_pool_batch_prepare": [_vp, _vp, C.c_uint64, C.POINTER(BatchConfig), C.POINTER(C.c_uint32)],
+    "sgc_pool_batch_run": [_vp, C.POINTER(BatchStats)],
+    "sgc_pool_batch_release": [_vp],
     "sgc_pool_kernel_times": [_vp, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                               C.c_uint32, C.POINTER(C.c_uint32)],
     "sgc_default_server_config": [C.POINTER(ServerConfig)],
@@ -195,7 +223,7 @@ SIGNATURES = {
 }
 _RESTYPE = {"sgc_default_config": None, "sgc_default_tuning": None, "sgc_last_error": C.c_char_p, "sgc_version": C.c_char_p,
             "sgc_pool_last_error": C.c_char_p, "sgc_server_last_error": C.c_char_p,
-            "sgc_default_server_config": None}
+            "sgc_default_server_config": None, "sgc_default_batch_config": None}
 
 _lib = None
 

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstreamgc_amd.so")
 CLI = os.path.join(HERE, "streaming_gc")
 
-HIP_SOURCES = ["sgc_kernels.hip", "sgc_engine.hip"]
+HIP_SOURCES = ["sgc_kernels.hip", "sgc_rounds.hip", "sgc_engine.hip"]
 CXX_SOURCES = ["sgc_host.cpp", "sgc_gen.cpp", "sgc_server.cpp", "sgc_output.cpp"]
 CLI_SOURCES = ["sgc_cli.cpp"]
 

@@ -163,6 +163,42 @@ struct Staged {
     }
 };
 
+// A prepared batched run (sgc_pool_batch_prepare): the mixed stream resident
+// on the device in round order, the round plan, the device-side control of
+// the captured rounds and one instantiated hipGraph per launch-shape
+// candidate.
+struct BatchRun {
+    sgc_batch_config cfg{};
+    uint32_t rounds = 0, ng = 0, max_count = 0, max_n = 0, sweeps = 0, ovf_cap = 0;
+    uint64_t updates = 0;
+    bool multi = false, any_history = false;
+    std::vector<uint32_t> tab;                 // per round: first record, record count
+    std::vector<uint8_t> inround, full;        // rounds x ng: takes part; reaches its epoch boundary
+    std::vector<uint32_t> counts;              // rounds x ng: records
+    std::vector<std::vector<uint32_t>> gids;   // graphs of each round
+    sgc_edge_update* d_stream = nullptr;
+    uint32_t *d_tab = nullptr, *d_seg = nullptr, *d_ctl = nullptr, *d_fl = nullptr, *d_pub = nullptr;
+    uint8_t* d_inround = nullptr;
+    RoundState* d_state = nullptr;
+    RoundGraph* d_rg = nullptr;
+    uint16_t** d_snapbase = nullptr;
+    HistDev* d_hist = nullptr;
+    uint32_t* h_ctl = nullptr;                 // pinned copy of the control block
+    std::vector<sgc_tuning> cands;
+    std::vector<hipGraphExec_t> execs;
+
+    void release() {
+        for (hipGraphExec_t x : execs)
+            if (x) (void)hipGraphExecDestroy(x);
+        execs.clear();
+        for (void* p : {(void*)d_stream, (void*)d_tab, (void*)d_seg, (void*)d_ctl, (void*)d_fl, (void*)d_pub,
+                        (void*)d_inround, (void*)d_state, (void*)d_rg, (void*)d_snapbase, (void*)d_hist})
+            if (p) (void)hipFree(p);
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        *this = BatchRun();
+    }
+};
+
 struct sgc_pool {
     uint32_t max_graphs = 0, batch_cap = 0, flags = 0;
     sgc_tuning tune{1024, 2048, 16384, 65536, 64, 2};
@@ -189,6 +225,7 @@ struct sgc_pool {
     uint16_t* d_qout = nullptr;
     uint32_t qcap = 0;
     std::mutex qmu;
+    std::mutex run_mu;                    // held by a batched run; queries wait for it (taken before qmu)
     uint64_t t1_sweeps = 0, t2_runs = 0;
     std::string err;
     bool ktiming = false;
@@ -196,6 +233,7 @@ struct sgc_pool {
     std::vector<hipEvent_t> free_events;
     std::map<std::string, KTime> ktimes;
     double last_ms[4] = {0, 0, 0, 0};
+    BatchRun* batch = nullptr;            // prepared batched run
 
     bool deterministic() const { return (flags & SGC_DETERMINISTIC) != 0; }
     bool exact_hist() const { return (flags & SGC_EXACT_HISTOGRAM) != 0; }
@@ -382,11 +420,22 @@ struct sgc_pool {
         if (e == hipSuccess) e = timed("rebuild", [&] { return launch_rebuild(L, exact_hist()); });
         return e;
     }
-    int tier1(const std::vector<uint32_t>& gids);
+    int tier1(const std::vector<uint32_t>& gids, uint32_t start_it = 0);
     int tier2(const std::vector<uint32_t>& gids);
     int publish(const std::vector<uint32_t>& gids);
     int read_drain(const std::vector<uint32_t>& gids);
     int max_colors(const std::vector<uint32_t>& gids);
+
+    // ------------------------------------------------------------ batched runs
+    int batch_prepare(const sgc_edge_update* upd, uint64_t count, const sgc_batch_config& cfg, uint32_t* rounds);
+    int batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_batch_config& cfg, uint32_t* rounds);
+    int batch_capture(const sgc_tuning& t, hipGraphExec_t* out);
+    int batch_run(sgc_batch_stats* out);
+    int batch_run_host(BatchRun& B);
+    int batch_host_round(BatchRun& B, uint32_t r);
+    hipError_t batch_push(BatchRun& B);
+    hipError_t batch_pull(BatchRun& B);
+    void batch_drop();
 
     void release();
 };
@@ -394,6 +443,7 @@ struct sgc_pool {
 // ============================================================== graph admission
 int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
                         const uint32_t* deg0, uint32_t* gid_out) {
+    batch_drop();   // a prepared plan names the graphs of the pool
     if (!cfg || !row_ptr || !deg0 || n == 0) return fail(SGC_ERR_INVALID, "add graph: bad argument");
     if (cfg->batch_size == 0 || cfg->epoch_size == 0) return fail(SGC_ERR_INVALID, "add graph: zero batch/epoch");
     const uint64_t nnz = row_ptr[n];
@@ -503,6 +553,7 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
 }
 
 int sgc_pool::remove_graph(uint32_t gid) {
+    batch_drop();
     if (gid >= max_graphs || !gs[gid].used) return fail(SGC_ERR_INVALID, "remove graph: no such graph");
     hipError_t e = hipStreamSynchronize(stream);
     if (e == hipSuccess) e = hipStreamSynchronize(qstream);
@@ -619,17 +670,21 @@ int sgc_pool::read_drain(const std::vector<uint32_t>& gids) {
 // run_conflict_repair (A/src/streamgc.cu:1317-1393) for every graph of the set
 // at once: each iteration is one rebuild and one sweep over the graphs still
 // iterating; each graph stops on its own (converged, plateau, iteration cap).
-int sgc_pool::tier1(const std::vector<uint32_t>& gids) {
+// start_it > 0 continues a loop whose state (iterating, prev, plateau,
+// statistics) the caller has set: the sweeps a captured round ran.
+int sgc_pool::tier1(const std::vector<uint32_t>& gids, uint32_t start_it) {
     uint32_t max_iters = 0;
     for (uint32_t g : gids) {
-        gs[g].prev = 0xFFFFFFFFu;
-        gs[g].plateau = 0;
-        gs[g].iterating = gs[g].cfg.max_repair_iters > 0;
-        gs[g].t1_clean = false;
-        gs[g].st.t1_iterations = 0;
+        if (start_it == 0) {
+            gs[g].prev = 0xFFFFFFFFu;
+            gs[g].plateau = 0;
+            gs[g].iterating = gs[g].cfg.max_repair_iters > 0;
+            gs[g].t1_clean = false;
+            gs[g].st.t1_iterations = 0;
+        }
         max_iters = std::max(max_iters, gs[g].cfg.max_repair_iters);
     }
-    for (uint32_t it = 0; it < max_iters; it++) {
+    for (uint32_t it = start_it; it < max_iters; it++) {
         bool any = false;
         for (uint32_t g : gids) any |= gs[g].iterating;
         if (!any) break;
@@ -853,6 +908,7 @@ int sgc_pool::epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<
 void sgc_pool::release() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (qstream) (void)hipStreamSynchronize(qstream);
+    batch_drop();
     for (auto& G : gs) G.release();
     graphs.release();
     vbase.release();
@@ -875,6 +931,446 @@ void sgc_pool::release() {
     if (stream) (void)hipStreamDestroy(stream);
     if (qstream) (void)hipStreamDestroy(qstream);
     stream = qstream = nullptr;
+}
+
+// ============================================================== batched runs
+void sgc_pool::batch_drop() {
+    if (!batch) return;
+    if (stream) (void)hipStreamSynchronize(stream);
+    batch->release();
+    delete batch;
+    batch = nullptr;
+}
+
+// The round plan.  Round r gives every graph with records left its next
+// min(E_g - pending, left) records (E_g = its epoch size), laid out graph by
+// graph in stream order; every graph of the round then takes its epoch step.
+int sgc_pool::batch_prepare(const sgc_edge_update* upd, uint64_t count, const sgc_batch_config& c,
+                            uint32_t* rounds_out) {
+    batch_drop();
+    const int rc = batch_plan(upd, count, c, rounds_out);
+    if (rc) batch_drop();
+    return rc;
+}
+
+int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_batch_config& c,
+                         uint32_t* rounds_out) {
+    if (!repair_every_epoch() || !tracking())
+        return fail(SGC_ERR_INVALID, "batch: needs SGC_REPAIR_EVERY_EPOCH and dirty tracking");
+    if (ng == 0) return fail(SGC_ERR_INVALID, "batch: the pool holds no graph");
+    if (count >= 0xFFFFFFFFull) return fail(SGC_ERR_RANGE, "batch: stream longer than 2^32 - 1 records");
+    std::vector<std::vector<uint32_t>> q(ng);
+    for (uint64_t i = 0; i < count; i++) {
+        uint32_t gid;
+        memcpy(&gid, upd[i].pad + 3, 4);
+        if (gid >= ng || !gs[gid].used) return fail(SGC_ERR_INVALID, "batch: unknown graph id");
+        q[gid].push_back((uint32_t)i);
+    }
+    BatchRun* B = new BatchRun();
+    batch = B;
+    B->cfg = c;
+    B->ng = ng;
+    B->multi = ng > 1;
+    B->updates = count;
+    std::vector<sgc_edge_update> ordered;
+    ordered.reserve(count);
+    std::vector<uint32_t> seg;
+    std::vector<size_t> head(ng, 0);
+    std::vector<uint64_t> since(ng, 0);
+    for (uint32_t g = 0; g < ng; g++) since[g] = gs[g].used ? gs[g].since_epoch : 0;
+    for (;;) {
+        const uint32_t off = (uint32_t)ordered.size();
+        std::vector<uint32_t> in;
+        const size_t at = B->inround.size();
+        B->inround.resize(at + ng, 0);
+        B->full.resize(at + ng, 0);
+        B->counts.resize(at + ng, 0);
+        seg.resize(2 * (at + ng), 0);
+        for (uint32_t g = 0; g < ng; g++) {
+            if (!gs[g].used || head[g] == q[g].size()) continue;
+            const uint64_t E = gs[g].cfg.epoch_size;
+            const uint64_t room = E - std::min<uint64_t>(since[g], E - 1);
+            const uint64_t take = std::min<uint64_t>(q[g].size() - head[g], room);
+            seg[2 * (at + g)] = (uint32_t)ordered.size();
+            for (uint64_t k = 0; k < take; k++) ordered.push_back(upd[q[g][head[g] + k]]);
+            seg[2 * (at + g) + 1] = (uint32_t)ordered.size();
+            head[g] += take;
+            B->inround[at + g] = 1;
+            B->full[at + g] = since[g] + take >= E;
+            B->counts[at + g] = (uint32_t)take;
+            since[g] = 0;
+            in.push_back(g);
+        }
+        if (in.empty()) {
+            B->inround.resize(at);
+            B->full.resize(at);
+            B->counts.resize(at);
+            seg.resize(2 * at);
+            break;
+        }
+        B->tab.push_back(off);
+        B->tab.push_back((uint32_t)ordered.size() - off);
+        B->max_count = std::max(B->max_count, (uint32_t)ordered.size() - off);
+        B->gids.push_back(in);
+        B->rounds++;
+    }
+    if ((uint64_t)B->rounds * 2 * ng >= 0xFFFFFFFFull) return fail(SGC_ERR_RANGE, "batch: round table too large");
+    if (!deterministic() && (B->max_count > eg.next_cap || 2ull * B->max_count > eg.size))
+        return fail(SGC_ERR_INVALID, "batch: a round holds more records than the pool's batch capacity (" +
+                                         std::to_string(B->max_count) + " > " + std::to_string(batch_cap) + ")");
+    uint32_t max_iters = 0;
+    std::vector<RoundGraph> rg(ng);
+    std::vector<uint16_t*> base(ng, nullptr);
+    for (uint32_t g = 0; g < ng; g++) {
+        if (!gs[g].used) continue;
+        rg[g].max_iters = gs[g].cfg.max_repair_iters;
+        rg[g].plateau_iters = gs[g].cfg.plateau_iters;
+        rg[g].n = gs[g].n;
+        base[g] = gs[g].snap;
+        max_iters = std::max(max_iters, gs[g].cfg.max_repair_iters);
+        B->max_n = std::max(B->max_n, gs[g].n);
+        B->ovf_cap = std::max(B->ovf_cap, gs[g].cfg.overflow_capacity);
+        B->any_history |= gs[g].history != nullptr;
+    }
+    B->sweeps = c.t1_sweeps ? std::min(c.t1_sweeps, max_iters) : max_iters;
+    hipError_t e = hipMalloc((void**)&B->d_stream, std::max<size_t>(ordered.size(), 1) * sizeof(sgc_edge_update));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_tab, std::max<size_t>(B->tab.size(), 1) * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_seg, std::max<size_t>(seg.size(), 1) * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_inround, std::max<size_t>(B->inround.size(), 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_ctl, RC_WORDS * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_fl, (size_t)F_KINDS * ng * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_pub, (size_t)ng * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_state, (size_t)ng * sizeof(RoundState));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_rg, (size_t)ng * sizeof(RoundGraph));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_snapbase, (size_t)ng * sizeof(uint16_t*));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_hist, (size_t)ng * sizeof(HistDev));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&B->h_ctl, RC_WORDS * 4);
+    auto up = [&](void* d, const void* h, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream);
+    };
+    up(B->d_stream, ordered.data(), ordered.size() * sizeof(sgc_edge_update));
+    up(B->d_tab, B->tab.data(), B->tab.size() * 4);
+    up(B->d_seg, seg.data(), seg.size() * 4);
+    up(B->d_inround, B->inround.data(), B->inround.size());
+    up(B->d_rg, rg.data(), rg.size() * sizeof(RoundGraph));
+    up(B->d_snapbase, base.data(), base.size() * sizeof(uint16_t*));
+    if (e == hipSuccess) e = hipMemsetAsync(B->d_fl, 0, (size_t)F_KINDS * ng * 4, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(B->d_state, 0, (size_t)ng * sizeof(RoundState), stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "batch prepare");
+    // launch-shape candidates: the pool's shapes, then one knob changed at a time
+    B->cands.push_back(tune);
+    if (c.use_graph && c.autotune_rounds) {
+        sgc_tuning t = tune;
+        t.long_row = tune.long_row > 256 ? 256 : 4096;
+        B->cands.push_back(t);
+        t = tune;
+        t.sweep_blocks = tune.sweep_blocks > 512 ? 512 : 4096;
+        B->cands.push_back(t);
+        t = tune;
+        t.apply_blocks = tune.apply_blocks > 4096 ? 4096 : 16384;
+        B->cands.push_back(t);
+    }
+    if (c.use_graph) {
+        for (const sgc_tuning& t : B->cands) {
+            hipGraphExec_t x = nullptr;
+            int r = batch_capture(t, &x);
+            if (r) return r;
+            B->execs.push_back(x);
+        }
+    }
+    if (rounds_out) *rounds_out = B->rounds;
+    return SGC_OK;
+}
+
+// One round, captured: apply the round's slice, compact, drain, the Tier-1
+// sweeps with device-side control, the Tier-2 check, rebuild, publish.  Every
+// launch takes its per-round inputs from device memory, so one instantiated
+// graph serves every round.
+int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
+    BatchRun& B = *batch;
+    const bool det = deterministic();
+    LaunchCtx base = ctx(false);
+    base.max_blocks = t.apply_blocks;
+    base.sweep_blocks = t.sweep_blocks;
+    base.long_row = t.long_row;
+    base.list = d_list;
+    base.list_count = d_list_count;
+    base.list_cap = list_cap;
+    auto F = [&](uint32_t k) { return (const uint32_t*)(B.d_fl + (size_t)k * ng); };
+    const uint32_t* t1_any = B.d_ctl + RC_T1_ANY;
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "batch capture");
+    e = launch_round_begin(stream, B.d_ctl, B.d_tab, B.d_inround, ng, B.d_fl, B.d_state, B.d_rg);
+    LaunchCtx La = base;
+    La.dyn = B.d_ctl + RC_OFF;
+    if (e == hipSuccess) {
+        if (det) {
+            e = launch_apply_seg(La, B.d_stream, B.d_seg, ng, nullptr, B.multi ? 1 : 0);
+        } else {
+            e = launch_group(La, B.d_stream, B.max_count, B.multi ? 1 : 0, &eg);
+            if (e == hipSuccess) e = launch_apply(La, B.d_stream, B.max_count, B.multi ? 1 : 0, &eg);
+        }
+    }
+    LaunchCtx Lr = base;
+    Lr.active = F(F_ROUND);
+    if (e == hipSuccess) e = launch_build_list(Lr, M_ROW, d_list, list_cap, d_list_count);
+    if (e == hipSuccess) e = launch_compact(Lr);
+    if (e == hipSuccess) e = launch_drain(Lr, det, B.ovf_cap);
+    for (uint32_t it = 0; it < B.sweeps && e == hipSuccess; it++) {
+        e = launch_t1_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.d_rg, it > 0, true, det,
+                              t.serial_repair, t.serial_conflicts);
+        if (e == hipSuccess && rebuild_each_sweep()) {
+            LaunchCtx Lb = base;
+            Lb.active = F(F_ITER);
+            Lb.gate = t1_any;
+            e = launch_build_list(Lb, M_BITS, d_list, list_cap, d_list_count);
+            if (e == hipSuccess) e = launch_rebuild(Lb, exact_hist());
+        }
+        if (e == hipSuccess && !det) {
+            LaunchCtx Lp = base;
+            Lp.active = F(F_PAR);
+            Lp.gate = t1_any;
+            e = launch_build_list(Lp, M_CHECK, d_list, list_cap, d_list_count);
+            if (e == hipSuccess) e = launch_tier1(Lp, false, (int)t.fresh_repair);
+        }
+        if (e == hipSuccess) {
+            LaunchCtx Ls = base;
+            Ls.active = F(F_SER);
+            e = launch_tier1(Ls, true, det ? 0 : (int)t.fresh_repair);
+        }
+    }
+    if (e == hipSuccess)
+        e = launch_t1_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.d_rg, true, false, det,
+                              t.serial_repair, t.serial_conflicts);
+    if (e == hipSuccess) e = launch_t2_prep(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state);
+    LaunchCtx Lt = base;
+    Lt.active = F(F_T2);
+    if (e == hipSuccess) e = launch_build_list(Lt, M_T2, d_list, list_cap, d_list_count);
+    if (e == hipSuccess) e = launch_t2_check(Lt);
+    if (e == hipSuccess) e = launch_round_end(stream, B.d_ctl, graphs.d, ng, B.d_fl);
+    if (e == hipSuccess && !det) {   // bitmaps and histograms exact again for the next updates
+        e = launch_build_list(Lr, M_BITS, d_list, list_cap, d_list_count);
+        if (e == hipSuccess) e = launch_rebuild(Lr, exact_hist());
+    }
+    if (e == hipSuccess) e = launch_snapshot(Lr);
+    if (e == hipSuccess)
+        e = launch_round_publish(stream, graphs.d, ng, B.max_n, F(F_ROUND), B.d_snapbase, B.d_pub, B.d_hist,
+                                 B.any_history);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(stream, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "batch capture");
+    return SGC_OK;
+}
+
+// Host state -> device: each graph's next snapshot buffer, its published
+// half and its history slot.
+hipError_t sgc_pool::batch_push(BatchRun& B) {
+    hipError_t e = graphs.writable();
+    if (e != hipSuccess) return e;
+    std::vector<uint32_t> pub(ng, 0xFFFFFFFFu);
+    std::vector<HistDev> hist(ng);
+    for (uint32_t g = 0; g < ng; g++) {
+        hist[g] = HistDev{nullptr, 0, 0};
+        if (!gs[g].used) continue;
+        const GraphState& G = gs[g];
+        graphs.h[g].snap_write = G.snap + (size_t)(G.published == 0 ? 1 : 0) * G.n;
+        if (G.published >= 0) pub[g] = (uint32_t)G.published;
+        hist[g] = HistDev{G.history, G.hist_cap, G.hist_count};
+    }
+    e = graphs.upload(stream, 0, ng);
+    if (e == hipSuccess) e = hipMemcpyAsync(B.d_pub, pub.data(), (size_t)ng * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(B.d_hist, hist.data(), (size_t)ng * sizeof(HistDev), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(B.d_ctl, B.h_ctl, RC_WORDS * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
+}
+
+// Device state -> host: published halves and history counts.
+hipError_t sgc_pool::batch_pull(BatchRun& B) {
+    std::vector<uint32_t> pub(ng);
+    std::vector<HistDev> hist(ng);
+    hipError_t e = hipMemcpyAsync(pub.data(), B.d_pub, (size_t)ng * 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hist.data(), B.d_hist, (size_t)ng * sizeof(HistDev), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    for (uint32_t g = 0; g < ng; g++) {
+        if (!gs[g].used) continue;
+        if (pub[g] != 0xFFFFFFFFu) gs[g].published = (int)pub[g];
+        if (gs[g].history) gs[g].hist_count = hist[g].count;
+    }
+    return hipSuccess;
+}
+
+// Finish round r on the host from where the captured round stopped: more
+// Tier-1 sweeps for graphs still iterating, the Tier-2 check and repair,
+// rebuild, publish.
+int sgc_pool::batch_host_round(BatchRun& B, uint32_t r) {
+    std::vector<RoundState> st(ng);
+    hipError_t e = hipMemcpyAsync(st.data(), B.d_state, (size_t)ng * sizeof(RoundState), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = batch_pull(B);
+    if (e != hipSuccess) return hip_fail(e, "batch state");
+    const std::vector<uint32_t>& gids = B.gids[r];
+    bool resume = false;
+    for (uint32_t g : gids) {
+        GraphState& G = gs[g];
+        G.iterating = st[g].iterating != 0;
+        G.prev = st[g].prev;
+        G.plateau = st[g].plateau;
+        G.t1_clean = st[g].clean != 0;
+        G.st.t1_iterations = st[g].iters;
+        G.st.t1_conflicts_last = st[g].last;
+        resume |= G.iterating;
+    }
+    int rc;
+    if (resume && (rc = tier1(gids, B.sweeps))) return rc;
+    e = read_ctr();
+    if (e != hipSuccess) return hip_fail(e, "counters");
+    if ((rc = tier2(gids))) return rc;
+    if (!deterministic()) {
+        e = set_active(gids);
+        if (e == hipSuccess) e = rebuild(ctx(true));
+        if (e != hipSuccess) return hip_fail(e, "rebuild");
+    }
+    if ((rc = publish(gids))) return rc;
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "batch round");
+    return SGC_OK;
+}
+
+// The same rounds with host-driven launches (no capture).
+int sgc_pool::batch_run_host(BatchRun& B) {
+    for (uint32_t r = 0; r < B.rounds; r++) {
+        LaunchCtx L = ctx(false);
+        const uint32_t off = B.tab[2 * r], cnt = B.tab[2 * r + 1];
+        hipError_t e;
+        if (deterministic()) {
+            e = timed("apply", [&] {
+                return launch_apply_seg(L, B.d_stream, B.d_seg + (size_t)r * 2 * ng, ng, nullptr, B.multi ? 1 : 0);
+            });
+        } else {
+            e = timed("group", [&] { return launch_group(L, B.d_stream + off, cnt, B.multi ? 1 : 0, &eg); });
+            if (e == hipSuccess)
+                e = timed("apply", [&] { return launch_apply(L, B.d_stream + off, cnt, B.multi ? 1 : 0, &eg); });
+        }
+        if (e != hipSuccess) return hip_fail(e, "batch apply");
+        for (uint32_t g : B.gids[r]) gs[g].since_epoch += B.counts[(size_t)r * ng + g];
+        int rc = epoch(B.gids[r], false, nullptr);
+        if (rc) return rc;
+        for (uint32_t g : B.gids[r])
+            if (!B.full[(size_t)r * ng + g]) gs[g].epochs--;
+    }
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "batch run");
+    resolve_timing();
+    return SGC_OK;
+}
+
+int sgc_pool::batch_run(sgc_batch_stats* out) {
+    if (!batch) return fail(SGC_ERR_INVALID, "batch run: nothing prepared");
+    BatchRun& B = *batch;
+    if (B.ng != ng) return fail(SGC_ERR_INVALID, "batch run: graphs were added or removed since the plan was made");
+    sgc_batch_stats S;
+    memset(&S, 0, sizeof S);
+    S.rounds = B.rounds;
+    S.updates = B.updates;
+    std::lock_guard<std::mutex> run(run_mu);   // queries wait for the run
+    hipError_t e = hipStreamSynchronize(qstream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "batch run");
+    auto t0 = std::chrono::steady_clock::now();
+    if (!B.cfg.use_graph) {
+        int rc = batch_run_host(B);
+        if (rc) return rc;
+    } else {
+        memset(B.h_ctl, 0, RC_WORDS * 4);
+        B.h_ctl[RC_ROUNDS] = B.rounds;
+        e = batch_push(B);
+        if (e != hipSuccess) return hip_fail(e, "batch run");
+        const uint32_t C = (uint32_t)B.execs.size();
+        const uint32_t tune_rounds = C > 1 ? std::min(B.rounds, B.cfg.autotune_rounds * C) : 0;
+        std::vector<hipEvent_t> ev(2 * (size_t)tune_rounds, nullptr);
+        for (auto& x : ev)
+            if (e == hipSuccess) e = hipEventCreate(&x);
+        std::vector<double> cand_ms(C, 0.0);
+        std::vector<uint64_t> cand_upd(C, 0);
+        uint32_t best = 0, r = 0;
+        bool tuned = tune_rounds == 0;
+        int rc = SGC_OK;
+        while (e == hipSuccess && rc == SGC_OK && r < B.rounds) {
+            const uint32_t end = tuned ? B.rounds : tune_rounds;
+            for (uint32_t k = r; k < end && e == hipSuccess; k++) {
+                if (!tuned) e = hipEventRecord(ev[2 * k], stream);
+                if (e == hipSuccess) e = hipGraphLaunch(B.execs[tuned ? best : k % C], stream);
+                if (e == hipSuccess && !tuned) e = hipEventRecord(ev[2 * k + 1], stream);
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(B.h_ctl, B.d_ctl, RC_WORDS * 4, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) break;
+            const bool stopped = B.h_ctl[RC_STOP] != 0;
+            const uint32_t done = stopped ? B.h_ctl[RC_STOP_ROUND] : end;
+            if (!tuned)
+                for (uint32_t k = r; k < done; k++) {
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+                    cand_ms[k % C] += ms;
+                    cand_upd[k % C] += B.tab[2 * k + 1];
+                }
+            if (stopped) {
+                const uint32_t s = B.h_ctl[RC_STOP_ROUND];
+                S.host_rounds++;
+                rc = batch_host_round(B, s);
+                if (rc) break;
+                r = s + 1;
+                B.h_ctl[RC_ROUND] = r;
+                B.h_ctl[RC_STOP] = 0;
+                e = batch_push(B);
+            } else {
+                r = end;
+            }
+            if (!tuned && r >= tune_rounds) {
+                tuned = true;
+                double best_v = 0;
+                for (uint32_t k = 0; k < C; k++) {
+                    if (!cand_upd[k]) continue;
+                    const double v = cand_ms[k] / (double)cand_upd[k];
+                    if (best_v == 0 || v < best_v) {
+                        best_v = v;
+                        best = k;
+                    }
+                }
+                tune = B.cands[best];
+            }
+        }
+        for (auto& x : ev)
+            if (x) (void)hipEventDestroy(x);
+        if (rc) return rc;
+        if (e == hipSuccess) e = batch_pull(B);
+        for (uint32_t g = 0; g < ng && e == hipSuccess; g++)
+            if (gs[g].used && gs[g].published >= 0) e = hipEventRecord(gs[g].pub_ev[gs[g].published], stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "batch run");
+        for (uint32_t r2 = 0; r2 < B.rounds; r2++)
+            for (uint32_t g : B.gids[r2]) gs[g].epochs += B.full[(size_t)r2 * ng + g];
+        for (uint32_t g = 0; g < ng; g++)
+            if (gs[g].used) gs[g].since_epoch = 0;
+        S.candidates = C > 1 ? C : 0;
+        S.chosen = best;
+        for (uint32_t k = 0; k < C && k < SGC_BATCH_CANDIDATES; k++) {
+            S.candidate[k] = B.cands[k];
+            S.candidate_ns_per_update[k] = cand_upd[k] ? cand_ms[k] * 1e6 / (double)cand_upd[k] : 0.0;
+        }
+    }
+    S.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (out) *out = S;
+    return SGC_OK;
 }
 
 // The single-graph engine is a pool of one.
@@ -1055,6 +1551,7 @@ int query_impl(sgc_pool* P, uint32_t gid, const uint32_t* ids, uint32_t count, u
     GraphState* G = graph_of(P, gid);
     if (!G || (!ids && count) || (!out && count)) return SGC_ERR_INVALID;
     if (count == 0) return SGC_OK;
+    std::lock_guard<std::mutex> run(P->run_mu);
     std::lock_guard<std::mutex> lk(P->qmu);
     if (G->published < 0) {
         memset(out, 0, (size_t)count * 2);
@@ -1180,6 +1677,7 @@ int sgc_pool_set_tuning(sgc_pool* p, const sgc_tuning* t) { return set_tuning(p,
 int sgc_set_flags(sgc_engine* e, uint32_t flags) {
     if (!e || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))) return SGC_ERR_INVALID;
     const bool was = e->tracking();
+    if (flags != e->flags) e->batch_drop();
     e->flags = flags;
     e->gs[0].cfg.flags = flags;
     if (e->tracking() != was) {
@@ -1547,6 +2045,7 @@ int sgc_pool_usage(sgc_pool* p, uint64_t* used, uint64_t* budget, uint32_t* grap
 int sgc_pool_set_flags(sgc_pool* p, uint32_t flags) {
     if (!p || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))) return SGC_ERR_INVALID;
     const bool was = p->tracking();
+    if (flags != p->flags) p->batch_drop();   // captured rounds follow the flags they were made with
     p->flags = flags;
     for (auto& G : p->gs) G.cfg.flags = flags;
     if (p->tracking() != was) {
@@ -1622,6 +2121,16 @@ int sgc_pool_download_words(sgc_pool* p, uint32_t gid, uint32_t* words) {
     if (!G || !words) return SGC_ERR_INVALID;
     return download(p, words, G->word, (size_t)G->n * 4);
 }
+int sgc_pool_upload_words(sgc_pool* p, uint32_t gid, const uint32_t* words) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || !words) return SGC_ERR_INVALID;
+    int r = upload(p, G->word, words, (size_t)G->n * 4);
+    if (r) return r;
+    hipError_t e = p->mark_all(gid);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return p->hip_fail(e, "mark");
+    return SGC_OK;
+}
 int sgc_pool_download_snapshot(sgc_pool* p, uint32_t gid, uint16_t* colors) {
     GraphState* G = graph_of(p, gid);
     if (!G || !colors) return SGC_ERR_INVALID;
@@ -1655,6 +2164,36 @@ int sgc_pool_validate_history(sgc_pool* p, uint32_t gid, const uint32_t* eu, con
 int sgc_pool_totals(sgc_pool* p, sgc_totals* out) {
     if (!p || !out) return SGC_ERR_INVALID;
     return totals_impl(p, out);
+}
+int sgc_pool_set_epoch_size(sgc_pool* p, uint32_t gid, uint32_t epoch_size) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || epoch_size == 0) return SGC_ERR_INVALID;
+    G->cfg.epoch_size = epoch_size;
+    return SGC_OK;
+}
+void sgc_default_batch_config(sgc_batch_config* c) {
+    if (!c) return;
+    c->use_graph = 1;
+    c->t1_sweeps = 0;
+    c->autotune_rounds = 0;
+    c->reserved = 0;
+}
+int sgc_pool_batch_prepare(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, const sgc_batch_config* cfg,
+                           uint32_t* rounds) {
+    if (!p || (!updates && count)) return SGC_ERR_INVALID;
+    sgc_batch_config c;
+    sgc_default_batch_config(&c);
+    if (cfg) c = *cfg;
+    return p->batch_prepare(updates, count, c, rounds);
+}
+int sgc_pool_batch_run(sgc_pool* p, sgc_batch_stats* out) {
+    if (!p) return SGC_ERR_INVALID;
+    return p->batch_run(out);
+}
+int sgc_pool_batch_release(sgc_pool* p) {
+    if (!p) return SGC_ERR_INVALID;
+    p->batch_drop();
+    return SGC_OK;
 }
 int sgc_pool_kernel_timing_enable(sgc_pool* p, int enable) {
     if (!p) return SGC_ERR_INVALID;

@@ -113,8 +113,10 @@ __device__ __forceinline__ uint32_t block_scan32(uint32_t x, uint32_t* total) {
 constexpr uint32_t LIST_TILE = 16;
 __global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__ graphs, Space sp, uint32_t kind,
                                                     uint32_t long_row, uint32_t* __restrict__ list, uint32_t cap,
-                                                    uint32_t* __restrict__ count) {
+                                                    uint32_t* __restrict__ count, const uint32_t* __restrict__ active,
+                                                    const uint32_t* __restrict__ gate) {
     __shared__ uint32_t wave_small[BLOCK / WAVE], wave_big[BLOCK / WAVE], block_at[2];
+    if (gate && *gate == 0) return;
     const uint64_t lt_mask = (1ull << lane()) - 1ull;
     const uint32_t wv = threadIdx.x / WAVE;
     const uint32_t span = BLOCK * LIST_TILE;
@@ -126,9 +128,9 @@ __global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__
         for (uint32_t t = 0; t < LIST_TILE; t++) {
             const uint32_t i = base + t * WAVE + lane();
             bool hit = false, big = false;
-            if (i < sp.total) {
-                uint32_t gid, v;
-                sp.locate(i, gid, v);
+            uint32_t gid = 0, v = 0;
+            if (i < sp.total) sp.locate(i, gid, v);
+            if (i < sp.total && (!active || active[gid])) {
                 const GraphDev& g = graphs[gid];
                 SGC_G uint8_t* d = gp(g.dirty) + v;
                 const size_t st = g.dstride;
@@ -240,7 +242,12 @@ __global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__
                                                     const uint32_t* __restrict__ gbase,
                                                     const uint4* __restrict__ upd, uint32_t count, int multi,
                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ head,
-                                                    uint32_t* __restrict__ next, uint32_t mask) {
+                                                    uint32_t* __restrict__ next, uint32_t mask,
+                                                    const uint32_t* __restrict__ dyn) {
+    if (dyn) {   // the batch is a slice of a device-resident stream: [offset, count] read here
+        upd += dyn[0];
+        count = dyn[1];
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
         uint32_t prev = NONE;
@@ -268,9 +275,13 @@ __global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__
                                                     const uint64_t* __restrict__ gkeys,
                                                     const uint32_t* __restrict__ ghead,
                                                     const uint32_t* __restrict__ gnext, uint32_t gmask,
-                                                    uint64_t* totals) {
+                                                    uint64_t* totals, const uint32_t* __restrict__ dyn) {
     WaveAcc acc;
     uint64_t grouped = 0;
+    if (dyn) {
+        upd += dyn[0];
+        count = dyn[1];
+    }
     for (uint32_t i = wave_id(); i < count; i += wave_count()) {
         // the updates this wave applies for position i, in stream order: i
         // alone, or the whole group of i's edge when this wave owns its list
@@ -311,8 +322,12 @@ __global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__
 __global__ __launch_bounds__(256) void k_apply_seg(const GraphDev* __restrict__ graphs,
                                                    const uint4* __restrict__ upd, const uint32_t* __restrict__ seg,
                                                    uint32_t nseg, const uint32_t* __restrict__ perm, int multi,
-                                                   uint64_t* totals) {
+                                                   uint64_t* totals, const uint32_t* __restrict__ dyn) {
     WaveAcc acc;
+    if (dyn) {   // segments of the current round (absolute stream positions); none when its count is 0
+        if (dyn[1] == 0) return;
+        seg += dyn[2];
+    }
     for (uint32_t k = wave_id(); k < nseg; k += wave_count())
         for (uint32_t i = seg[2 * k]; i < seg[2 * k + 1]; i++)
             apply_record(graphs, upd, perm ? perm[i] : i, multi, acc);
@@ -351,10 +366,12 @@ __global__ __launch_bounds__(256) void k_max_color(const GraphDev* __restrict__ 
 
 // ---------------------------------------------------------------- snapshot
 // A/src/kernels/epoch_maintenance.cu:5-16, into the back buffer.
-__global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ graphs, Space sp) {
+__global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ graphs, Space sp,
+                                                  const uint32_t* __restrict__ active) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sp.total; i += gridDim.x * blockDim.x) {
         uint32_t gid, v;
         sp.locate(i, gid, v);
+        if (active && !active[gid]) continue;
         const GraphDev& g = graphs[gid];
         gp(g.snap_write)[v] = color_of(gp(g.word)[v]);
     }
@@ -1146,7 +1163,7 @@ hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int
     }
     uint32_t grid = grid_for_waves(count, 4, L.max_blocks);
     k_apply_elem<<<grid, 256, 0, L.stream>>>(L.graphs, (const uint4*)upd, count, multi, L.gbase, keys, head, next,
-                                              mask, L.totals);
+                                              mask, L.totals, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg) {
@@ -1157,14 +1174,14 @@ hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int
     k_group_clear<<<cg, 256, 0, L.stream>>>(eg->keys, eg->head, eg->size);
     uint32_t mg = grid_for_waves(count, 4 * WAVE, 1024);
     k_group_link<<<mg, 256, 0, L.stream>>>(L.graphs, L.gbase, (const uint4*)upd, count, multi, eg->keys, eg->head,
-                                           eg->next, mask);
+                                           eg->next, mask, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg,
                             const uint32_t* perm, int multi) {
     if (nseg == 0) return hipSuccess;
     uint32_t grid = grid_for_waves(nseg, 1, L.max_blocks);
-    k_apply_seg<<<grid, 64, 0, L.stream>>>(L.graphs, (const uint4*)upd, seg, nseg, perm, multi, L.totals);
+    k_apply_seg<<<grid, 64, 0, L.stream>>>(L.graphs, (const uint4*)upd, seg, nseg, perm, multi, L.totals, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_zero_ctr(const LaunchCtx& L, uint32_t idx) {
@@ -1178,7 +1195,7 @@ hipError_t launch_max_color(const LaunchCtx& L) {
 }
 hipError_t launch_snapshot(const LaunchCtx& L) {
     uint32_t grid = grid_for_waves(L.V, 4 * WAVE, L.max_blocks);
-    k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, space(L));
+    k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), L.active);
     return hipGetLastError();
 }
 // Sweep grids: one wavefront per row (blocks first take the long rows of a
@@ -1192,8 +1209,8 @@ hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, 
     if (kind >= M_STICKY || cap < L.V) return hipErrorInvalidValue;
     k_zero_list_count<<<1, 1, 0, L.stream>>>(count);
     uint32_t grid = grid_for_waves((L.V + WAVE * LIST_TILE - 1) / (WAVE * LIST_TILE), BLOCK / WAVE, 4096);
-    k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, list, cap,
-                                             count);
+    k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, list, cap, count, L.active,
+                                             L.gate);
     return hipGetLastError();
 }
 hipError_t launch_compact(const LaunchCtx& L) {
@@ -1205,7 +1222,9 @@ hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
     if (serial) {
         k_drain_serial<<<L.ng < 1024 ? L.ng : 1024, 64, 0, L.stream>>>(L.graphs, L.ng, L.active);
     } else {
-        dim3 grid(grid_for_waves(max_count, 4, 4096), L.ng);
+        // entries of one graph share grid.x blocks; many graphs take fewer each
+        const uint32_t per_graph = L.ng > 4 ? (16384 / L.ng > 0 ? 16384 / L.ng : 1) : 4096;
+        dim3 grid(grid_for_waves(max_count, 4, per_graph), L.ng);
         k_drain_elem<<<grid, 256, 0, L.stream>>>(L.graphs, L.active);
     }
     return hipGetLastError();

@@ -27,6 +27,10 @@ struct LaunchCtx {
     const uint32_t* list = nullptr;    // device worklist of virtual vertices (null = sweep all)
     const uint32_t* list_count = nullptr;  // device [rows at the front, long rows at the back]
     uint32_t list_cap = 0;
+    // Device-resident launch parameters (captured launches of a batched run):
+    const uint32_t* dyn = nullptr;     // [offset, count, segment offset] of the batch in the resident stream;
+                                       // the launchers' count then only sizes the grid
+    const uint32_t* gate = nullptr;    // worklist builds do nothing while *gate == 0
 };
 
 // Per-edge grouping of a parallel batch: hash table (size: power of two >=
@@ -61,6 +65,43 @@ hipError_t launch_find_color(const LaunchCtx& L, uint32_t gid, uint32_t v, uint1
 hipError_t launch_validate_history(hipStream_t s, const uint16_t* snaps, uint32_t n, const uint32_t* eu,
                                    const uint32_t* ev, const uint64_t* born, const uint64_t* died, uint64_t m,
                                    const uint64_t* pos, uint32_t count, unsigned long long* bad_out);
+// ---------------------------------------------------------------- batched runs
+// Control block of a batched run (device u32 words).
+enum : uint32_t {
+    RC_ROUND = 0,     // next round to run
+    RC_ROUNDS,        // rounds in the plan
+    RC_STOP,          // 1: a round needs the host; later replays do nothing
+    RC_STOP_ROUND,    // the round that stopped
+    RC_T1_ANY,        // some graph runs another Tier-1 sweep
+    RC_OFF,           // current round: first record in the resident stream
+    RC_COUNT,         //   its record count
+    RC_SEG,           //   offset of its segment table (deterministic replay)
+    RC_WORDS = 16
+};
+// Per-graph flag arrays (ng u32 each): taking part in the round, Tier-1
+// iterating, parallel sweep next, serial sweep next, Tier-2 check.
+enum : uint32_t { F_ROUND = 0, F_ITER, F_PAR, F_SER, F_T2, F_KINDS };
+struct RoundState {   // Tier-1 loop state of one graph in the current round
+    uint32_t iterating, prev, plateau, iters, last, clean;
+};
+struct RoundGraph {   // constant per graph during a run
+    uint32_t max_iters, plateau_iters, n;
+};
+struct HistDev {      // a graph's snapshot history
+    uint16_t* base;
+    uint32_t cap, count;
+};
+hipError_t launch_round_begin(hipStream_t s, uint32_t* ctl, const uint32_t* tab, const uint8_t* inround, uint32_t ng,
+                              uint32_t* fl, RoundState* st, const RoundGraph* rg);
+hipError_t launch_t1_control(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
+                             RoundState* st, const RoundGraph* rg, bool after, bool before, bool deterministic,
+                             uint32_t serial_repair, uint32_t serial_conflicts);
+hipError_t launch_t2_prep(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
+                          const RoundState* st);
+hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl);
+hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, uint32_t max_n, const uint32_t* fl_round,
+                                uint16_t* const* snap_base, uint32_t* pub_half, HistDev* hist, bool any_history);
+
 hipError_t launch_fill_rows(hipStream_t s, uint32_t n, const uint32_t* rp0, const uint32_t* col0, const uint32_t* rp,
                             uint32_t* col, uint32_t* fill);
 

@@ -1,0 +1,242 @@
+// sgc_rounds.hip -- device-side control of a batched run.
+//
+// A batched run replays one captured round body per round.  Everything that
+// changes from round to round lives in device memory and is read by the
+// kernels themselves: the round index, the round's slice of the resident
+// stream, which graphs take part, the Tier-1 loop state of every graph
+// (iterations, plateau, serial or parallel sweep) and the snapshot buffer each
+// graph writes next.  The host replays the body and synchronises only when a
+// round needs it (a Tier-2 repair, or Tier-1 still running after the captured
+// iterations); then the control block says which round stopped, and every
+// kernel of the replays after it does nothing.
+//
+// The Tier-1 control follows run_conflict_repair (A/src/streamgc.cu:1317-1393):
+// a graph stops at zero conflicts, at its iteration cap, or after
+// plateau_iters sweeps with an unchanged count.
+#include <hip/hip_runtime.h>
+
+#include "sgc_device.hpp"
+#include "sgc_launch.hpp"
+
+namespace sgc {
+
+constexpr uint32_t CTL_THREADS = 1024;
+
+// Block-wide OR of a flag (single-block control kernels).
+__device__ __forceinline__ uint32_t block_any(uint32_t x) {
+    __shared__ uint32_t part[CTL_THREADS / WAVE];
+    const uint32_t w = threadIdx.x / WAVE;
+    const uint32_t any = ballot(x != 0) != 0;
+    if (lane() == 0) part[w] = any;
+    __syncthreads();
+    uint32_t t = 0;
+    for (uint32_t k = 0; k < blockDim.x / WAVE; k++) t |= part[k];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ uint32_t* flag(uint32_t* fl, uint32_t ng, uint32_t kind, uint32_t g) {
+    return fl + (size_t)kind * ng + g;
+}
+
+// Start of a round: the graphs taking part, their Tier-1 state, the slice.
+// After a stop (or past the last round) every flag is cleared and the slice
+// is empty, and the stopped round's Tier-1 state is left for the host.
+__global__ __launch_bounds__(CTL_THREADS) void k_round_begin(uint32_t* __restrict__ ctl,
+                                                             const uint32_t* __restrict__ tab,
+                                                             const uint8_t* __restrict__ inround, uint32_t ng,
+                                                             uint32_t* __restrict__ fl, RoundState* __restrict__ st,
+                                                             const RoundGraph* __restrict__ rg) {
+    const uint32_t r = ctl[RC_ROUND];
+    const bool live = ctl[RC_STOP] == 0 && r < ctl[RC_ROUNDS];
+    for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
+        const uint32_t in = live ? inround[(size_t)r * ng + g] : 0u;
+        const uint32_t it = in && rg[g].max_iters > 0;
+        if (live) {
+            RoundState s;
+            s.prev = 0xFFFFFFFFu;
+            s.plateau = 0;
+            s.iters = 0;
+            s.last = 0;
+            s.clean = 0;
+            s.iterating = it;
+            st[g] = s;
+        }
+        *flag(fl, ng, F_ROUND, g) = in;
+        *flag(fl, ng, F_ITER, g) = it;
+        *flag(fl, ng, F_PAR, g) = 0;
+        *flag(fl, ng, F_SER, g) = 0;
+        *flag(fl, ng, F_T2, g) = 0;
+    }
+    if (threadIdx.x == 0) {
+        ctl[RC_OFF] = live ? tab[2 * (size_t)r] : 0u;
+        ctl[RC_COUNT] = live ? tab[2 * (size_t)r + 1] : 0u;
+        ctl[RC_SEG] = live ? r * 2u * ng : 0u;
+        ctl[RC_T1_ANY] = 0;
+    }
+}
+
+// Between Tier-1 sweeps.  after: fold the last sweep's conflict count into
+// each iterating graph's state (stop on zero, cap or plateau).  before: pick
+// the next sweep of each graph still iterating -- serial (vertex order, one
+// wavefront) when deterministic or when the graph is small and its last sweep
+// left few conflicts, else parallel -- and clear its conflict counter.
+__global__ __launch_bounds__(CTL_THREADS) void k_t1_control(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
+                                                            uint32_t ng, uint32_t* __restrict__ fl,
+                                                            RoundState* __restrict__ st,
+                                                            const RoundGraph* __restrict__ rg, int after, int before,
+                                                            int deterministic, uint32_t serial_repair,
+                                                            uint32_t serial_conflicts) {
+    uint32_t any = 0;
+    for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
+        uint32_t it = *flag(fl, ng, F_ITER, g);
+        if (!it) {
+            *flag(fl, ng, F_PAR, g) = 0;
+            *flag(fl, ng, F_SER, g) = 0;
+            continue;
+        }
+        RoundState s = st[g];
+        SGC_G uint32_t* c = gp(graphs[g].ctr);
+        if (after) {
+            const uint32_t k = c[C_CONFLICTS];
+            s.iters++;
+            s.last = k;
+            s.clean = k == 0;
+            if (k == 0 || s.iters >= rg[g].max_iters) {
+                it = 0;
+            } else if (k == s.prev) {
+                if (++s.plateau >= rg[g].plateau_iters) it = 0;
+            } else {
+                s.plateau = 0;
+            }
+            s.prev = k;
+            s.iterating = it;
+            st[g] = s;
+            *flag(fl, ng, F_ITER, g) = it;
+        }
+        const bool run = before && it;
+        const bool serial = deterministic || (rg[g].n <= serial_repair && s.prev <= serial_conflicts);
+        *flag(fl, ng, F_PAR, g) = run && !serial;
+        *flag(fl, ng, F_SER, g) = run && serial;
+        if (run) c[C_CONFLICTS] = 0;
+        any |= run;
+    }
+    any = block_any(any);
+    if (threadIdx.x == 0) ctl[RC_T1_ANY] = any;
+}
+
+// After the captured Tier-1 sweeps: a graph still iterating stops the round
+// (the host finishes its repair).  The others need the Tier-2 check unless
+// their last sweep was clean and no overflow ever left an edge in one row.
+__global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
+                                                         uint32_t ng, uint32_t* __restrict__ fl,
+                                                         const RoundState* __restrict__ st) {
+    uint32_t stop = 0;
+    for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
+        uint32_t need = 0;
+        if (*flag(fl, ng, F_ROUND, g)) {
+            if (*flag(fl, ng, F_ITER, g)) {
+                stop = 1;
+            } else {
+                SGC_G uint32_t* c = gp(graphs[g].ctr);
+                need = !(st[g].clean && c[C_ASYM] == 0);
+                if (need) c[C_T2BAD] = 0;
+            }
+        }
+        *flag(fl, ng, F_T2, g) = need;
+    }
+    stop = block_any(stop);
+    if (threadIdx.x == 0 && stop && ctl[RC_STOP] == 0) {
+        ctl[RC_STOP] = 1;
+        ctl[RC_STOP_ROUND] = ctl[RC_ROUND];
+    }
+}
+
+// After the Tier-2 check: a graph with a bad slot stops the round.  A stopped
+// round publishes nothing (its graphs leave the round); otherwise the round
+// index advances.
+__global__ __launch_bounds__(CTL_THREADS) void k_round_end(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
+                                                           uint32_t ng, uint32_t* __restrict__ fl) {
+    uint32_t bad = 0;
+    for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x)
+        if (*flag(fl, ng, F_ROUND, g) && *flag(fl, ng, F_T2, g) && gp(graphs[g].ctr)[C_T2BAD] != 0) bad = 1;
+    bad = block_any(bad);
+    const uint32_t stopped = ctl[RC_STOP];
+    const uint32_t r = ctl[RC_ROUND];
+    __syncthreads();
+    const bool stop = stopped || bad;
+    if (stop)
+        for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) *flag(fl, ng, F_ROUND, g) = 0;
+    if (threadIdx.x == 0) {
+        if (bad && !stopped) {
+            ctl[RC_STOP] = 1;
+            ctl[RC_STOP_ROUND] = r;
+        }
+        if (!stop && r < ctl[RC_ROUNDS]) ctl[RC_ROUND] = r + 1;
+    }
+}
+
+// Keep a copy of the snapshot just written (the graph's history, if any).
+// Grid: x over 2048-vertex chunks, y over graphs.
+__global__ __launch_bounds__(256) void k_round_history(const GraphDev* __restrict__ graphs,
+                                                       const uint32_t* __restrict__ fl_round,
+                                                       const HistDev* __restrict__ hist) {
+    const uint32_t g = blockIdx.y;
+    if (!fl_round[g]) return;
+    const HistDev h = hist[g];
+    if (!h.base || h.count >= h.cap) return;
+    const uint32_t n = graphs[g].n;
+    const uint16_t* src = graphs[g].snap_write;
+    uint16_t* dst = h.base + (size_t)h.count * n;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) dst[v] = src[v];
+}
+
+// The snapshot just written becomes the published one; the next publish
+// writes the other buffer.
+__global__ void k_round_flip(GraphDev* __restrict__ graphs, uint32_t ng, const uint32_t* __restrict__ fl_round,
+                             uint16_t* const* __restrict__ snap_base, uint32_t* __restrict__ pub_half,
+                             HistDev* __restrict__ hist) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng || !fl_round[g]) return;
+    uint16_t* base = snap_base[g];
+    const uint32_t n = graphs[g].n;
+    const uint32_t half = graphs[g].snap_write == base ? 0u : 1u;
+    pub_half[g] = half;
+    graphs[g].snap_write = base + (half ? 0u : (size_t)n);
+    if (hist[g].base && hist[g].count < hist[g].cap) hist[g].count++;
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_round_begin(hipStream_t s, uint32_t* ctl, const uint32_t* tab, const uint8_t* inround, uint32_t ng,
+                              uint32_t* fl, RoundState* st, const RoundGraph* rg) {
+    k_round_begin<<<1, CTL_THREADS, 0, s>>>(ctl, tab, inround, ng, fl, st, rg);
+    return hipGetLastError();
+}
+hipError_t launch_t1_control(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
+                             RoundState* st, const RoundGraph* rg, bool after, bool before, bool deterministic,
+                             uint32_t serial_repair, uint32_t serial_conflicts) {
+    k_t1_control<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, st, rg, after ? 1 : 0, before ? 1 : 0,
+                                           deterministic ? 1 : 0, serial_repair, serial_conflicts);
+    return hipGetLastError();
+}
+hipError_t launch_t2_prep(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
+                          const RoundState* st) {
+    k_t2_prep<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, st);
+    return hipGetLastError();
+}
+hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl) {
+    k_round_end<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl);
+    return hipGetLastError();
+}
+hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, uint32_t max_n, const uint32_t* fl_round,
+                                uint16_t* const* snap_base, uint32_t* pub_half, HistDev* hist, bool any_history) {
+    if (any_history) {
+        uint32_t chunks = (max_n + 2047) / 2048;
+        dim3 grid(chunks ? chunks : 1, ng);
+        k_round_history<<<grid, 256, 0, s>>>(graphs, fl_round, hist);
+    }
+    k_round_flip<<<(ng + 255) / 256, 256, 0, s>>>(graphs, ng, fl_round, snap_base, pub_half, hist);
+    return hipGetLastError();
+}
+
+}  // namespace sgc

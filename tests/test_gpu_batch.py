@@ -1,0 +1,174 @@
+"""Batched pattern: one resident mixed stream, cut into rounds, each round one
+replay of a captured hipGraph (or host-driven launches).
+
+Deterministic runs give every graph exactly what the oracle gives it alone
+(batch = epoch = E_g over the churn, then the final repair): captured or
+host-driven, with the full Tier-1 loop captured or with one sweep captured
+(the host then finishes those rounds), and with a graph whose rows overflow
+(Tier-2 runs on the host).  Parallel runs publish a proper snapshot at every
+epoch of every graph, with and without the launch-shape autotuner.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import streamgc_amd as S
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["er2k", "chunglu4k", "rmat_s10", "delaunay1k", "c100", "k4", "star6", "weighted_general"]
+EPOCH = [96, 160, 64, 128, 200, 32, 48, 80]          # E_g per graph
+LENGTH = [700, 300, 1000, 450, 129, 64, 90, 333]     # churn records per graph
+OVERFLOW = dict(slack_pct=0, min_row_capacity=1, overflow_capacity=8)   # er2k: rows overflow, edges drop
+
+
+def _mixed(streams, seed):
+    """Interleave per-graph streams at random, keeping each graph's own order."""
+    rng = np.random.default_rng(seed)
+    owners = np.concatenate([np.full(len(s), g, np.int64) for g, s in streams.items()])
+    rng.shuffle(owners)
+    pos = {g: 0 for g in streams}
+    out = np.zeros(len(owners), dtype=S.EDGE_UPDATE)
+    for i, g in enumerate(owners.tolist()):
+        out[i] = streams[g][pos[g]]
+        pos[g] += 1
+    return out
+
+
+def _config(k, overflow):
+    extra = OVERFLOW if overflow and k == 0 else {}
+    return S.StreamGCConfig(epoch_size=EPOCH[k], batch_size=EPOCH[k], **extra)
+
+
+def _oracle(g, cfg, churn, planted=None):
+    oc = O.default_config(epoch_size=cfg.epoch_size, batch_size=cfg.batch_size, slack_pct=cfg.slack_pct,
+                          min_row_capacity=cfg.min_row_capacity, overflow_capacity=cfg.overflow_capacity,
+                          flags=O.FLAG_REPAIR_EVERY_EPOCH | O.FLAG_EXACT_HISTOGRAM)
+    orc = O.Engine(O.Csr(g.n, g.row_ptr, g.col_idx, g.degree), oc)
+    init = S.generate_init_stream(g)
+    orc.submit(init["u"], init["v"], init["type"])
+    orc.finish()
+    if planted is not None:
+        orc.set_words(planted)
+    orc.submit(churn["u"], churn["v"], churn["type"])
+    orc.finish()
+    return orc
+
+
+def _pool(graph_path, deterministic, overflow=False, batch_capacity=8192):
+    graphs = {k: S.load_graph_csr(graph_path(n)) for k, n in enumerate(NAMES)}
+    pool = S.GraphPool(max_graphs=16, batch_capacity=batch_capacity, deterministic=deterministic,
+                       repair_every_epoch=True, exact_histogram=True)
+    gid = {k: pool.add(g, _config(k, overflow)) for k, g in graphs.items()}
+    init = {gid[k]: S.tag_graph(S.generate_init_stream(g), gid[k]) for k, g in graphs.items()}
+    pool.submit(_mixed(init, 1), max_stats=0)
+    pool.finish()
+    churn = {k: S.generate_bench_stream_edges(g, *_churn(k, overflow), 11 + k) for k, g in graphs.items()}
+    mixed = _mixed({gid[k]: S.tag_graph(s, gid[k]) for k, s in churn.items()}, 2)
+    return pool, graphs, gid, churn, mixed
+
+
+def _churn(k, overflow):
+    """(records, insert %) of graph k's churn; the overflowing graph grows, so its rows keep dropping edges."""
+    return (3000, 80) if overflow and k == 0 else (LENGTH[k], 50)
+
+
+def _rounds(k, overflow):
+    return -(-_churn(k, overflow)[0] // EPOCH[k])
+
+
+@pytest.mark.parametrize("use_graph,t1_sweeps,overflow", [(True, 0, False), (True, 1, False), (True, 0, True),
+                                                          (True, 1, True), (False, 0, True)])
+def test_deterministic_batch_equals_the_oracle_per_graph(use_graph, t1_sweeps, overflow, graph_path):
+    pool, graphs, gid, churn, mixed = _pool(graph_path, True, overflow)
+    rounds = pool.batch_prepare(mixed, use_graph=use_graph, t1_sweeps=t1_sweeps)
+    assert rounds == max(_rounds(k, overflow) for k in graphs)
+    st = pool.batch_run()
+    assert st["rounds"] == rounds and st["updates"] == len(mixed)
+    print(f"rounds {rounds}, host-finished {st['host_rounds']}", flush=True)
+    for k, g in graphs.items():
+        orc = _oracle(g, _config(k, overflow), churn[k])
+        assert np.array_equal(pool.words(gid[k]), orc.words()), NAMES[k]
+        assert np.array_equal(pool.snapshot(gid[k]), orc.snapshot()), NAMES[k]
+        assert pool.validate(gid[k], snapshot=True)[:2] == (0, 0), NAMES[k]
+    if overflow:
+        assert pool.totals()["overflow_events"] > 0
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_rounds_the_host_finishes_stay_bit_exact(use_graph, graph_path):
+    # planted same-colour neighbours need two Tier-1 sweeps (fix, then a clean
+    # one); with one sweep captured, the host finishes the first round of
+    # every graph, and the replays go on from the next round
+    pool, graphs, gid, churn, mixed = _pool(graph_path, True)
+    planted = {}
+    for k, g in graphs.items():
+        w = pool.words(gid[k])
+        w[: g.n // 2] = (w[: g.n // 2] & 0xFFFF0000) | 1
+        planted[k] = w
+        pool.set_words(gid[k], w)
+    pool.batch_prepare(mixed, use_graph=use_graph, t1_sweeps=1)
+    st = pool.batch_run()
+    if use_graph:
+        assert st["host_rounds"] >= 1
+    for k, g in graphs.items():
+        orc = _oracle(g, _config(k, False), churn[k], planted[k])
+        assert np.array_equal(pool.words(gid[k]), orc.words()), NAMES[k]
+        assert np.array_equal(pool.snapshot(gid[k]), orc.snapshot()), NAMES[k]
+
+
+@pytest.mark.parametrize("autotune,t1_sweeps", [(0, 0), (2, 0), (0, 1)])
+def test_parallel_batch_publishes_proper_snapshots(autotune, t1_sweeps, graph_path):
+    pool, graphs, gid, churn, mixed = _pool(graph_path, False)
+    for k in graphs:
+        pool.history_reserve(gid[k], 64)
+    rounds = pool.batch_prepare(mixed, use_graph=True, autotune_rounds=autotune, t1_sweeps=t1_sweeps)
+    st = pool.batch_run()
+    print(f"rounds {rounds}, host-finished {st['host_rounds']}, chosen {st['chosen']}, "
+          f"ns/update {st['candidate_ns_per_update']}", flush=True)
+    if autotune:
+        assert st["candidates"] == 4 and 0 <= st["chosen"] < 4
+        assert all(v > 0 for v in st["candidate_ns_per_update"][:min(4, rounds // autotune)])
+    for k, g in graphs.items():
+        n_snap = pool.history_count(gid[k])
+        epochs = -(-LENGTH[k] // EPOCH[k])
+        assert n_snap == epochs, NAMES[k]
+        ends = [min(EPOCH[k] * (j + 1), LENGTH[k]) for j in range(epochs)]
+        bad = pool.validate_history(gid[k], S.edge_intervals(g, churn[k]), ends)
+        assert bad.tolist() == [0] * epochs, NAMES[k]
+        ids = np.arange(g.n, dtype=np.uint32)
+        assert np.array_equal(pool.query(gid[k], ids), pool.snapshot(gid[k]))
+    # colour counts stay close to each graph run alone by the same parallel engine
+    together = sum(pool.validate(gid[k])[2] for k in graphs)
+    alone = 0
+    for k, g in graphs.items():
+        solo = S.GPUPartition(g, S.StreamGCConfig(epoch_size=EPOCH[k], batch_size=EPOCH[k], repair_every_epoch=True,
+                                                  exact_histogram=True))
+        solo.process_stream(S.generate_init_stream(g))
+        solo.finish()
+        solo.process_stream(churn[k])
+        solo.finish()
+        alone += solo.validate()[2]
+    print(f"max colours, summed over graphs: batched {together}, alone {alone}", flush=True)
+    assert together <= 1.1 * alone + 2
+
+
+def test_batch_then_plain_submit_continues(graph_path):
+    # a run leaves every graph at an epoch boundary; later batches go on as usual
+    pool, graphs, gid, churn, mixed = _pool(graph_path, True)
+    pool.batch_prepare(mixed)
+    pool.batch_run()
+    more = {k: S.generate_bench_stream_edges(g, 200, 50, 99 + k) for k, g in graphs.items()}
+    pool.submit(_mixed({gid[k]: S.tag_graph(s, gid[k]) for k, s in more.items()}, 3), max_stats=0)
+    pool.finish()
+    for k, g in graphs.items():
+        orc = _oracle(g, _config(k, False), churn[k])
+        orc.submit(more[k]["u"], more[k]["v"], more[k]["type"])
+        orc.finish()
+        assert np.array_equal(pool.words(gid[k]), orc.words()), NAMES[k]
+
+
+def test_batch_refuses_rounds_over_the_batch_capacity(graph_path):
+    pool, graphs, gid, churn, mixed = _pool(graph_path, False, batch_capacity=256)
+    with pytest.raises(S.SgcError):
+        pool.batch_prepare(mixed)
