@@ -44,6 +44,10 @@ WORKLOADS = {
                kind="rmat", scale=20, edge_factor=48.0),
     "c1": dict(name="Delaunay 2^17 uniform points (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
                kind="delaunay", points=1 << 17),
+    "c4": dict(name="batched: 256 graphs (128 Delaunay with 2^U[10,16] points, 128 R-MAT scale U[10,16] edge "
+                    "factor U[8,40], seeds 42+i) in one pool; per graph a churn of U[1K,64K] updates at U[30,70]% "
+                    "insert, E_g = min(stream, 4096); rounds replayed as one captured hipGraph each",
+               kind="batched"),
 }
 
 
@@ -150,6 +154,9 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="served workload: requests per phase")
     ap.add_argument("--load", type=float, default=0.5, help="served workload: offered load / served capacity")
     ap.add_argument("--max-delay-us", type=int, default=2000, help="served workload: publish deadline")
+    ap.add_argument("--autotune-rounds", type=int, default=1,
+                    help="batched workload: rounds each launch-shape candidate is timed on (0: no tuning)")
+    ap.add_argument("--graphs", type=int, default=256, help="batched workload: graphs in the pool")
     ap.add_argument("--init-batch", type=int, default=256,
                     help="apply the (untimed) init stream in launches of this many updates, epochs still every E")
     args = ap.parse_args()
@@ -167,6 +174,8 @@ def main():
     w = WORKLOADS[args.workload]
     if w["kind"] == "served":
         return run_served(S, args, w)
+    if w["kind"] == "batched":
+        return run_batched(S, args, w)
     E = 32768
     K, W = args.steps, args.warmup
     t = time.perf_counter()
@@ -340,6 +349,155 @@ def profile_summary(args):
                                              for k, v in per.items()}
     print(json.dumps(out, indent=1))
 
+
+
+# ---------------------------------------------------------------- batched (C4)
+def make_batch_graphs(S, count, seed=42):
+    """(kind, graph, churn length, insert %) of the batched workload, graph i seeded 42 + i."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        if i < count // 2:
+            g = S.gen_delaunay(int(round(2 ** rng.uniform(10, 16))), seed=seed + i)
+            kind = "delaunay"
+        else:
+            g = S.gen_rmat(int(rng.integers(10, 17)), float(rng.uniform(8, 40)), seed=seed + i)
+            kind = "rmat"
+        out.append((kind, g, int(rng.integers(1000, 64001)), int(rng.integers(30, 71))))
+    return out
+
+
+def chunked_launches(streams, chunk):
+    """Launch j holds records [j*chunk, (j+1)*chunk) of every stream that has them."""
+    allr = np.concatenate(streams)
+    launch = np.concatenate([np.arange(len(s), dtype=np.int64) // chunk for s in streams])
+    order = np.argsort(launch, kind="stable")
+    bounds = np.concatenate([[0], np.cumsum(np.bincount(launch))])
+    allr = allr[order]
+    return [allr[bounds[j]:bounds[j + 1]] for j in range(len(bounds) - 1)]
+
+
+def run_batched(S, args, w):
+    """The batched pattern on 256 graphs, timed three ways.
+
+    Every graph's churn is one seeded stream of 3 x L_g updates, cut into
+    thirds: the first third is run one graph at a time (each graph's own
+    batches and epochs, host-driven, as a per-graph engine would), the second
+    as host-driven rounds over all graphs, the third -- the measured value --
+    as rounds replayed from captured hipGraphs, with the launch-shape
+    autotuner on its first rounds.  Every snapshot of the third is validated.
+    """
+    t = time.perf_counter()
+    spec = make_batch_graphs(S, args.graphs)
+    nv = sum(g.n for _, g, _, _ in spec)
+    ne = sum(g.num_edges // 2 for _, g, _, _ in spec)
+    log(f"{len(spec)} graphs, {nv} vertices, {ne} edges ({time.perf_counter() - t:.1f}s)")
+    pool = S.GraphPool(max_graphs=len(spec), batch_capacity=1 << 20, repair_every_epoch=True, exact_histogram=True)
+    gids, E = [], []
+    for _, g, L, _ in spec:
+        e = min(L, 4096)
+        E.append(e)
+        gids.append(pool.add(g, S.StreamGCConfig(epoch_size=e, batch_size=e)))
+    used, _, _ = pool.usage()
+    t = time.perf_counter()
+    init = [S.tag_graph(S.generate_init_stream(g), gid) for (_, g, _, _), gid in zip(spec, gids)]
+    for launch in chunked_launches(init, args.init_batch):
+        pool.apply(launch)
+    pool.finish()
+    log(f"init: {sum(len(x) for x in init)} inserts in launches of {args.init_batch} per graph, repaired "
+        f"({used / 2**30:.2f} GiB resident, {time.perf_counter() - t:.1f}s)")
+    del init
+    t = time.perf_counter()
+    whole = [S.generate_bench_stream_edges(g, 3 * L, pct, 42 + i) for i, (_, g, L, pct) in enumerate(spec)]
+    parts = [[S.tag_graph(x[k * L:(k + 1) * L], gid) for x, (_, _, L, _), gid in zip(whole, spec, gids)]
+             for k in range(3)]
+    n_upd = sum(L for _, _, L, _ in spec)
+    log(f"streams: 3 x {n_upd} updates ({time.perf_counter() - t:.1f}s)")
+
+    def counters():
+        tot = pool.totals()
+        inf = [pool.info(gid) for gid in gids]
+        return {"t1_sweeps": tot["t1_sweeps"], "t2_runs": tot["t2_runs"], "overflow_events": tot["overflow_events"],
+                "host_t2_ms": sum(i["tier2_ms"] for i in inf)}
+
+    def delta(a, b):
+        return {k: round(b[k] - a[k], 2) for k in a}
+
+    c0 = counters()
+    # 1. one graph at a time
+    t0 = time.perf_counter()
+    for gid, recs in zip(gids, parts[0]):
+        pool.submit(recs, max_stats=0)
+        pool.finish([gid])
+    alone_s = time.perf_counter() - t0
+    log(f"one graph at a time: {alone_s:.2f}s = {n_upd / alone_s / 1e6:.2f} M updates/s")
+    c1 = counters()
+    # 2. host-driven rounds
+    rounds = pool.batch_prepare(np.concatenate(parts[1]), use_graph=False)
+    pool.kernel_timing(True)
+    st_host = pool.batch_run()
+    host_k = pool.kernel_times()
+    pool.kernel_timing(False)
+    c2 = counters()
+    log(f"host-driven rounds: {rounds} rounds, {st_host['ms']:.1f} ms = "
+        f"{n_upd / st_host['ms'] / 1e3:.2f} M updates/s")
+    # 3. captured rounds (the value)
+    epochs = [-(-L // e) for (_, _, L, _), e in zip(spec, E)]
+    for gid, k in zip(gids, epochs):
+        pool.history_reserve(gid, k)
+    rounds = pool.batch_prepare(np.concatenate(parts[2]), use_graph=True, autotune_rounds=args.autotune_rounds)
+    st = pool.batch_run()
+    value = n_upd / (st["ms"] / 1e3)
+    c3 = counters()
+    log(f"captured rounds: {rounds} rounds ({st['host_rounds']} finished by the host), {st['ms']:.1f} ms = "
+        f"{value / 1e6:.2f} M updates/s; autotuner ns/update {st['candidate_ns_per_update']} -> {st['chosen']}")
+    pool.batch_release()
+
+    # Every graph's live colouring must be proper on its rows (the reference's
+    # check).  A graph whose rows never overflowed holds exactly the stream's
+    # live edges, so each of its published snapshots is also checked against
+    # the edge set the stream implies at that point; rows that overflowed may
+    # have dropped edges (as the reference's do), so theirs are not.
+    parity = {"graphs_proper": 0, "graphs_overflowed": 0, "snapshots_validated": 0, "snapshots_improper": 0}
+    colours = {"delaunay": [], "rmat": []}
+    if not args.no_validate:
+        t = time.perf_counter()
+        for (kind, g, L, _), gid, x, k, e in zip(spec, gids, whole, epochs, E):
+            v = pool.validate(gid)
+            parity["graphs_proper"] += v[:2] == (0, 0)
+            colours[kind].append(v[2])
+            if pool.info(gid)["overflow_events"]:
+                parity["graphs_overflowed"] += 1
+                continue
+            iv = S.edge_intervals(g, x)
+            bad = pool.validate_history(gid, iv, [2 * L + min(e * (j + 1), L) for j in range(k)])
+            parity["snapshots_validated"] += k
+            parity["snapshots_improper"] += int(np.count_nonzero(bad))
+        log(f"{parity['graphs_proper']} graphs proper on their rows, {parity['graphs_overflowed']} overflowed; "
+            f"validated {parity['snapshots_validated']} snapshots of the others against the stream: "
+            f"{parity['snapshots_improper']} improper ({time.perf_counter() - t:.1f}s)")
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "updates/s", "n_gpus": 1, "steps": rounds,
+        "warmup": 0, "ms_per_step": round(st["ms"] / max(rounds, 1), 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": w["name"], "graphs": len(spec), "vertices": nv, "edges": ne,
+                   "resident_gib": round(used / 2**30, 2), "updates_timed": n_upd, "rounds": rounds,
+                   "init": f"init streams applied in launches of {args.init_batch} updates per graph, then repaired"},
+        "legs_updates_per_s": {"one_graph_at_a_time": round(n_upd / alone_s, 1),
+                               "host_driven_rounds": round(n_upd / st_host["ms"] * 1e3, 1),
+                               "captured_rounds": round(value, 1)},
+        "batched_vs_one_at_a_time": round(value / (n_upd / alone_s), 2),
+        "host_finished_rounds": st["host_rounds"],
+        "legs_counters": {"one_graph_at_a_time": delta(c0, c1), "host_driven_rounds": delta(c1, c2),
+                          "captured_rounds": delta(c2, c3)},
+        "host_rounds_kernels_ms": {k: round(v[0], 2) for k, v in host_k.items()},
+        "autotune": {"candidates": st["candidate"], "ns_per_update": st["candidate_ns_per_update"],
+                     "chosen": st["chosen"], "rounds_each": args.autotune_rounds},
+        "parity": {**parity, "max_color_by_shape": {k: (max(v) if v else None) for k, v in colours.items()},
+                   "max_color_sum": int(sum(sum(v) for v in colours.values()))},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
 
 
 # ---------------------------------------------------------------- served (C5)

@@ -262,6 +262,25 @@ int sgc_pool_submit(sgc_pool* p, const sgc_edge_update* updates, uint64_t count,
 int sgc_pool_query(sgc_pool* p, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors);
 int sgc_pool_download_words(sgc_pool* p, uint32_t gid, uint32_t* words);
 int sgc_pool_upload_words(sgc_pool* p, uint32_t gid, const uint32_t* words);  /* every sweep then visits every vertex */
+/* One resident graph (no reference counterpart). */
+typedef struct sgc_graph_info {
+    uint32_t n;
+    uint32_t num_hd;             /* vertices in histogram mode */
+    uint64_t slots;              /* slack CSR capacity */
+    uint64_t bytes;              /* device bytes charged to the pool's budget */
+    uint32_t epoch_size;
+    uint32_t epochs;             /* epoch steps taken */
+    uint32_t overflow_events;    /* row-full events ever: edges whose two rows may disagree, or that were dropped */
+    int32_t published;           /* published snapshot buffer (-1: none yet) */
+    double tier2_ms;             /* host Tier-2 time, all runs */
+    uint32_t tier2_runs;
+    uint32_t reserved;
+} sgc_graph_info;
+int sgc_pool_graph_info(sgc_pool* p, uint32_t gid, sgc_graph_info* out);
+/* Per-graph state downloads (as sgc_download_csr / _bitmaps / _hist). */
+int sgc_pool_download_csr(sgc_pool* p, uint32_t gid, uint32_t* row_ptr, uint32_t* col_idx);  /* n+1, slots */
+int sgc_pool_download_bitmaps(sgc_pool* p, uint32_t gid, uint64_t* bitmaps);
+int sgc_pool_download_hist(sgc_pool* p, uint32_t gid, uint32_t* hist);                     /* num_hd * 1024 */
 int sgc_pool_download_snapshot(sgc_pool* p, uint32_t gid, uint16_t* colors);
 int sgc_pool_validate(sgc_pool* p, uint32_t gid, int snapshot, uint64_t* violations, uint64_t* uncolored_with_edges,
                       uint32_t* max_color);

@@ -601,6 +601,30 @@ class GraphPool:
     def colors(self, gid):
         return (self.words(gid) & 0xFFFF).astype(np.uint16)
 
+    def info(self, gid):
+        """Size, epochs, overflow events ever, published buffer and host Tier-2 time of one graph."""
+        out = _lib.GraphInfo()
+        self._ck(lib().sgc_pool_graph_info(self.h, gid, C.byref(out)))
+        return out.as_dict()
+
+    def csr(self, gid):
+        """(row_ptr, slots) of one graph's slack CSR (INVALID = free slot)."""
+        inf = self.info(gid)
+        rp = np.zeros(inf["n"] + 1, np.uint32)
+        col = np.zeros(max(inf["slots"], 1), np.uint32)
+        self._ck(lib().sgc_pool_download_csr(self.h, gid, rp, col))
+        return rp, col[:inf["slots"]]
+
+    def bitmaps(self, gid):
+        b = np.zeros(self.info(gid)["n"], np.uint64)
+        self._ck(lib().sgc_pool_download_bitmaps(self.h, gid, b))
+        return b
+
+    def hist(self, gid):
+        h = np.zeros((self.info(gid)["num_hd"], 1024), np.uint32)
+        self._ck(lib().sgc_pool_download_hist(self.h, gid, h.ctypes.data if h.size else None))
+        return h
+
     def set_words(self, gid, words):
         self._ck(lib().sgc_pool_upload_words(self.h, gid, np.ascontiguousarray(words, np.uint32)))
 

@@ -104,6 +104,16 @@ class BatchStats(C.Structure):
                 "candidate": [{f: getattr(self.candidate[i], f) for f, _ in Tuning._fields_} for i in range(k)]}
 
 
+class GraphInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("num_hd", C.c_uint32), ("slots", C.c_uint64), ("bytes", C.c_uint64),
+                ("epoch_size", C.c_uint32), ("epochs", C.c_uint32), ("overflow_events", C.c_uint32),
+                ("published", C.c_int32), ("tier2_ms", C.c_double), ("tier2_runs", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class PoolConfig(C.Structure):
     _fields_ = [("max_graphs", C.c_uint32), ("batch_capacity", C.c_uint32), ("flags", C.c_uint32),
                 ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64)]
@@ -173,6 +183,10 @@ SIGNATURES = {
     "sgc_pool_query": [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p],
     "sgc_pool_download_words": [_vp, C.c_uint32, _u32p],
     "sgc_pool_upload_words": [_vp, C.c_uint32, _u32p],
+    "sgc_pool_graph_info": [_vp, C.c_uint32, C.POINTER(GraphInfo)],
+    "sgc_pool_download_csr": [_vp, C.c_uint32, _u32p, _u32p],
+    "sgc_pool_download_bitmaps": [_vp, C.c_uint32, _u64p],
+    "sgc_pool_download_hist": [_vp, C.c_uint32, _vp],
     "sgc_pool_download_snapshot": [_vp, C.c_uint32, _u16p],
     "sgc_pool_validate": [_vp, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                           C.POINTER(C.c_uint32)],

@@ -114,7 +114,10 @@ struct GraphState {
     uint32_t prev = 0, plateau = 0;
     bool iterating = false;
     bool t1_clean = false;     // the last Tier-1 sweep found nothing to repair
+    bool settled = false;      // a repair has completed since the graph was loaded or its state uploaded
     sgc_epoch_stats st{};
+    double t2_ms = 0;          // host Tier-2 time and runs
+    uint32_t t2_count = 0;
 
     void release() {
         for (void* p : {(void*)word, (void*)deg0, (void*)row_ptr, (void*)col, (void*)hd_map, (void*)hubs, (void*)hist,
@@ -171,7 +174,7 @@ struct BatchRun {
     sgc_batch_config cfg{};
     uint32_t rounds = 0, ng = 0, max_count = 0, max_n = 0, sweeps = 0, ovf_cap = 0;
     uint64_t updates = 0;
-    bool multi = false, any_history = false;
+    bool multi = false, any_history = false, all_settled = true;
     std::vector<uint32_t> tab;                 // per round: first record, record count
     std::vector<uint8_t> inround, full;        // rounds x ng: takes part; reaches its epoch boundary
     std::vector<uint32_t> counts;              // rounds x ng: records
@@ -215,6 +218,8 @@ struct sgc_pool {
     uint64_t* h_totals = nullptr;
     EdgeGroups eg;
     sgc_edge_update* d_batch = nullptr;   // staging of host batches
+    uint32_t* d_t2_items = nullptr;       // (graph, vertex, old colour) recoloured by the host Tier-2
+    size_t t2_items_cap = 0;
     uint32_t* d_list = nullptr;           // sweep worklist (virtual vertex ids)
     uint32_t* d_list_count = nullptr;
     uint32_t list_cap = 0;
@@ -240,6 +245,12 @@ struct sgc_pool {
     bool repair_every_epoch() const { return (flags & SGC_REPAIR_EVERY_EPOCH) != 0; }
     bool tracking() const { return (flags & SGC_FULL_SWEEPS) == 0; }
     bool rebuild_each_sweep() const { return deterministic() || tune.fresh_repair < 2; }
+    // Tier-1 selection of a graph: exact neighbour colours (tune.fresh_repair)
+    // once it has been repaired; until then -- its first repair colours and
+    // settles every vertex of its rows at once -- the reference's selection
+    // from rebuilt bitmaps and histograms, whose result does not depend on
+    // the order in which the wavefronts run.
+    int fresh_level(uint32_t g) const { return deterministic() || !gs[g].settled ? 0 : (int)tune.fresh_repair; }
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -390,6 +401,7 @@ struct sgc_pool {
     // All sweeps must visit every vertex of graph gid (its state changed from the host).
     hipError_t mark_all(uint32_t gid) {
         GraphState& G = gs[gid];
+        G.settled = false;
         return hipMemsetAsync(G.dirty, 1, (size_t)M_STICKY * mark_stride(G.n), stream);
     }
     // Descriptors follow the tracking flag (dirty == null: no marking).
@@ -688,38 +700,45 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids, uint32_t start_it) {
         bool any = false;
         for (uint32_t g : gids) any |= gs[g].iterating;
         if (!any) break;
-        hipError_t e = set_active_iterating(gids);
-        LaunchCtx L = ctx(true);
+        hipError_t e = hipSuccess;
         // The reference rebuilds before every sweep (A/src/streamgc.cu:1317-1393).
         // A parallel-mode sweep that selects from exact neighbour colours for
-        // every row (fresh_repair 2) reads no bitmap or histogram, so there the
+        // every row (fresh level 2) reads no bitmap or histogram, so there the
         // rebuild runs once, after the repair.
-        if (e == hipSuccess && rebuild_each_sweep()) e = rebuild(L);
+        std::vector<uint32_t> reb;
+        for (uint32_t g : gids)
+            if (gs[g].iterating && fresh_level(g) < 2) reb.push_back(g);
+        if (!reb.empty()) {
+            e = set_active(reb);
+            if (e == hipSuccess) e = rebuild(ctx(true));
+        }
+        if (e == hipSuccess) e = set_active_iterating(gids);
+        LaunchCtx L = ctx(true);
         if (e == hipSuccess) e = launch_zero_ctr(L, C_CONFLICTS);
         // A graph sweeps in vertex order on one wavefront (Gauss-Seidel) when
         // deterministic, or when it is small and its last sweep left few
         // conflicts (a racy sweep over a small conflict set can chase the same
         // conflicts for many iterations); otherwise one wavefront per listed row.
-        std::vector<uint32_t> small, large;
+        // Graphs of different selection levels take separate launches.
+        std::vector<uint32_t> part[2][3];   // [serial][fresh level]
         for (uint32_t g : gids) {
             const GraphState& G = gs[g];
             if (!G.iterating) continue;
             const bool serial = deterministic() || (G.n <= tune.serial_repair && G.prev <= tune.serial_conflicts);
-            (serial ? small : large).push_back(g);
+            part[serial][fresh_level(g)].push_back(g);
         }
-        if (e == hipSuccess && !large.empty()) {
-            if (!small.empty()) e = set_active(large);
-            LaunchCtx L1 = ctx(true);
-            if (e == hipSuccess) e = sweep(L1, M_CHECK);
-            if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, false, (int)tune.fresh_repair); });
-        }
-        if (e == hipSuccess && !small.empty()) {
-            if (!large.empty()) e = set_active(small);
-            LaunchCtx L1 = ctx(true);
-            if (e == hipSuccess)
-                e = timed("tier1_serial",
-                          [&] { return launch_tier1(L1, true, deterministic() ? 0 : (int)tune.fresh_repair); });
-        }
+        for (int serial = 0; serial < 2 && e == hipSuccess; serial++)
+            for (int lvl = 0; lvl < 3 && e == hipSuccess; lvl++) {
+                if (part[serial][lvl].empty()) continue;
+                e = set_active(part[serial][lvl]);
+                LaunchCtx L1 = ctx(true);
+                if (!serial) {
+                    if (e == hipSuccess) e = sweep(L1, M_CHECK);
+                    if (e == hipSuccess) e = timed("tier1", [&] { return launch_tier1(L1, false, lvl); });
+                } else if (e == hipSuccess) {
+                    e = timed("tier1_serial", [&] { return launch_tier1(L1, true, lvl); });
+                }
+            }
         if (e == hipSuccess) e = read_ctr();
         if (e != hipSuccess) return hip_fail(e, "tier-1");
         t1_sweeps++;
@@ -766,28 +785,84 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
     if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(Lc); });
     if (e == hipSuccess) e = read_ctr();
     if (e != hipSuccess) return hip_fail(e, "tier-2 check");
-    std::vector<uint32_t> ran;
+    // Every graph with bad slots at once: one download, the repairs in
+    // parallel on the host cores, one upload; then only the recoloured
+    // vertices and their neighbourhoods are marked for the next sweeps (as a
+    // recolour on the device marks them).
+    struct Job {
+        uint32_t g;
+        std::vector<uint32_t> w, w0, rp, c;
+        uint32_t fixed = 0;
+        double ms = 0;
+    };
+    std::vector<Job> jobs;
     for (uint32_t g : check) {
         GraphState& G = gs[g];
         G.st.t2_remaining = ctr(g, C_T2BAD);
-        if (G.st.t2_remaining == 0) continue;
-        auto t0 = std::chrono::steady_clock::now();
-        std::vector<uint32_t> w(G.n), rp(G.n + 1), c(G.S);
-        e = hipMemcpyAsync(w.data(), G.word, (size_t)G.n * 4, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(rp.data(), G.row_ptr, ((size_t)G.n + 1) * 4, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c.data(), G.col, G.S * 4, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) return hip_fail(e, "tier-2 download");
-        G.st.t2_fixed = tier2_host(G.n, rp.data(), c.data(), w.data(), G.cfg.tier2_passes);
+        if (G.st.t2_remaining) jobs.push_back(Job{g, {}, {}, {}, {}, 0, 0});
+    }
+    if (jobs.empty()) return SGC_OK;
+    auto t0 = std::chrono::steady_clock::now();
+    for (Job& J : jobs) {
+        const GraphState& G = gs[J.g];
+        J.w.resize(G.n);
+        J.rp.resize((size_t)G.n + 1);
+        J.c.resize(G.S);
+        if (e == hipSuccess) e = hipMemcpyAsync(J.w.data(), G.word, (size_t)G.n * 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(J.rp.data(), G.row_ptr, ((size_t)G.n + 1) * 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(J.c.data(), G.col, G.S * 4, hipMemcpyDeviceToHost, stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "tier-2 download");
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t k = 0; k < jobs.size(); k++) {
+        Job& J = jobs[k];
+        auto j0 = std::chrono::steady_clock::now();
+        J.w0 = J.w;
+        J.fixed = tier2_host(gs[J.g].n, J.rp.data(), J.c.data(), J.w.data(), gs[J.g].cfg.tier2_passes);
+        J.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - j0).count();
+    }
+    std::vector<uint32_t> items;   // (graph, vertex, old colour) of every recoloured vertex
+    for (Job& J : jobs)
+        for (uint32_t v = 0; v < gs[J.g].n; v++)
+            if (J.w[v] != J.w0[v]) {
+                items.push_back(J.g);
+                items.push_back(v);
+                items.push_back(J.w0[v] & 0xFFFFu);
+            }
+    const uint32_t nitems = (uint32_t)(items.size() / 3);
+    if (items.size() > t2_items_cap) {
+        e = hipStreamSynchronize(stream);
+        if (d_t2_items) (void)hipFree(d_t2_items);
+        d_t2_items = nullptr;
+        t2_items_cap = 0;
+        if (e == hipSuccess) e = hipMalloc((void**)&d_t2_items, items.size() * 4);
+        if (e == hipSuccess) t2_items_cap = items.size();
+    }
+    for (Job& J : jobs)
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(gs[J.g].word, J.w.data(), (size_t)gs[J.g].n * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && nitems)
+        e = hipMemcpyAsync(d_t2_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && nitems) e = launch_mark_recolored(stream, graphs.d, d_t2_items, nitems);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "tier-2 upload");
+    const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    double cpu = 0;
+    for (const Job& J : jobs) cpu += J.ms;
+    std::vector<uint32_t> ran;
+    for (const Job& J : jobs) {
+        GraphState& G = gs[J.g];
+        G.st.t2_fixed = J.fixed;
         G.st.t2_ran = 1;
         t2_runs++;
-        e = hipMemcpyAsync(G.word, w.data(), (size_t)G.n * 4, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = mark_all(g);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) return hip_fail(e, "tier-2 upload");
-        last_ms[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        ran.push_back(g);
+        G.t2_count++;
+        // the transfers are shared; each graph is charged its own repair time and a share of the rest
+        G.t2_ms += J.ms + std::max(0.0, total - cpu) / (double)jobs.size();
+        ran.push_back(J.g);
     }
+    last_ms[2] += total;
     if (ran.empty()) return SGC_OK;
     e = set_active(ran);
     L = ctx(true);
@@ -893,6 +968,7 @@ int sgc_pool::epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<
             if (e != hipSuccess) return hip_fail(e, "rebuild");
         }
         if ((r = publish(gids))) return r;
+        for (uint32_t g : gids) gs[g].settled = true;
     } else {
         e = rebuild(L);
         if (e != hipSuccess) return hip_fail(e, "rebuild");
@@ -919,7 +995,8 @@ void sgc_pool::release() {
     perm.release();
     seg.release();
     for (void* p : {(void*)d_ctr, (void*)d_totals, (void*)d_batch, (void*)d_tmp16, (void*)d_qids, (void*)d_qout,
-                    (void*)eg.keys, (void*)eg.head, (void*)eg.next, (void*)d_list, (void*)d_list_count})
+                    (void*)eg.keys, (void*)eg.head, (void*)eg.next, (void*)d_list, (void*)d_list_count,
+                    (void*)d_t2_items})
         if (p) (void)hipFree(p);
     for (void* p : {(void*)h_ctr, (void*)h_totals})
         if (p) (void)hipHostFree(p);
@@ -1031,6 +1108,7 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
         B->max_n = std::max(B->max_n, gs[g].n);
         B->ovf_cap = std::max(B->ovf_cap, gs[g].cfg.overflow_capacity);
         B->any_history |= gs[g].history != nullptr;
+        B->all_settled &= gs[g].settled;
     }
     B->sweeps = c.t1_sweeps ? std::min(c.t1_sweeps, max_iters) : max_iters;
     hipError_t e = hipMalloc((void**)&B->d_stream, std::max<size_t>(ordered.size(), 1) * sizeof(sgc_edge_update));
@@ -1099,6 +1177,9 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
     base.list_cap = list_cap;
     auto F = [&](uint32_t k) { return (const uint32_t*)(B.d_fl + (size_t)k * ng); };
     const uint32_t* t1_any = B.d_ctl + RC_T1_ANY;
+    // graphs not yet repaired once take the reference's selection (fresh_level)
+    const int fresh = B.all_settled ? (int)t.fresh_repair : 0;
+    const bool rebuild_sweep = det || fresh < 2;
     hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "batch capture");
     e = launch_round_begin(stream, B.d_ctl, B.d_tab, B.d_inround, ng, B.d_fl, B.d_state, B.d_rg);
@@ -1120,7 +1201,7 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
     for (uint32_t it = 0; it < B.sweeps && e == hipSuccess; it++) {
         e = launch_t1_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.d_rg, it > 0, true, det,
                               t.serial_repair, t.serial_conflicts);
-        if (e == hipSuccess && rebuild_each_sweep()) {
+        if (e == hipSuccess && rebuild_sweep) {
             LaunchCtx Lb = base;
             Lb.active = F(F_ITER);
             Lb.gate = t1_any;
@@ -1132,12 +1213,12 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
             Lp.active = F(F_PAR);
             Lp.gate = t1_any;
             e = launch_build_list(Lp, M_CHECK, d_list, list_cap, d_list_count);
-            if (e == hipSuccess) e = launch_tier1(Lp, false, (int)t.fresh_repair);
+            if (e == hipSuccess) e = launch_tier1(Lp, false, fresh);
         }
         if (e == hipSuccess) {
             LaunchCtx Ls = base;
             Ls.active = F(F_SER);
-            e = launch_tier1(Ls, true, det ? 0 : (int)t.fresh_repair);
+            e = launch_tier1(Ls, true, det ? 0 : fresh);
         }
     }
     if (e == hipSuccess)
@@ -1361,6 +1442,8 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
             for (uint32_t g : B.gids[r2]) gs[g].epochs += B.full[(size_t)r2 * ng + g];
         for (uint32_t g = 0; g < ng; g++)
             if (gs[g].used) gs[g].since_epoch = 0;
+        for (uint32_t r2 = 0; r2 < B.rounds; r2++)
+            for (uint32_t g : B.gids[r2]) gs[g].settled = true;
         S.candidates = C > 1 ? C : 0;
         S.chosen = best;
         for (uint32_t k = 0; k < C && k < SGC_BATCH_CANDIDATES; k++) {
@@ -2129,6 +2212,41 @@ int sgc_pool_upload_words(sgc_pool* p, uint32_t gid, const uint32_t* words) {
     hipError_t e = p->mark_all(gid);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     if (e != hipSuccess) return p->hip_fail(e, "mark");
+    return SGC_OK;
+}
+int sgc_pool_download_csr(sgc_pool* p, uint32_t gid, uint32_t* rp, uint32_t* col) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || !rp || !col) return SGC_ERR_INVALID;
+    int r = download(p, rp, G->row_ptr, ((size_t)G->n + 1) * 4);
+    return r ? r : download(p, col, G->col, G->S * 4);
+}
+int sgc_pool_download_bitmaps(sgc_pool* p, uint32_t gid, uint64_t* bitmaps) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || !bitmaps) return SGC_ERR_INVALID;
+    return download(p, bitmaps, G->bitmap, (size_t)G->n * 8);
+}
+int sgc_pool_download_hist(sgc_pool* p, uint32_t gid, uint32_t* hist) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || (!hist && G->nhd)) return SGC_ERR_INVALID;
+    return G->nhd ? download(p, hist, G->hist, (size_t)G->nhd * HIST_BINS * 4) : SGC_OK;
+}
+int sgc_pool_graph_info(sgc_pool* p, uint32_t gid, sgc_graph_info* out) {
+    GraphState* G = graph_of(p, gid);
+    if (!G || !out) return SGC_ERR_INVALID;
+    uint32_t asym = 0;
+    int r = download(p, &asym, p->d_ctr + (size_t)gid * C_COUNT + C_ASYM, 4);
+    if (r) return r;
+    memset(out, 0, sizeof *out);
+    out->n = G->n;
+    out->num_hd = G->nhd;
+    out->slots = G->S;
+    out->bytes = G->bytes;
+    out->epoch_size = G->cfg.epoch_size;
+    out->epochs = G->epochs;
+    out->overflow_events = asym;
+    out->published = G->published;
+    out->tier2_ms = G->t2_ms;
+    out->tier2_runs = G->t2_count;
     return SGC_OK;
 }
 int sgc_pool_download_snapshot(sgc_pool* p, uint32_t gid, uint16_t* colors) {

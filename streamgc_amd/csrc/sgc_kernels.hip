@@ -989,6 +989,22 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
     if (lane() == 0 && totals && rows) add64(&totals[T_SWEEP_ROWS + 2], rows);
 }
 
+// Vertices the host Tier-2 recoloured: their own status and their
+// neighbourhoods are marked as a recolour on the device marks them.  One
+// wavefront per (graph, vertex, old colour) item.
+__global__ __launch_bounds__(256) void k_mark_recolored(const GraphDev* __restrict__ graphs,
+                                                        const uint32_t* __restrict__ items, uint32_t count) {
+    for (uint32_t k = wave_id(); k < count; k += wave_count()) {
+        const GraphDev& g = graphs[items[3 * (size_t)k]];
+        const uint32_t v = items[3 * (size_t)k + 1];
+        if (v >= g.n) continue;
+        const uint16_t oldc = (uint16_t)items[3 * (size_t)k + 2];
+        const uint16_t newc = color_of(gp(g.word)[v]);
+        if (lane() == 0) mark(g, v, D_STATUS);
+        mark_neighbors(g, v, oldc, newc);
+    }
+}
+
 // ---------------------------------------------------------------- checks
 // Tier-2 has work iff some row slot joins an uncoloured endpoint or two
 // endpoints of one colour (A/src/streamgc.cu:1190-1218 then change nothing).
@@ -1247,6 +1263,11 @@ hipError_t launch_tier1(const LaunchCtx& L, bool serial, int fresh) {
 hipError_t launch_t2_check(const LaunchCtx& L) {
     k_t2_check<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L),
                                                     L.totals);
+    return hipGetLastError();
+}
+hipError_t launch_mark_recolored(hipStream_t s, const GraphDev* graphs, const uint32_t* items, uint32_t count) {
+    if (count == 0) return hipSuccess;
+    k_mark_recolored<<<grid_for_waves(count, 4, 4096), 256, 0, s>>>(graphs, items, count);
     return hipGetLastError();
 }
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors) {

@@ -60,6 +60,8 @@ hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist);
 // 2 all rows; see tier1_vertex).
 hipError_t launch_tier1(const LaunchCtx& L, bool serial, int fresh);
 hipError_t launch_t2_check(const LaunchCtx& L);
+// Marks for the vertices a host Tier-2 recoloured: (graph, vertex, old colour) triples.
+hipError_t launch_mark_recolored(hipStream_t s, const GraphDev* graphs, const uint32_t* items, uint32_t count);
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors);
 hipError_t launch_find_color(const LaunchCtx& L, uint32_t gid, uint32_t v, uint16_t* out);
 hipError_t launch_validate_history(hipStream_t s, const uint16_t* snaps, uint32_t n, const uint32_t* eu,

@@ -150,7 +150,7 @@ def test_parallel_batch_publishes_proper_snapshots(autotune, t1_sweeps, graph_pa
         solo.finish()
         alone += solo.validate()[2]
     print(f"max colours, summed over graphs: batched {together}, alone {alone}", flush=True)
-    assert together <= 1.1 * alone + 2
+    assert together <= 1.05 * alone + 1
 
 
 def test_batch_then_plain_submit_continues(graph_path):
