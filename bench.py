@@ -380,75 +380,88 @@ def chunked_launches(streams, chunk):
 def run_batched(S, args, w):
     """The batched pattern on 256 graphs, timed three ways.
 
-    Every graph's churn is one seeded stream of 3 x L_g updates, cut into
-    thirds: the first third is run one graph at a time (each graph's own
-    batches and epochs, host-driven, as a per-graph engine would), the second
-    as host-driven rounds over all graphs, the third -- the measured value --
-    as rounds replayed from captured hipGraphs, with the launch-shape
-    autotuner on its first rounds.  Every snapshot of the third is validated.
+    Each leg starts from its own pool with every graph loaded and its init
+    stream applied and repaired, then runs the same churn (graph i: L_i
+    updates at pct_i % insert, seed 42 + i): one graph at a time (each
+    graph's own batches and epochs, host-driven, as a per-graph engine
+    would), host-driven rounds over all graphs, and -- the measured value --
+    rounds replayed from captured hipGraphs with the launch-shape autotuner
+    on its first rounds.  Every snapshot of the captured leg is validated.
     """
     t = time.perf_counter()
     spec = make_batch_graphs(S, args.graphs)
     nv = sum(g.n for _, g, _, _ in spec)
     ne = sum(g.num_edges // 2 for _, g, _, _ in spec)
-    log(f"{len(spec)} graphs, {nv} vertices, {ne} edges ({time.perf_counter() - t:.1f}s)")
-    pool = S.GraphPool(max_graphs=len(spec), batch_capacity=1 << 20, repair_every_epoch=True, exact_histogram=True)
-    gids, E = [], []
-    for _, g, L, _ in spec:
-        e = min(L, 4096)
-        E.append(e)
-        gids.append(pool.add(g, S.StreamGCConfig(epoch_size=e, batch_size=e)))
-    used, _, _ = pool.usage()
-    t = time.perf_counter()
-    init = [S.tag_graph(S.generate_init_stream(g), gid) for (_, g, _, _), gid in zip(spec, gids)]
-    for launch in chunked_launches(init, args.init_batch):
-        pool.apply(launch)
-    pool.finish()
-    log(f"init: {sum(len(x) for x in init)} inserts in launches of {args.init_batch} per graph, repaired "
-        f"({used / 2**30:.2f} GiB resident, {time.perf_counter() - t:.1f}s)")
-    del init
-    t = time.perf_counter()
-    whole = [S.generate_bench_stream_edges(g, 3 * L, pct, 42 + i) for i, (_, g, L, pct) in enumerate(spec)]
-    parts = [[S.tag_graph(x[k * L:(k + 1) * L], gid) for x, (_, _, L, _), gid in zip(whole, spec, gids)]
-             for k in range(3)]
-    n_upd = sum(L for _, _, L, _ in spec)
-    log(f"streams: 3 x {n_upd} updates ({time.perf_counter() - t:.1f}s)")
+    E = [min(L, 4096) for _, _, L, _ in spec]
+    churn = [S.generate_bench_stream_edges(g, L, pct, 42 + i) for i, (_, g, L, pct) in enumerate(spec)]
+    n_upd = sum(len(x) for x in churn)
+    log(f"{len(spec)} graphs, {nv} vertices, {ne} edges; churn {n_upd} updates ({time.perf_counter() - t:.1f}s)")
 
-    def counters():
+    def fresh_pool():
+        t = time.perf_counter()
+        pool = S.GraphPool(max_graphs=len(spec), batch_capacity=1 << 20, repair_every_epoch=True,
+                           exact_histogram=True)
+        gids = [pool.add(g, S.StreamGCConfig(epoch_size=e, batch_size=e)) for (_, g, _, _), e in zip(spec, E)]
+        init = [S.tag_graph(S.generate_init_stream(g), gid) for (_, g, _, _), gid in zip(spec, gids)]
+        for launch in chunked_launches(init, args.init_batch):
+            pool.apply(launch)
+        pool.finish()
+        log(f"pool: {sum(len(x) for x in init)} init inserts in launches of {args.init_batch} per graph, "
+            f"repaired ({pool.usage()[0] / 2**30:.2f} GiB, {time.perf_counter() - t:.1f}s)")
+        return pool, gids
+
+    def counters(pool, gids):
         tot = pool.totals()
         inf = [pool.info(gid) for gid in gids]
         return {"t1_sweeps": tot["t1_sweeps"], "t2_runs": tot["t2_runs"], "overflow_events": tot["overflow_events"],
                 "host_t2_ms": sum(i["tier2_ms"] for i in inf)}
 
-    def delta(a, b):
-        return {k: round(b[k] - a[k], 2) for k in a}
+    def finish_leg(pool, gids, c0):
+        c1 = counters(pool, gids)
+        d = {k: round(c1[k] - c0[k], 2) for k in c0}
+        cols = [pool.validate(gid) for gid in gids]
+        d["graphs_proper"] = sum(v[:2] == (0, 0) for v in cols)
+        d["max_color_sum"] = int(sum(v[2] for v in cols))
+        return d
 
-    c0 = counters()
+    legs, stats = {}, {}
     # 1. one graph at a time
+    pool, gids = fresh_pool()
+    c0 = counters(pool, gids)
     t0 = time.perf_counter()
-    for gid, recs in zip(gids, parts[0]):
-        pool.submit(recs, max_stats=0)
+    for gid, x in zip(gids, churn):
+        pool.submit(S.tag_graph(x, gid), max_stats=0)
         pool.finish([gid])
     alone_s = time.perf_counter() - t0
-    log(f"one graph at a time: {alone_s:.2f}s = {n_upd / alone_s / 1e6:.2f} M updates/s")
-    c1 = counters()
+    legs["one_graph_at_a_time"] = n_upd / alone_s
+    stats["one_graph_at_a_time"] = finish_leg(pool, gids, c0)
+    log(f"one graph at a time: {alone_s:.2f}s = {legs['one_graph_at_a_time'] / 1e6:.2f} M updates/s")
+    pool.close()
     # 2. host-driven rounds
-    rounds = pool.batch_prepare(np.concatenate(parts[1]), use_graph=False)
+    pool, gids = fresh_pool()
+    c0 = counters(pool, gids)
+    mixed = np.concatenate([S.tag_graph(x, gid) for x, gid in zip(churn, gids)])
+    rounds = pool.batch_prepare(mixed, use_graph=False)
     pool.kernel_timing(True)
     st_host = pool.batch_run()
     host_k = pool.kernel_times()
     pool.kernel_timing(False)
-    c2 = counters()
+    legs["host_driven_rounds"] = n_upd / st_host["ms"] * 1e3
+    stats["host_driven_rounds"] = finish_leg(pool, gids, c0)
     log(f"host-driven rounds: {rounds} rounds, {st_host['ms']:.1f} ms = "
-        f"{n_upd / st_host['ms'] / 1e3:.2f} M updates/s")
+        f"{legs['host_driven_rounds'] / 1e6:.2f} M updates/s")
+    pool.close()
     # 3. captured rounds (the value)
+    pool, gids = fresh_pool()
+    c0 = counters(pool, gids)
     epochs = [-(-L // e) for (_, _, L, _), e in zip(spec, E)]
     for gid, k in zip(gids, epochs):
         pool.history_reserve(gid, k)
-    rounds = pool.batch_prepare(np.concatenate(parts[2]), use_graph=True, autotune_rounds=args.autotune_rounds)
+    rounds = pool.batch_prepare(mixed, use_graph=True, autotune_rounds=args.autotune_rounds)
     st = pool.batch_run()
     value = n_upd / (st["ms"] / 1e3)
-    c3 = counters()
+    legs["captured_rounds"] = value
+    stats["captured_rounds"] = finish_leg(pool, gids, c0)
     log(f"captured rounds: {rounds} rounds ({st['host_rounds']} finished by the host), {st['ms']:.1f} ms = "
         f"{value / 1e6:.2f} M updates/s; autotuner ns/update {st['candidate_ns_per_update']} -> {st['chosen']}")
     pool.batch_release()
@@ -458,43 +471,38 @@ def run_batched(S, args, w):
     # live edges, so each of its published snapshots is also checked against
     # the edge set the stream implies at that point; rows that overflowed may
     # have dropped edges (as the reference's do), so theirs are not.
-    parity = {"graphs_proper": 0, "graphs_overflowed": 0, "snapshots_validated": 0, "snapshots_improper": 0}
+    parity = {"graphs_overflowed": 0, "snapshots_validated": 0, "snapshots_improper": 0}
     colours = {"delaunay": [], "rmat": []}
     if not args.no_validate:
         t = time.perf_counter()
-        for (kind, g, L, _), gid, x, k, e in zip(spec, gids, whole, epochs, E):
-            v = pool.validate(gid)
-            parity["graphs_proper"] += v[:2] == (0, 0)
-            colours[kind].append(v[2])
+        for (kind, g, L, _), gid, x, k, e in zip(spec, gids, churn, epochs, E):
+            colours[kind].append(pool.validate(gid)[2])
             if pool.info(gid)["overflow_events"]:
                 parity["graphs_overflowed"] += 1
                 continue
-            iv = S.edge_intervals(g, x)
-            bad = pool.validate_history(gid, iv, [2 * L + min(e * (j + 1), L) for j in range(k)])
+            bad = pool.validate_history(gid, S.edge_intervals(g, x), [min(e * (j + 1), L) for j in range(k)])
             parity["snapshots_validated"] += k
             parity["snapshots_improper"] += int(np.count_nonzero(bad))
-        log(f"{parity['graphs_proper']} graphs proper on their rows, {parity['graphs_overflowed']} overflowed; "
-            f"validated {parity['snapshots_validated']} snapshots of the others against the stream: "
-            f"{parity['snapshots_improper']} improper ({time.perf_counter() - t:.1f}s)")
+        log(f"{parity['graphs_overflowed']} graphs overflowed; validated {parity['snapshots_validated']} snapshots "
+            f"of the others against the stream: {parity['snapshots_improper']} improper ({time.perf_counter() - t:.1f}s)")
+    pool.close()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "updates/s", "n_gpus": 1, "steps": rounds,
         "warmup": 0, "ms_per_step": round(st["ms"] / max(rounds, 1), 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": w["name"], "graphs": len(spec), "vertices": nv, "edges": ne,
-                   "resident_gib": round(used / 2**30, 2), "updates_timed": n_upd, "rounds": rounds,
+                   "updates_timed": n_upd, "rounds": rounds,
                    "init": f"init streams applied in launches of {args.init_batch} updates per graph, then repaired"},
-        "legs_updates_per_s": {"one_graph_at_a_time": round(n_upd / alone_s, 1),
-                               "host_driven_rounds": round(n_upd / st_host["ms"] * 1e3, 1),
-                               "captured_rounds": round(value, 1)},
-        "batched_vs_one_at_a_time": round(value / (n_upd / alone_s), 2),
+        "legs_updates_per_s": {k: round(v, 1) for k, v in legs.items()},
+        "batched_vs_one_at_a_time": round(value / legs["one_graph_at_a_time"], 2),
         "host_finished_rounds": st["host_rounds"],
-        "legs_counters": {"one_graph_at_a_time": delta(c0, c1), "host_driven_rounds": delta(c1, c2),
-                          "captured_rounds": delta(c2, c3)},
+        "legs": stats,
+        "max_color_sum_captured_vs_one_at_a_time": round(stats["captured_rounds"]["max_color_sum"] /
+                                                         stats["one_graph_at_a_time"]["max_color_sum"], 4),
         "host_rounds_kernels_ms": {k: round(v[0], 2) for k, v in host_k.items()},
         "autotune": {"candidates": st["candidate"], "ns_per_update": st["candidate_ns_per_update"],
                      "chosen": st["chosen"], "rounds_each": args.autotune_rounds},
-        "parity": {**parity, "max_color_by_shape": {k: (max(v) if v else None) for k, v in colours.items()},
-                   "max_color_sum": int(sum(sum(v) for v in colours.values()))},
+        "parity": {**parity, "max_color_by_shape": {k: (max(v) if v else None) for k, v in colours.items()}},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))

@@ -56,6 +56,8 @@ typedef struct sgc_edge_update {
 #define SGC_REPAIR_EVERY_EPOCH (1u << 1) /* epoch = compact, drain, Tier-1, Tier-2, publish (proper snapshots) */
 #define SGC_EXACT_HISTOGRAM (1u << 2)    /* hub histograms recounted wherever bitmaps are rebuilt */
 #define SGC_FULL_SWEEPS (1u << 3)        /* maintenance sweeps visit every vertex (no dirty tracking); same results */
+#define SGC_HOST_TIER2 (1u << 4)         /* Tier-2 repair on the host (the reference's placement); same results */
+#define SGC_ALL_FLAGS 0x1Fu
 
 /* Engine configuration; the defaults (sgc_default_config) are the reference's. */
 typedef struct sgc_config {

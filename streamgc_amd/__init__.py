@@ -46,12 +46,14 @@ class StreamGCConfig:
     repair_every_epoch: bool = False
     exact_histogram: bool = False
     full_sweeps: bool = False
+    host_tier2: bool = False       # Tier-2 on the host, as the reference places it (same results)
 
     def to_c(self):
         flags = (_lib.SGC_DETERMINISTIC if self.deterministic else 0) | \
                 (_lib.SGC_REPAIR_EVERY_EPOCH if self.repair_every_epoch else 0) | \
                 (_lib.SGC_EXACT_HISTOGRAM if self.exact_histogram else 0) | \
-                (_lib.SGC_FULL_SWEEPS if self.full_sweeps else 0)
+                (_lib.SGC_FULL_SWEEPS if self.full_sweeps else 0) | \
+                (_lib.SGC_HOST_TIER2 if self.host_tier2 else 0)
         return _lib.Config(self.epoch_size, self.batch_size, self.histogram_threshold, self.slack_pct,
                            self.min_row_capacity, self.overflow_capacity, self.max_repair_iters,
                            self.plateau_iters, self.tier2_passes, flags)
@@ -319,8 +321,11 @@ class GPUPartition:
         """Launch-shape knobs (long_row, sweep_blocks, apply_blocks); results never change."""
         self._ck(lib().sgc_set_tuning(self.h, C.byref(make_tuning(**knobs))))
 
-    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None, full_sweeps=None):
+    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None, full_sweeps=None,
+                 host_tier2=None):
         """Switch the epoch procedure (and determinism) between phases."""
+        if host_tier2 is not None:
+            self.config.host_tier2 = bool(host_tier2)
         if deterministic is not None:
             self.config.deterministic = bool(deterministic)
         if repair_every_epoch is not None:
@@ -459,9 +464,10 @@ class GraphPool:
     """
 
     def __init__(self, max_graphs=256, batch_capacity=1 << 18, hbm_budget=0, deterministic=False,
-                 repair_every_epoch=False, exact_histogram=False, full_sweeps=False):
+                 repair_every_epoch=False, exact_histogram=False, full_sweeps=False, host_tier2=False):
         flags = StreamGCConfig(deterministic=deterministic, repair_every_epoch=repair_every_epoch,
-                               exact_histogram=exact_histogram, full_sweeps=full_sweeps).to_c().flags
+                               exact_histogram=exact_histogram, full_sweeps=full_sweeps,
+                               host_tier2=host_tier2).to_c().flags
         cfg = _lib.PoolConfig(max_graphs, batch_capacity, flags, 0, hbm_budget)
         h = C.c_void_p()
         check(lib().sgc_pool_create(C.byref(cfg), C.byref(h)))
@@ -509,11 +515,15 @@ class GraphPool:
     def set_tuning(self, **knobs):
         self._ck(lib().sgc_pool_set_tuning(self.h, C.byref(make_tuning(**knobs))))
 
-    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None, full_sweeps=None):
+    def set_mode(self, deterministic=None, repair_every_epoch=None, exact_histogram=None, full_sweeps=None,
+                 host_tier2=None):
         c = StreamGCConfig(deterministic=bool(self.flags & _lib.SGC_DETERMINISTIC),
                            repair_every_epoch=bool(self.flags & _lib.SGC_REPAIR_EVERY_EPOCH),
                            exact_histogram=bool(self.flags & _lib.SGC_EXACT_HISTOGRAM),
-                           full_sweeps=bool(self.flags & _lib.SGC_FULL_SWEEPS))
+                           full_sweeps=bool(self.flags & _lib.SGC_FULL_SWEEPS),
+                           host_tier2=bool(self.flags & _lib.SGC_HOST_TIER2))
+        if host_tier2 is not None:
+            c.host_tier2 = host_tier2
         if deterministic is not None:
             c.deterministic = deterministic
         if repair_every_epoch is not None:

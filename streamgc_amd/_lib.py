@@ -17,6 +17,7 @@ SGC_DETERMINISTIC = 1 << 0
 SGC_REPAIR_EVERY_EPOCH = 1 << 1
 SGC_EXACT_HISTOGRAM = 1 << 2
 SGC_FULL_SWEEPS = 1 << 3
+SGC_HOST_TIER2 = 1 << 4
 
 # numpy view of the 16-byte update record (A/include/types.h:16-21)
 EDGE_UPDATE = np.dtype([("u", "<u4"), ("v", "<u4"), ("type", "u1"), ("pad", "u1", 3), ("graph", "<u4")])

@@ -41,6 +41,10 @@ enum : uint32_t {
     C_UNCOL,            // validator: uncoloured vertices with a live slot
     C_MAXC,             // validator: max colour
     C_ASYM,             // overflow events ever (edges whose two rows may disagree); never reset
+    C_T2TODO,           // device Tier-2: the last pass found a conflict
+    C_T2FIXED,          //   vertices the last pass recoloured
+    C_T2TOTAL,          //   vertices recoloured by this repair
+    C_REVOVF,           //   reverse-only neighbours that did not fit the scratch list
     C_COUNT = 16
 };
 
@@ -53,7 +57,10 @@ enum : uint32_t {
 // store of 1 and concurrent marks never lose one another.  The endpoints of
 // an overflowed edge, whose two rows may disagree, are sticky (M_STICKY) and
 // visited by every sweep.
-enum : uint32_t { M_ROW = 0, M_BITS = 1, M_CHECK = 2, M_T2 = 3, M_STICKY = 4, M_ARRAYS = 5 };
+// Two more byte arrays serve the device Tier-2: M_TODO (vertices to recolour
+// in the current pass) and M_REV (vertices with a neighbour found only in the
+// neighbour's row).
+enum : uint32_t { M_ROW = 0, M_BITS = 1, M_CHECK = 2, M_T2 = 3, M_STICKY = 4, M_TODO = 5, M_REV = 6, M_ARRAYS = 7 };
 enum : uint32_t {
     D_ROW = 1u << M_ROW, D_BITS = 1u << M_BITS, D_CHECK = 1u << M_CHECK, D_T2 = 1u << M_T2,
     D_STATUS = D_CHECK | D_T2, D_ALL = D_ROW | D_BITS | D_CHECK | D_T2
@@ -149,8 +156,8 @@ __device__ __forceinline__ uint32_t add32(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ void add64(uint64_t* p, uint64_t v) {
     __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void max32(uint32_t* p, uint32_t v) {
-    __hip_atomic_fetch_max(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint32_t max32(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_max(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Mark v for the sweeps in `bits`.
@@ -290,6 +297,25 @@ __device__ __forceinline__ void on_neighbor_recolor(const GraphDev& g, uint32_t 
 // CAS) or already holds nb.  Returns the slots scanned; a full row appends to
 // the overflow buffer (entries past capacity are dropped, the counter keeps
 // counting) or, with overflow disabled, reports failure.
+// An insert that lands in a hole may duplicate a copy further along the row
+// (the first-free-or-equal scan meets the hole first,
+// A/src/kernels/local_update.cu:12-40); a later delete then removes one copy
+// per row and the two rows disagree.  Such endpoints are maintained like an
+// overflowed edge's (sticky, and counted in C_ASYM).
+__device__ __forceinline__ void flag_duplicate(const GraphDev& g, uint32_t x, uint32_t nb, uint32_t from,
+                                               uint32_t end) {
+    bool dup = false;
+    for (uint32_t base = from; base < end && !dup; base += WAVE) {
+        const uint32_t j = base + lane();
+        dup = ballot(j < end && ld(&g.col[j]) == nb) != 0;
+    }
+    if (dup && lane() == 0) {
+        add32(&g.ctr[C_ASYM], 1u);
+        mark_sticky(g, x);
+        mark_sticky(g, nb);
+    }
+}
+
 __device__ __forceinline__ uint32_t row_insert(const GraphDev& g, uint32_t x, uint32_t nb, bool use_overflow, bool* failed,
                                WaveAcc* acc) {
     const uint32_t s = gp(g.row_ptr)[x], e = gp(g.row_ptr)[x + 1];
@@ -309,12 +335,15 @@ __device__ __forceinline__ uint32_t row_insert(const GraphDev& g, uint32_t x, ui
                 const uint32_t k = (uint32_t)__builtin_ctzll(m);
                 const uint32_t v0 = __shfl(val[u], (int)k);
                 if (v0 == nb) return b + k - s + 1;
-                uint32_t old = 0;
+                uint32_t old = 0, prev_fill = 0;
                 if (lane() == 0) {
                     old = cas32(&g.col[b + k], INVALID, nb);
-                    if (old == INVALID) max32(&g.fill[x], b + k - s + 1);
+                    if (old == INVALID) prev_fill = max32(&g.fill[x], b + k - s + 1);
                 }
                 old = bcast0(old);
+                prev_fill = bcast0(prev_fill);
+                if (old == INVALID && prev_fill > b + k - s + 1 && nb < g.n)
+                    flag_duplicate(g, x, nb, b + k + 1, s + prev_fill);
                 if (old == INVALID || old == nb) return b + k - s + 1;
                 m &= m - 1;
             }

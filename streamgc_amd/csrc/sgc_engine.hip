@@ -218,6 +218,9 @@ struct sgc_pool {
     uint64_t* h_totals = nullptr;
     EdgeGroups eg;
     sgc_edge_update* d_batch = nullptr;   // staging of host batches
+    void* d_rev = nullptr;                // device Tier-2: reverse-only neighbour list
+    uint32_t* d_rev_count = nullptr;      // its length (2 words)
+    uint32_t rev_cap = 0;
     uint32_t* d_t2_items = nullptr;       // (graph, vertex, old colour) recoloured by the host Tier-2
     size_t t2_items_cap = 0;
     uint32_t* d_list = nullptr;           // sweep worklist (virtual vertex ids)
@@ -434,6 +437,8 @@ struct sgc_pool {
     }
     int tier1(const std::vector<uint32_t>& gids, uint32_t start_it = 0);
     int tier2(const std::vector<uint32_t>& gids);
+    int host_tier2(const std::vector<uint32_t>& list, std::vector<uint32_t>& ran);
+    int device_tier2(const std::vector<uint32_t>& list, std::vector<uint32_t>& host_jobs, std::vector<uint32_t>& ran);
     int publish(const std::vector<uint32_t>& gids);
     int read_drain(const std::vector<uint32_t>& gids);
     int max_colors(const std::vector<uint32_t>& gids);
@@ -514,7 +519,8 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     if (e == hipSuccess && nhd) e = hipMemsetAsync(G.hist, 0, (size_t)nhd * HIST_BINS * 4, s);
     if (e == hipSuccess) e = hipMemsetAsync(G.snap, 0, (size_t)n * 4, s);
     if (e == hipSuccess) e = hipMemsetAsync(G.dirty, 1, (size_t)M_STICKY * mark_stride(n), s);
-    if (e == hipSuccess) e = hipMemsetAsync(G.dirty + (size_t)M_STICKY * mark_stride(n), 0, mark_stride(n), s);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(G.dirty + (size_t)M_STICKY * mark_stride(n), 0, (size_t)(M_ARRAYS - M_STICKY) * mark_stride(n), s);
     if (e == hipSuccess) e = hipMemsetAsync(d_ctr + (size_t)gid * C_COUNT, 0, C_COUNT * 4, s);
     if (e == hipSuccess) e = hipMemcpyAsync(G.deg0, deg0, (size_t)n * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(G.row_ptr, rp.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, s);
@@ -785,10 +791,37 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
     if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(Lc); });
     if (e == hipSuccess) e = read_ctr();
     if (e != hipSuccess) return hip_fail(e, "tier-2 check");
-    // Every graph with bad slots at once: one download, the repairs in
-    // parallel on the host cores, one upload; then only the recoloured
-    // vertices and their neighbourhoods are marked for the next sweeps (as a
-    // recolour on the device marks them).
+    std::vector<uint32_t> bad, host_jobs, ran;
+    for (uint32_t g : check) {
+        gs[g].st.t2_remaining = ctr(g, C_T2BAD);
+        if (gs[g].st.t2_remaining) bad.push_back(g);
+    }
+    if (bad.empty()) return SGC_OK;
+    int rc;
+    if (tracking() && !(flags & SGC_HOST_TIER2)) {
+        if ((rc = device_tier2(bad, host_jobs, ran))) return rc;
+    } else {
+        host_jobs = bad;
+    }
+    if ((rc = host_tier2(host_jobs, ran))) return rc;
+    if (ran.empty()) return SGC_OK;
+    e = set_active(ran);
+    L = ctx(true);
+    Lc = L;
+    if (e == hipSuccess) e = launch_zero_ctr(L, C_T2BAD);
+    if (e == hipSuccess) e = sweep(Lc, M_T2);
+    if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(Lc); });
+    if (e == hipSuccess) e = read_ctr();
+    if (e != hipSuccess) return hip_fail(e, "tier-2 check");
+    for (uint32_t g : ran) gs[g].st.t2_remaining = ctr(g, C_T2BAD);
+    return SGC_OK;
+}
+
+// cpu_serial_repair on the host for the graphs of `list`, all at once: one
+// download, the repairs in parallel on the host cores, one upload; then only
+// the recoloured vertices and their neighbourhoods are marked for the next
+// sweeps (as a recolour on the device marks them).
+int sgc_pool::host_tier2(const std::vector<uint32_t>& list, std::vector<uint32_t>& ran) {
     struct Job {
         uint32_t g;
         std::vector<uint32_t> w, w0, rp, c;
@@ -796,12 +829,9 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
         double ms = 0;
     };
     std::vector<Job> jobs;
-    for (uint32_t g : check) {
-        GraphState& G = gs[g];
-        G.st.t2_remaining = ctr(g, C_T2BAD);
-        if (G.st.t2_remaining) jobs.push_back(Job{g, {}, {}, {}, {}, 0, 0});
-    }
+    for (uint32_t g : list) jobs.push_back(Job{g, {}, {}, {}, {}, 0, 0});
     if (jobs.empty()) return SGC_OK;
+    hipError_t e = hipSuccess;
     auto t0 = std::chrono::steady_clock::now();
     for (Job& J : jobs) {
         const GraphState& G = gs[J.g];
@@ -851,7 +881,6 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
     const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     double cpu = 0;
     for (const Job& J : jobs) cpu += J.ms;
-    std::vector<uint32_t> ran;
     for (const Job& J : jobs) {
         GraphState& G = gs[J.g];
         G.st.t2_fixed = J.fixed;
@@ -863,16 +892,66 @@ int sgc_pool::tier2(const std::vector<uint32_t>& gids) {
         ran.push_back(J.g);
     }
     last_ms[2] += total;
-    if (ran.empty()) return SGC_OK;
-    e = set_active(ran);
-    L = ctx(true);
-    Lc = L;
-    if (e == hipSuccess) e = launch_zero_ctr(L, C_T2BAD);
-    if (e == hipSuccess) e = sweep(Lc, M_T2);
-    if (e == hipSuccess) e = timed("t2_check", [&] { return launch_t2_check(Lc); });
+    return SGC_OK;
+}
+
+// The same repair on the device (k_t2_* kernels) for the graphs of `list`;
+// a graph whose reverse-only neighbours do not fit the scratch list goes to
+// the host instead (host_jobs).
+int sgc_pool::device_tier2(const std::vector<uint32_t>& list, std::vector<uint32_t>& host_jobs,
+                           std::vector<uint32_t>& ran) {
+    auto t0 = std::chrono::steady_clock::now();
+    hipError_t e = hipSuccess;
+    const uint32_t want = std::max<uint32_t>(1u << 20, V);
+    if (rev_cap < want) {
+        e = hipStreamSynchronize(stream);
+        if (d_rev) (void)hipFree(d_rev);
+        d_rev = nullptr;
+        rev_cap = 0;
+        if (e == hipSuccess) e = hipMalloc(&d_rev, (size_t)want * 12);
+        if (e == hipSuccess) rev_cap = want;
+        if (e != hipSuccess) return hip_fail(e, "tier-2 scratch");
+    }
+    e = set_active(list);
+    LaunchCtx L = ctx(true);
+    if (e == hipSuccess) e = launch_zero_ctr(L, C_T2TOTAL);
+    if (e == hipSuccess) e = launch_zero_ctr(L, C_REVOVF);
+    if (e == hipSuccess) e = timed("t2_device", [&] { return launch_t2_begin(L, d_rev, d_rev_count, rev_cap); });
     if (e == hipSuccess) e = read_ctr();
-    if (e != hipSuccess) return hip_fail(e, "tier-2 check");
-    for (uint32_t g : ran) gs[g].st.t2_remaining = ctr(g, C_T2BAD);
+    if (e != hipSuccess) return hip_fail(e, "tier-2");
+    std::vector<uint32_t> dev;
+    for (uint32_t g : list) (ctr(g, C_REVOVF) ? host_jobs : dev).push_back(g);
+    if (dev.empty()) return SGC_OK;
+    if (dev.size() != list.size()) e = set_active(dev);
+    L = ctx(true);
+    if (e == hipSuccess)
+        e = timed("t2_device", [&] { return launch_t2_pass(L, d_rev, d_rev_count, rev_cap, true); });
+    std::vector<uint32_t> act = dev;
+    for (uint32_t pass = 0; e == hipSuccess && !act.empty(); pass++) {
+        e = set_active(act);
+        L = ctx(true);
+        if (e == hipSuccess) e = launch_zero_ctr(L, C_T2TODO);
+        if (e == hipSuccess) e = launch_zero_ctr(L, C_T2FIXED);
+        if (e == hipSuccess)
+            e = timed("t2_device", [&] { return launch_t2_pass(L, d_rev, d_rev_count, rev_cap, false); });
+        if (e == hipSuccess) e = read_ctr();
+        std::vector<uint32_t> next;
+        for (uint32_t g : act)
+            if (pass + 1 < gs[g].cfg.tier2_passes && ctr(g, C_T2TODO) && ctr(g, C_T2FIXED)) next.push_back(g);
+        act.swap(next);
+    }
+    if (e != hipSuccess) return hip_fail(e, "tier-2");
+    const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (uint32_t g : dev) {
+        GraphState& G = gs[g];
+        G.st.t2_fixed = ctr(g, C_T2TOTAL);
+        G.st.t2_ran = 1;
+        t2_runs++;
+        G.t2_count++;
+        G.t2_ms += total / (double)dev.size();
+        ran.push_back(g);
+    }
+    last_ms[2] += total;
     return SGC_OK;
 }
 
@@ -996,7 +1075,7 @@ void sgc_pool::release() {
     seg.release();
     for (void* p : {(void*)d_ctr, (void*)d_totals, (void*)d_batch, (void*)d_tmp16, (void*)d_qids, (void*)d_qout,
                     (void*)eg.keys, (void*)eg.head, (void*)eg.next, (void*)d_list, (void*)d_list_count,
-                    (void*)d_t2_items})
+                    (void*)d_t2_items, d_rev, (void*)d_rev_count})
         if (p) (void)hipFree(p);
     for (void* p : {(void*)h_ctr, (void*)h_totals})
         if (p) (void)hipHostFree(p);
@@ -1496,6 +1575,7 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipMalloc(&P->d_batch, (size_t)batch_cap * sizeof(sgc_edge_update));
     if (e == hipSuccess) e = hipMalloc(&P->d_tmp16, 64);
     if (e == hipSuccess) e = hipMalloc(&P->d_list_count, 8);
+    if (e == hipSuccess) e = hipMalloc(&P->d_rev_count, 8);
     P->eg.size = 16;
     while (P->eg.size < 2u * batch_cap) P->eg.size <<= 1;
     P->eg.next_cap = batch_cap;
@@ -1758,7 +1838,7 @@ int sgc_set_tuning(sgc_engine* e, const sgc_tuning* t) { return set_tuning(e, t)
 int sgc_pool_set_tuning(sgc_pool* p, const sgc_tuning* t) { return set_tuning(p, t); }
 
 int sgc_set_flags(sgc_engine* e, uint32_t flags) {
-    if (!e || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))) return SGC_ERR_INVALID;
+    if (!e || (flags & ~SGC_ALL_FLAGS)) return SGC_ERR_INVALID;
     const bool was = e->tracking();
     if (flags != e->flags) e->batch_drop();
     e->flags = flags;
@@ -2076,8 +2156,7 @@ int sgc_kernel_times(sgc_engine* e, char* names, uint32_t stride, double* total_
 int sgc_pool_create(const sgc_pool_config* cfg, sgc_pool** out) {
     if (!cfg || !out || cfg->max_graphs == 0 || cfg->batch_capacity == 0)
         return global_fail(SGC_ERR_INVALID, "pool create: bad argument");
-    if (cfg->flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))
-        return global_fail(SGC_ERR_INVALID, "pool create: unknown flags");
+    if (cfg->flags & ~SGC_ALL_FLAGS) return global_fail(SGC_ERR_INVALID, "pool create: unknown flags");
     sgc_pool* P = new sgc_pool;
     int r = pool_init(P, cfg->max_graphs, cfg->batch_capacity, cfg->flags, cfg->hbm_budget);
     if (r != SGC_OK) {
@@ -2126,7 +2205,7 @@ int sgc_pool_usage(sgc_pool* p, uint64_t* used, uint64_t* budget, uint32_t* grap
     return SGC_OK;
 }
 int sgc_pool_set_flags(sgc_pool* p, uint32_t flags) {
-    if (!p || (flags & ~(SGC_DETERMINISTIC | SGC_REPAIR_EVERY_EPOCH | SGC_EXACT_HISTOGRAM | SGC_FULL_SWEEPS))) return SGC_ERR_INVALID;
+    if (!p || (flags & ~SGC_ALL_FLAGS)) return SGC_ERR_INVALID;
     const bool was = p->tracking();
     if (flags != p->flags) p->batch_drop();   // captured rounds follow the flags they were made with
     p->flags = flags;

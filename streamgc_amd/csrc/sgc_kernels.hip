@@ -1005,6 +1005,230 @@ __global__ __launch_bounds__(256) void k_mark_recolored(const GraphDev* __restri
     }
 }
 
+// ---------------------------------------------------------------- Tier-2 on the device
+// cpu_serial_repair (A/src/streamgc.cu:1057-1240) without leaving the GPU,
+// with the same result as the host restatement: the adjacency of a vertex is
+// its row plus every vertex whose row holds it ("reverse-only" neighbours
+// exist only where the two rows of an edge disagree, i.e. between sticky
+// vertices); uncoloured vertices with an edge are coloured first-fit in id
+// order; then each pass takes the higher-id endpoint of every same-coloured
+// edge and recolours them in ascending order with the lowest colour in
+// [1, 1024) no neighbour holds.  The sequential parts run on one 1024-thread
+// block per graph; the scans over all rows are grid-wide.
+struct RevPair {
+    uint32_t gid, v, u;   // u holds v in its row; v's row lacks u
+};
+constexpr uint32_t T2_BLOCK = 1024;
+
+__device__ __forceinline__ bool mark_byte(const GraphDev& g, uint32_t kind, uint32_t v) {
+    return gp(g.dirty)[(size_t)kind * g.dstride + v] != 0;
+}
+
+// Clear the pass flags and the reverse-only flags of the graphs taking part.
+__global__ __launch_bounds__(256) void k_t2_clear(const GraphDev* __restrict__ graphs, Space sp,
+                                                  const uint32_t* __restrict__ active) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sp.total; i += gridDim.x * blockDim.x) {
+        uint32_t gid, v;
+        sp.locate(i, gid, v);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        gp(g.dirty)[(size_t)M_TODO * g.dstride + v] = 0;
+        gp(g.dirty)[(size_t)M_REV * g.dstride + v] = 0;
+    }
+}
+
+// Reverse-only neighbours: for a sticky u and a sticky v in u's row, (v, u)
+// when v's row lacks u.  One wavefront per vertex.
+__global__ __launch_bounds__(256) void k_t2_rev(const GraphDev* __restrict__ graphs, Space sp,
+                                                const uint32_t* __restrict__ active, RevPair* __restrict__ rev,
+                                                uint32_t* __restrict__ rev_count, uint32_t cap) {
+    for (uint32_t i = wave_id(); i < sp.total; i += wave_count()) {
+        uint32_t gid, u;
+        sp.locate(i, gid, u);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        if (!mark_byte(g, M_STICKY, u)) continue;
+        const uint32_t s = gp(g.row_ptr)[u], e = s + ld(&g.fill[u]);
+        for (uint32_t base = s; base < e; base += WAVE) {
+            const uint32_t j = base + lane();
+            const uint32_t v = j < e ? ld(&g.col[j]) : INVALID;
+            uint64_t m = ballot(v != INVALID && v < g.n && v != u && mark_byte(g, M_STICKY, v));
+            while (m) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const uint32_t w = (uint32_t)__shfl((int)v, (int)k);
+                const uint32_t s2 = gp(g.row_ptr)[w], e2 = s2 + ld(&g.fill[w]);
+                bool found = false;
+                for (uint32_t b2 = s2; b2 < e2 && !found; b2 += WAVE) {
+                    const uint32_t j2 = b2 + lane();
+                    found = ballot(j2 < e2 && ld(&g.col[j2]) == u) != 0;
+                }
+                if (!found && lane() == 0) {
+                    const uint32_t at = add32(rev_count, 1u);
+                    if (at < cap) {
+                        st(&rev[at].gid, gid);
+                        st(&rev[at].v, w);
+                        st(&rev[at].u, u);
+                    } else {
+                        add32(&g.ctr[C_REVOVF], 1u);
+                    }
+                    gp(g.dirty)[(size_t)M_REV * g.dstride + w] = 1;
+                }
+            }
+        }
+    }
+}
+
+// first_fit of the host restatement for vertex v, by the whole block.
+// Returns true when v changed colour (it is then marked like any recolour).
+__device__ __forceinline__ bool t2_first_fit(const GraphDev& g, uint32_t gid, uint32_t v,
+                                             const RevPair* __restrict__ rev, uint32_t nrev) {
+    __shared__ uint32_t seen[HIST_BINS / 32];
+    __shared__ uint32_t pick;
+    if (threadIdx.x < HIST_BINS / 32) seen[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
+    for (uint32_t j = s + threadIdx.x; j < e; j += T2_BLOCK) {
+        const uint32_t nb = ld(&g.col[j]);
+        if (nb == INVALID || nb >= g.n || nb == v) continue;
+        const uint32_t c = color_of(ld(&g.word[nb]));
+        if (c != 0 && c < HIST_BINS) atomicOr(&seen[c >> 5], 1u << (c & 31));
+    }
+    if (mark_byte(g, M_REV, v))
+        for (uint32_t k = threadIdx.x; k < nrev; k += T2_BLOCK) {
+            if (rev[k].gid != gid || rev[k].v != v) continue;
+            const uint32_t c = color_of(ld(&g.word[rev[k].u]));
+            if (c != 0 && c < HIST_BINS) atomicOr(&seen[c >> 5], 1u << (c & 31));
+        }
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+        uint32_t word = threadIdx.x < HIST_BINS / 32 ? seen[threadIdx.x] : 0xFFFFFFFFu;
+        if (threadIdx.x == 0) word |= 1u;   // colour 0 is not a colour
+        const uint64_t open = ballot(word != 0xFFFFFFFFu);
+        uint32_t c = 0;
+        if (open) {
+            const uint32_t k = (uint32_t)__builtin_ctzll(open);
+            const uint32_t wk = (uint32_t)__shfl((int)word, (int)k);
+            c = 32 * k + (uint32_t)__builtin_ctz(~wk);
+        }
+        if (threadIdx.x == 0) {
+            const uint32_t w = ld(&g.word[v]);
+            const uint32_t oc = color_of(w);
+            if (c == 0 || c == oc) {
+                pick = 0;
+            } else {
+                st(&g.word[v], bump(w, (uint16_t)c));
+                pick = 0x80000000u | (oc << 16) | c;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t p = pick;
+    __syncthreads();
+    if (!p) return false;
+    if (threadIdx.x < WAVE) {
+        if (lane() == 0) mark(g, v, D_STATUS);
+        mark_neighbors(g, v, (uint16_t)((p >> 16) & 0x7FFFu), (uint16_t)(p & 0xFFFFu));
+    }
+    return true;
+}
+
+// Vertices of one graph flagged in mark array `kind` (or, for kind ==
+// M_ARRAYS, uncoloured vertices with an edge), visited in ascending order by
+// the block: flags are cleared and first_fit applied.  Returns the recolours.
+__device__ __forceinline__ uint32_t t2_ordered_pass(const GraphDev& g, uint32_t gid, uint32_t kind,
+                                                    const RevPair* __restrict__ rev, uint32_t nrev) {
+    __shared__ uint64_t cand[T2_BLOCK / WAVE];
+    __shared__ uint32_t has;
+    uint32_t fixed = 0;
+    for (uint32_t base = 0; base < g.n; base += T2_BLOCK) {
+        const uint32_t v = base + threadIdx.x;
+        bool c = false;
+        if (v < g.n) {
+            if (kind == M_ARRAYS) {
+                c = color_of(ld(&g.word[v])) == 0;
+            } else if (mark_byte(g, kind, v)) {
+                c = true;
+                gp(g.dirty)[(size_t)kind * g.dstride + v] = 0;
+            }
+        }
+        const uint64_t m = ballot(c);
+        if (lane() == 0) cand[threadIdx.x / WAVE] = m;
+        __syncthreads();
+        for (uint32_t w = 0; w < T2_BLOCK / WAVE; w++) {
+            uint64_t mw = cand[w];
+            while (mw) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(mw);
+                mw &= mw - 1;
+                const uint32_t u = base + w * WAVE + b;
+                if (kind == M_ARRAYS) {   // only a vertex with an edge is coloured
+                    if (threadIdx.x == 0) has = mark_byte(g, M_REV, u) ? 1u : 0u;
+                    __syncthreads();
+                    const uint32_t s = gp(g.row_ptr)[u], e = s + ld(&g.fill[u]);
+                    bool any = false;
+                    for (uint32_t j = s + threadIdx.x; j < e && !any; j += T2_BLOCK) {
+                        const uint32_t nb = ld(&g.col[j]);
+                        any = nb != INVALID && nb < g.n && nb != u;
+                    }
+                    if (any) has = 1;
+                    __syncthreads();
+                    const bool go = has != 0;
+                    __syncthreads();
+                    if (!go) continue;
+                }
+                fixed += t2_first_fit(g, gid, u, rev, nrev) ? 1u : 0u;
+            }
+        }
+        __syncthreads();
+    }
+    return fixed;
+}
+
+// The uncoloured pass (pass == 0) or one conflict pass over the flagged
+// vertices, one block per graph taking part.
+__global__ __launch_bounds__(T2_BLOCK) void k_t2_serial(const GraphDev* __restrict__ graphs, uint32_t ng,
+                                                        const uint32_t* __restrict__ active,
+                                                        const RevPair* __restrict__ rev,
+                                                        const uint32_t* __restrict__ rev_count, uint32_t cap,
+                                                        int uncoloured) {
+    const uint32_t nrev = *rev_count < cap ? *rev_count : cap;
+    for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
+        if (active && !active[gid]) continue;
+        const GraphDev g = graphs[gid];
+        const uint32_t f = t2_ordered_pass(g, gid, uncoloured ? M_ARRAYS : M_TODO, rev, nrev);
+        if (threadIdx.x == 0) {
+            if (!uncoloured) gp(g.ctr)[C_T2FIXED] = f;
+            gp(g.ctr)[C_T2TOTAL] += f;
+        }
+        __syncthreads();
+    }
+}
+
+// The next pass's list: the higher-id endpoint of every same-coloured slot
+// pair of every row.  One wavefront per row; C_T2TODO becomes 1 if any.
+__global__ __launch_bounds__(256) void k_t2_flags(const GraphDev* __restrict__ graphs, Space sp,
+                                                  const uint32_t* __restrict__ active) {
+    for (uint32_t i = wave_id(); i < sp.total; i += wave_count()) {
+        uint32_t gid, x;
+        sp.locate(i, gid, x);
+        if (active && !active[gid]) continue;
+        const GraphDev& g = graphs[gid];
+        const uint32_t cx = color_of(ld(&g.word[x]));
+        if (cx == 0) continue;
+        const uint32_t s = gp(g.row_ptr)[x], e = s + ld(&g.fill[x]);
+        bool any = false;
+        for (uint32_t base = s; base < e; base += WAVE) {
+            const uint32_t j = base + lane();
+            const uint32_t y = j < e ? ld(&g.col[j]) : INVALID;
+            if (y != INVALID && y < g.n && y != x && color_of(ld(&g.word[y])) == cx) {
+                gp(g.dirty)[(size_t)M_TODO * g.dstride + (y > x ? y : x)] = 1;
+                any = true;
+            }
+        }
+        if (ballot(any) && lane() == 0) gp(g.ctr)[C_T2TODO] = 1;
+    }
+}
+
 // ---------------------------------------------------------------- checks
 // Tier-2 has work iff some row slot joins an uncoloured endpoint or two
 // endpoints of one colour (A/src/streamgc.cu:1190-1218 then change nothing).
@@ -1268,6 +1492,21 @@ hipError_t launch_t2_check(const LaunchCtx& L) {
 hipError_t launch_mark_recolored(hipStream_t s, const GraphDev* graphs, const uint32_t* items, uint32_t count) {
     if (count == 0) return hipSuccess;
     k_mark_recolored<<<grid_for_waves(count, 4, 4096), 256, 0, s>>>(graphs, items, count);
+    return hipGetLastError();
+}
+hipError_t launch_t2_begin(const LaunchCtx& L, void* rev, uint32_t* rev_count, uint32_t cap) {
+    k_zero_list_count<<<1, 1, 0, L.stream>>>(rev_count);   // [0] is the pair count
+    k_t2_clear<<<grid_for_waves(L.V, 4 * WAVE, 4096), 256, 0, L.stream>>>(L.graphs, space(L), L.active);
+    k_t2_rev<<<grid_for_waves(L.V, 4, L.max_blocks), 256, 0, L.stream>>>(L.graphs, space(L), L.active,
+                                                                         (RevPair*)rev, rev_count, cap);
+    return hipGetLastError();
+}
+hipError_t launch_t2_pass(const LaunchCtx& L, const void* rev, const uint32_t* rev_count, uint32_t cap,
+                          bool uncoloured) {
+    if (!uncoloured)
+        k_t2_flags<<<grid_for_waves(L.V, 4, L.max_blocks), 256, 0, L.stream>>>(L.graphs, space(L), L.active);
+    k_t2_serial<<<L.ng < 1024 ? L.ng : 1024, T2_BLOCK, 0, L.stream>>>(L.graphs, L.ng, L.active, (const RevPair*)rev,
+                                                                      rev_count, cap, uncoloured ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors) {

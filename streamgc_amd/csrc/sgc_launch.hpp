@@ -60,6 +60,13 @@ hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist);
 // 2 all rows; see tier1_vertex).
 hipError_t launch_tier1(const LaunchCtx& L, bool serial, int fresh);
 hipError_t launch_t2_check(const LaunchCtx& L);
+// Tier-2 on the device for the active graphs: begin clears the pass flags and
+// lists the reverse-only neighbours of sticky vertices (rev: cap 12-byte
+// entries, rev_count: 2 device words); each pass is the uncoloured pass or a
+// conflict pass (flag scan, then the ordered recolouring).
+hipError_t launch_t2_begin(const LaunchCtx& L, void* rev, uint32_t* rev_count, uint32_t cap);
+hipError_t launch_t2_pass(const LaunchCtx& L, const void* rev, const uint32_t* rev_count, uint32_t cap,
+                          bool uncoloured);
 // Marks for the vertices a host Tier-2 recoloured: (graph, vertex, old colour) triples.
 hipError_t launch_mark_recolored(hipStream_t s, const GraphDev* graphs, const uint32_t* items, uint32_t count);
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors);

@@ -96,16 +96,17 @@ def test_deterministic_histogram_path_bit_exact(threshold, exact, graph_path):
         _same_state(gpu, orc, f"{name} threshold {threshold}")
 
 
-@pytest.mark.parametrize("sweeps", ["tracked", "full"])
+@pytest.mark.parametrize("sweeps,t2", [("tracked", "device"), ("tracked", "host"), ("full", "host")])
 @pytest.mark.parametrize("repair", [False, True])
 @pytest.mark.parametrize("slack,min_cap,cap", [(0, 1, 8), (0, 2, 4096), (50, 2, 64)])
-def test_deterministic_overflow_and_tier2_bit_exact(slack, min_cap, cap, repair, sweeps, graph_path):
+def test_deterministic_overflow_and_tier2_bit_exact(slack, min_cap, cap, repair, sweeps, t2, graph_path):
     # overflowed edges leave the two rows disagreeing (sticky endpoints of the
-    # dirty tracking); both sweep modes must match the oracle's full sweeps
+    # dirty tracking); both sweep modes, and Tier-2 on the device or on the
+    # host, must match the oracle's full sweeps and host Tier-2
     g = S.load_graph_csr(graph_path("er2k"))
     gpu, orc, og = _pair(g, deterministic=True, epoch_size=128, batch_size=128, slack_pct=slack,
                          min_row_capacity=min_cap, overflow_capacity=cap, repair_every_epoch=repair,
-                         full_sweeps=(sweeps == "full"))
+                         full_sweeps=(sweeps == "full"), host_tier2=(t2 == "host"))
     init = S.generate_init_stream(g)
     assert gpu.process_stream(init) == orc.submit(init["u"], init["v"], init["type"])
     assert gpu.finish() == orc.finish()
