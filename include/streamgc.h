@@ -319,7 +319,8 @@ typedef struct sgc_batch_config {
     uint32_t use_graph;        /* 1: replay a captured hipGraph per round; 0: host-driven launches */
     uint32_t t1_sweeps;        /* Tier-1 sweeps captured per round (0: the graphs' max_repair_iters) */
     uint32_t autotune_rounds;  /* > 0: rounds given to each launch-shape candidate (hipGraph mode) */
-    uint32_t reserved;
+    uint32_t t2_passes;        /* device Tier-2 passes captured per round (0: 4); a graph needing more, or a
+                                  Tier-2 on the host (SGC_HOST_TIER2), makes the host finish the round */
 } sgc_batch_config;
 #define SGC_BATCH_CANDIDATES 4
 typedef struct sgc_batch_stats {

@@ -552,7 +552,7 @@ class GraphPool:
         """E_g of one graph (takes effect at its next batch)."""
         self._ck(lib().sgc_pool_set_epoch_size(self.h, gid, epoch_size))
 
-    def batch_prepare(self, records, use_graph=True, t1_sweeps=0, autotune_rounds=0):
+    def batch_prepare(self, records, use_graph=True, t1_sweeps=0, autotune_rounds=0, t2_passes=0):
         """Batched pattern: make the round plan of a mixed stream and keep it on the device.
 
         Round r gives every graph its next min(E_g, left) records, then every
@@ -565,6 +565,7 @@ class GraphPool:
         cfg.use_graph = 1 if use_graph else 0
         cfg.t1_sweeps = t1_sweeps
         cfg.autotune_rounds = autotune_rounds
+        cfg.t2_passes = t2_passes
         r = C.c_uint32()
         self._ck(lib().sgc_pool_batch_prepare(self.h, records.ctypes.data, len(records), C.byref(cfg), C.byref(r)))
         return r.value

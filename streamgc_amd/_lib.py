@@ -85,7 +85,7 @@ class RunMeta(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("use_graph", C.c_uint32), ("t1_sweeps", C.c_uint32), ("autotune_rounds", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("t2_passes", C.c_uint32)]
 
 
 BATCH_CANDIDATES = 4

@@ -172,7 +172,7 @@ struct Staged {
 // candidate.
 struct BatchRun {
     sgc_batch_config cfg{};
-    uint32_t rounds = 0, ng = 0, max_count = 0, max_n = 0, sweeps = 0, ovf_cap = 0;
+    uint32_t rounds = 0, ng = 0, max_count = 0, max_n = 0, sweeps = 0, ovf_cap = 0, t2_passes = 0;
     uint64_t updates = 0;
     bool multi = false, any_history = false, all_settled = true;
     std::vector<uint32_t> tab;                 // per round: first record, record count
@@ -438,6 +438,19 @@ struct sgc_pool {
     int tier1(const std::vector<uint32_t>& gids, uint32_t start_it = 0);
     int tier2(const std::vector<uint32_t>& gids);
     int host_tier2(const std::vector<uint32_t>& list, std::vector<uint32_t>& ran);
+    // Scratch of the device Tier-2's reverse-only neighbour list (12 bytes an entry).
+    int ensure_rev_scratch() {
+        const uint32_t want = std::max<uint32_t>(1u << 20, V);
+        if (rev_cap >= want) return SGC_OK;
+        hipError_t e = hipStreamSynchronize(stream);
+        if (d_rev) (void)hipFree(d_rev);
+        d_rev = nullptr;
+        rev_cap = 0;
+        if (e == hipSuccess) e = hipMalloc(&d_rev, (size_t)want * 12);
+        if (e == hipSuccess) rev_cap = want;
+        if (e != hipSuccess) return hip_fail(e, "tier-2 scratch");
+        return SGC_OK;
+    }
     int device_tier2(const std::vector<uint32_t>& list, std::vector<uint32_t>& host_jobs, std::vector<uint32_t>& ran);
     int publish(const std::vector<uint32_t>& gids);
     int read_drain(const std::vector<uint32_t>& gids);
@@ -901,18 +914,9 @@ int sgc_pool::host_tier2(const std::vector<uint32_t>& list, std::vector<uint32_t
 int sgc_pool::device_tier2(const std::vector<uint32_t>& list, std::vector<uint32_t>& host_jobs,
                            std::vector<uint32_t>& ran) {
     auto t0 = std::chrono::steady_clock::now();
-    hipError_t e = hipSuccess;
-    const uint32_t want = std::max<uint32_t>(1u << 20, V);
-    if (rev_cap < want) {
-        e = hipStreamSynchronize(stream);
-        if (d_rev) (void)hipFree(d_rev);
-        d_rev = nullptr;
-        rev_cap = 0;
-        if (e == hipSuccess) e = hipMalloc(&d_rev, (size_t)want * 12);
-        if (e == hipSuccess) rev_cap = want;
-        if (e != hipSuccess) return hip_fail(e, "tier-2 scratch");
-    }
-    e = set_active(list);
+    int rc = ensure_rev_scratch();
+    if (rc) return rc;
+    hipError_t e = set_active(list);
     LaunchCtx L = ctx(true);
     if (e == hipSuccess) e = launch_zero_ctr(L, C_T2TOTAL);
     if (e == hipSuccess) e = launch_zero_ctr(L, C_REVOVF);
@@ -1182,6 +1186,7 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
         rg[g].max_iters = gs[g].cfg.max_repair_iters;
         rg[g].plateau_iters = gs[g].cfg.plateau_iters;
         rg[g].n = gs[g].n;
+        rg[g].t2_passes = gs[g].cfg.tier2_passes;
         base[g] = gs[g].snap;
         max_iters = std::max(max_iters, gs[g].cfg.max_repair_iters);
         B->max_n = std::max(B->max_n, gs[g].n);
@@ -1190,6 +1195,11 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
         B->all_settled &= gs[g].settled;
     }
     B->sweeps = c.t1_sweeps ? std::min(c.t1_sweeps, max_iters) : max_iters;
+    // Tier-2 passes captured per round (host placement or no dirty tracking: none, the host repairs)
+    uint32_t max_passes = 0;
+    for (uint32_t g = 0; g < ng; g++)
+        if (gs[g].used) max_passes = std::max(max_passes, gs[g].cfg.tier2_passes);
+    B->t2_passes = (flags & SGC_HOST_TIER2) ? 0 : std::min(c.t2_passes ? c.t2_passes : 4u, max_passes);
     hipError_t e = hipMalloc((void**)&B->d_stream, std::max<size_t>(ordered.size(), 1) * sizeof(sgc_edge_update));
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_tab, std::max<size_t>(B->tab.size(), 1) * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_seg, std::max<size_t>(seg.size(), 1) * 4);
@@ -1215,6 +1225,10 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
     if (e == hipSuccess) e = hipMemsetAsync(B->d_state, 0, (size_t)ng * sizeof(RoundState), stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "batch prepare");
+    if (c.use_graph && B->t2_passes) {
+        int r = ensure_rev_scratch();
+        if (r) return r;
+    }
     // launch-shape candidates: the pool's shapes, then one knob changed at a time
     B->cands.push_back(tune);
     if (c.use_graph && c.autotune_rounds) {
@@ -1308,6 +1322,26 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
     Lt.active = F(F_T2);
     if (e == hipSuccess) e = launch_build_list(Lt, M_T2, d_list, list_cap, d_list_count);
     if (e == hipSuccess) e = launch_t2_check(Lt);
+    if (B.t2_passes) {   // Tier-2 on the device for the graphs whose check found bad slots
+        LaunchCtx L2 = base;
+        L2.active = F(F_T2RUN);
+        L2.gate = B.d_ctl + RC_T2_ANY;
+        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 0, 0);
+        if (e == hipSuccess) e = launch_t2_begin(L2, d_rev, d_rev_count, rev_cap);
+        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 1, 0);
+        if (e == hipSuccess) e = launch_t2_pass(L2, d_rev, d_rev_count, rev_cap, true);
+        for (uint32_t p = 0; p < B.t2_passes && e == hipSuccess; p++) {
+            e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 2, p);
+            if (e == hipSuccess) e = launch_t2_pass(L2, d_rev, d_rev_count, rev_cap, false);
+            if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 3, p);
+        }
+        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 4, 0);
+        LaunchCtx Lc = base;   // the remaining bad slots, as the host's check after a repair counts them
+        Lc.active = F(F_T2RAN);
+        Lc.gate = B.d_ctl + RC_T2_RAN;
+        if (e == hipSuccess) e = launch_build_list(Lc, M_T2, d_list, list_cap, d_list_count);
+        if (e == hipSuccess) e = launch_t2_check(Lc);
+    }
     if (e == hipSuccess) e = launch_round_end(stream, B.d_ctl, graphs.d, ng, B.d_fl);
     if (e == hipSuccess && !det) {   // bitmaps and histograms exact again for the next updates
         e = launch_build_list(Lr, M_BITS, d_list, list_cap, d_list_count);
@@ -2373,7 +2407,7 @@ void sgc_default_batch_config(sgc_batch_config* c) {
     c->use_graph = 1;
     c->t1_sweeps = 0;
     c->autotune_rounds = 0;
-    c->reserved = 0;
+    c->t2_passes = 0;
 }
 int sgc_pool_batch_prepare(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, const sgc_batch_config* cfg,
                            uint32_t* rounds) {

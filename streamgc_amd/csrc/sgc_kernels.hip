@@ -1026,7 +1026,8 @@ __device__ __forceinline__ bool mark_byte(const GraphDev& g, uint32_t kind, uint
 
 // Clear the pass flags and the reverse-only flags of the graphs taking part.
 __global__ __launch_bounds__(256) void k_t2_clear(const GraphDev* __restrict__ graphs, Space sp,
-                                                  const uint32_t* __restrict__ active) {
+                                                  const uint32_t* __restrict__ active, const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sp.total; i += gridDim.x * blockDim.x) {
         uint32_t gid, v;
         sp.locate(i, gid, v);
@@ -1041,7 +1042,9 @@ __global__ __launch_bounds__(256) void k_t2_clear(const GraphDev* __restrict__ g
 // when v's row lacks u.  One wavefront per vertex.
 __global__ __launch_bounds__(256) void k_t2_rev(const GraphDev* __restrict__ graphs, Space sp,
                                                 const uint32_t* __restrict__ active, RevPair* __restrict__ rev,
-                                                uint32_t* __restrict__ rev_count, uint32_t cap) {
+                                                uint32_t* __restrict__ rev_count, uint32_t cap,
+                                                const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     for (uint32_t i = wave_id(); i < sp.total; i += wave_count()) {
         uint32_t gid, u;
         sp.locate(i, gid, u);
@@ -1190,7 +1193,8 @@ __global__ __launch_bounds__(T2_BLOCK) void k_t2_serial(const GraphDev* __restri
                                                         const uint32_t* __restrict__ active,
                                                         const RevPair* __restrict__ rev,
                                                         const uint32_t* __restrict__ rev_count, uint32_t cap,
-                                                        int uncoloured) {
+                                                        int uncoloured, const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const uint32_t nrev = *rev_count < cap ? *rev_count : cap;
     for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
         if (active && !active[gid]) continue;
@@ -1207,7 +1211,8 @@ __global__ __launch_bounds__(T2_BLOCK) void k_t2_serial(const GraphDev* __restri
 // The next pass's list: the higher-id endpoint of every same-coloured slot
 // pair of every row.  One wavefront per row; C_T2TODO becomes 1 if any.
 __global__ __launch_bounds__(256) void k_t2_flags(const GraphDev* __restrict__ graphs, Space sp,
-                                                  const uint32_t* __restrict__ active) {
+                                                  const uint32_t* __restrict__ active, const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     for (uint32_t i = wave_id(); i < sp.total; i += wave_count()) {
         uint32_t gid, x;
         sp.locate(i, gid, x);
@@ -1496,17 +1501,18 @@ hipError_t launch_mark_recolored(hipStream_t s, const GraphDev* graphs, const ui
 }
 hipError_t launch_t2_begin(const LaunchCtx& L, void* rev, uint32_t* rev_count, uint32_t cap) {
     k_zero_list_count<<<1, 1, 0, L.stream>>>(rev_count);   // [0] is the pair count
-    k_t2_clear<<<grid_for_waves(L.V, 4 * WAVE, 4096), 256, 0, L.stream>>>(L.graphs, space(L), L.active);
+    k_t2_clear<<<grid_for_waves(L.V, 4 * WAVE, 4096), 256, 0, L.stream>>>(L.graphs, space(L), L.active, L.gate);
     k_t2_rev<<<grid_for_waves(L.V, 4, L.max_blocks), 256, 0, L.stream>>>(L.graphs, space(L), L.active,
-                                                                         (RevPair*)rev, rev_count, cap);
+                                                                         (RevPair*)rev, rev_count, cap, L.gate);
     return hipGetLastError();
 }
 hipError_t launch_t2_pass(const LaunchCtx& L, const void* rev, const uint32_t* rev_count, uint32_t cap,
                           bool uncoloured) {
     if (!uncoloured)
-        k_t2_flags<<<grid_for_waves(L.V, 4, L.max_blocks), 256, 0, L.stream>>>(L.graphs, space(L), L.active);
+        k_t2_flags<<<grid_for_waves(L.V, 4, L.max_blocks), 256, 0, L.stream>>>(L.graphs, space(L), L.active,
+                                                                               L.gate);
     k_t2_serial<<<L.ng < 1024 ? L.ng : 1024, T2_BLOCK, 0, L.stream>>>(L.graphs, L.ng, L.active, (const RevPair*)rev,
-                                                                      rev_count, cap, uncoloured ? 1 : 0);
+                                                                      rev_count, cap, uncoloured ? 1 : 0, L.gate);
     return hipGetLastError();
 }
 hipError_t launch_validate(const LaunchCtx& L, uint32_t gid, uint32_t n, const uint16_t* colors) {

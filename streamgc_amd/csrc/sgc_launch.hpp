@@ -85,16 +85,19 @@ enum : uint32_t {
     RC_OFF,           // current round: first record in the resident stream
     RC_COUNT,         //   its record count
     RC_SEG,           //   offset of its segment table (deterministic replay)
+    RC_T2_ANY,        // some graph runs another device Tier-2 pass
+    RC_T2_RAN,        // some graph ran the device Tier-2 this round
     RC_WORDS = 16
 };
 // Per-graph flag arrays (ng u32 each): taking part in the round, Tier-1
-// iterating, parallel sweep next, serial sweep next, Tier-2 check.
-enum : uint32_t { F_ROUND = 0, F_ITER, F_PAR, F_SER, F_T2, F_KINDS };
+// iterating, parallel sweep next, serial sweep next, Tier-2 check, device
+// Tier-2 passes still running, device Tier-2 ran this round.
+enum : uint32_t { F_ROUND = 0, F_ITER, F_PAR, F_SER, F_T2, F_T2RUN, F_T2RAN, F_KINDS };
 struct RoundState {   // Tier-1 loop state of one graph in the current round
     uint32_t iterating, prev, plateau, iters, last, clean;
 };
 struct RoundGraph {   // constant per graph during a run
-    uint32_t max_iters, plateau_iters, n;
+    uint32_t max_iters, plateau_iters, n, t2_passes;
 };
 struct HistDev {      // a graph's snapshot history
     uint16_t* base;
@@ -108,6 +111,12 @@ hipError_t launch_t1_control(hipStream_t s, uint32_t* ctl, const GraphDev* graph
 hipError_t launch_t2_prep(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
                           const RoundState* st);
 hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl);
+// Device Tier-2 control of a round: 0 after the check (graphs with bad slots
+// run it), 1 after the reverse-only list (a graph whose list overflowed stops
+// the round), 2 before pass `pass`, 3 after it, 4 at the end (graphs still
+// running stop the round; the graphs that ran are checked again).
+hipError_t launch_t2_control(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
+                             const RoundGraph* rg, int step, uint32_t pass);
 hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, uint32_t max_n, const uint32_t* fl_round,
                                 uint16_t* const* snap_base, uint32_t* pub_half, HistDev* hist, bool any_history);
 

@@ -67,8 +67,12 @@ __global__ __launch_bounds__(CTL_THREADS) void k_round_begin(uint32_t* __restric
         *flag(fl, ng, F_PAR, g) = 0;
         *flag(fl, ng, F_SER, g) = 0;
         *flag(fl, ng, F_T2, g) = 0;
+        *flag(fl, ng, F_T2RUN, g) = 0;
+        *flag(fl, ng, F_T2RAN, g) = 0;
     }
     if (threadIdx.x == 0) {
+        ctl[RC_T2_ANY] = 0;
+        ctl[RC_T2_RAN] = 0;
         ctl[RC_OFF] = live ? tab[2 * (size_t)r] : 0u;
         ctl[RC_COUNT] = live ? tab[2 * (size_t)r + 1] : 0u;
         ctl[RC_SEG] = live ? r * 2u * ng : 0u;
@@ -152,28 +156,74 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ 
     }
 }
 
-// After the Tier-2 check: a graph with a bad slot stops the round.  A stopped
-// round publishes nothing (its graphs leave the round); otherwise the round
-// index advances.
+// Device Tier-2 inside a round (k_t2_* kernels), controlled per graph as the
+// host's device_tier2 controls it.
+__global__ __launch_bounds__(CTL_THREADS) void k_t2_control(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
+                                                            uint32_t ng, uint32_t* __restrict__ fl,
+                                                            const RoundGraph* __restrict__ rg, int step, uint32_t pass) {
+    const bool stopped = ctl[RC_STOP] != 0;
+    uint32_t any = 0, halt = 0;
+    for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
+        uint32_t* run = flag(fl, ng, F_T2RUN, g);
+        if (step == 0) {   // the graphs whose check found bad slots
+            const uint32_t r = !stopped && *flag(fl, ng, F_T2, g) && gp(graphs[g].ctr)[C_T2BAD] != 0;
+            *run = r;
+            *flag(fl, ng, F_T2RAN, g) = r;
+            if (r) {
+                SGC_G uint32_t* c = gp(graphs[g].ctr);
+                c[C_T2TOTAL] = 0;
+                c[C_REVOVF] = 0;
+            }
+            any |= r;
+        } else if (step == 1) {   // a reverse-only list that did not fit: the host repairs
+            if (*run && gp(graphs[g].ctr)[C_REVOVF] != 0) halt = 1;
+            any |= *run;
+        } else if (step == 2) {
+            if (*run) {
+                SGC_G uint32_t* c = gp(graphs[g].ctr);
+                c[C_T2TODO] = 0;
+                c[C_T2FIXED] = 0;
+            }
+            any |= *run;
+        } else if (step == 3) {
+            if (*run) {
+                SGC_G uint32_t* c = gp(graphs[g].ctr);
+                *run = pass + 1 < rg[g].t2_passes && c[C_T2TODO] != 0 && c[C_T2FIXED] != 0;
+            }
+            any |= *run;
+        } else {   // step 4
+            if (*run) halt = 1;
+            if (*flag(fl, ng, F_T2RAN, g)) gp(graphs[g].ctr)[C_T2BAD] = 0;
+        }
+    }
+    any = block_any(any);
+    halt = block_any(halt);
+    if (step == 1 && halt)
+        for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
+            *flag(fl, ng, F_T2RUN, g) = 0;
+            *flag(fl, ng, F_T2RAN, g) = 0;
+        }
+    if (threadIdx.x == 0) {
+        if (halt && !stopped) {
+            ctl[RC_STOP] = 1;
+            ctl[RC_STOP_ROUND] = ctl[RC_ROUND];
+        }
+        if (step == 0) ctl[RC_T2_RAN] = any;
+        if (step == 1 && halt) ctl[RC_T2_RAN] = 0;
+        if (step <= 3) ctl[RC_T2_ANY] = halt ? 0u : any;
+    }
+}
+
+// End of the repair: a stopped round publishes nothing (its graphs leave the
+// round); otherwise the round index advances.
 __global__ __launch_bounds__(CTL_THREADS) void k_round_end(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
                                                            uint32_t ng, uint32_t* __restrict__ fl) {
-    uint32_t bad = 0;
-    for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x)
-        if (*flag(fl, ng, F_ROUND, g) && *flag(fl, ng, F_T2, g) && gp(graphs[g].ctr)[C_T2BAD] != 0) bad = 1;
-    bad = block_any(bad);
     const uint32_t stopped = ctl[RC_STOP];
     const uint32_t r = ctl[RC_ROUND];
     __syncthreads();
-    const bool stop = stopped || bad;
-    if (stop)
+    if (stopped)
         for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) *flag(fl, ng, F_ROUND, g) = 0;
-    if (threadIdx.x == 0) {
-        if (bad && !stopped) {
-            ctl[RC_STOP] = 1;
-            ctl[RC_STOP_ROUND] = r;
-        }
-        if (!stop && r < ctl[RC_ROUNDS]) ctl[RC_ROUND] = r + 1;
-    }
+    if (threadIdx.x == 0 && !stopped && r < ctl[RC_ROUNDS]) ctl[RC_ROUND] = r + 1;
 }
 
 // Keep a copy of the snapshot just written (the graph's history, if any).
@@ -226,6 +276,11 @@ hipError_t launch_t2_prep(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, 
 }
 hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl) {
     k_round_end<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl);
+    return hipGetLastError();
+}
+hipError_t launch_t2_control(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
+                             const RoundGraph* rg, int step, uint32_t pass) {
+    k_t2_control<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, rg, step, pass);
     return hipGetLastError();
 }
 hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, uint32_t max_n, const uint32_t* fl_round,

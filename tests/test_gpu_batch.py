@@ -55,10 +55,10 @@ def _oracle(g, cfg, churn, planted=None):
     return orc
 
 
-def _pool(graph_path, deterministic, overflow=False, batch_capacity=8192):
+def _pool(graph_path, deterministic, overflow=False, batch_capacity=8192, host_tier2=False):
     graphs = {k: S.load_graph_csr(graph_path(n)) for k, n in enumerate(NAMES)}
     pool = S.GraphPool(max_graphs=16, batch_capacity=batch_capacity, deterministic=deterministic,
-                       repair_every_epoch=True, exact_histogram=True)
+                       repair_every_epoch=True, exact_histogram=True, host_tier2=host_tier2)
     gid = {k: pool.add(g, _config(k, overflow)) for k, g in graphs.items()}
     init = {gid[k]: S.tag_graph(S.generate_init_stream(g), gid[k]) for k, g in graphs.items()}
     pool.submit(_mixed(init, 1), max_stats=0)
@@ -77,11 +77,15 @@ def _rounds(k, overflow):
     return -(-_churn(k, overflow)[0] // EPOCH[k])
 
 
-@pytest.mark.parametrize("use_graph,t1_sweeps,overflow", [(True, 0, False), (True, 1, False), (True, 0, True),
-                                                          (True, 1, True), (False, 0, True)])
-def test_deterministic_batch_equals_the_oracle_per_graph(use_graph, t1_sweeps, overflow, graph_path):
-    pool, graphs, gid, churn, mixed = _pool(graph_path, True, overflow)
-    rounds = pool.batch_prepare(mixed, use_graph=use_graph, t1_sweeps=t1_sweeps)
+@pytest.mark.parametrize("use_graph,t1_sweeps,overflow,t2", [
+    (True, 0, False, "device"), (True, 1, False, "device"), (True, 0, True, "device"), (True, 1, True, "device"),
+    (True, 0, True, "device-1-pass"), (True, 0, True, "host"), (False, 0, True, "device"), (False, 0, True, "host")])
+def test_deterministic_batch_equals_the_oracle_per_graph(use_graph, t1_sweeps, overflow, t2, graph_path):
+    # Tier-2 inside the captured rounds, with one pass captured (the host
+    # finishes a graph that needs more), or on the host
+    pool, graphs, gid, churn, mixed = _pool(graph_path, True, overflow, host_tier2=(t2 == "host"))
+    rounds = pool.batch_prepare(mixed, use_graph=use_graph, t1_sweeps=t1_sweeps,
+                                t2_passes=1 if t2 == "device-1-pass" else 0)
     assert rounds == max(_rounds(k, overflow) for k in graphs)
     st = pool.batch_run()
     assert st["rounds"] == rounds and st["updates"] == len(mixed)
