@@ -118,8 +118,8 @@ typedef struct sgc_tuning {
     uint32_t sweep_blocks;  /* grid cap of worklist sweeps, in 256-thread blocks (default 2048) */
     uint32_t apply_blocks;  /* grid cap of the per-update launch and of full sweeps (default 16384) */
     uint32_t serial_repair; /* a graph of at most this many vertices whose last Tier-1 sweep found at most
-                               serial_conflicts conflicts sweeps next in vertex order on one wavefront, in
-                               parallel mode too (default 65536; deterministic mode: every sweep) */
+                               serial_conflicts conflicts, after two parallel sweeps, sweeps next in vertex
+                               order on one wavefront (default 65536; deterministic mode: every sweep) */
     uint32_t serial_conflicts; /* default 64 */
     uint32_t fresh_repair;  /* parallel-mode Tier-1 selects from the exact colours of the loser's neighbours
                                gathered by its conflict scan: 0 never (the reference's bitmap/histogram),
@@ -317,7 +317,8 @@ int sgc_pool_set_epoch_size(sgc_pool* p, uint32_t gid, uint32_t epoch_size);
  * while a run is in progress. */
 typedef struct sgc_batch_config {
     uint32_t use_graph;        /* 1: replay a captured hipGraph per round; 0: host-driven launches */
-    uint32_t t1_sweeps;        /* Tier-1 sweeps captured per round (0: the graphs' max_repair_iters) */
+    uint32_t t1_sweeps;        /* Tier-1 sweeps captured per round (0: 4, at most max_repair_iters); a graph
+                                  still repairing after them makes the host finish the round */
     uint32_t autotune_rounds;  /* > 0: rounds given to each launch-shape candidate (hipGraph mode) */
     uint32_t t2_passes;        /* device Tier-2 passes captured per round (0: 4); a graph needing more, or a
                                   Tier-2 on the host (SGC_HOST_TIER2), makes the host finish the round */

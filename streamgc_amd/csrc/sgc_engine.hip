@@ -735,15 +735,16 @@ int sgc_pool::tier1(const std::vector<uint32_t>& gids, uint32_t start_it) {
         LaunchCtx L = ctx(true);
         if (e == hipSuccess) e = launch_zero_ctr(L, C_CONFLICTS);
         // A graph sweeps in vertex order on one wavefront (Gauss-Seidel) when
-        // deterministic, or when it is small and its last sweep left few
-        // conflicts (a racy sweep over a small conflict set can chase the same
-        // conflicts for many iterations); otherwise one wavefront per listed row.
-        // Graphs of different selection levels take separate launches.
+        // deterministic, or when it is small and two parallel sweeps have left
+        // a few conflicts (a racy sweep over a small conflict set can chase the
+        // same conflicts for many iterations); otherwise one wavefront per
+        // listed row.  Graphs of different selection levels take separate launches.
         std::vector<uint32_t> part[2][3];   // [serial][fresh level]
         for (uint32_t g : gids) {
             const GraphState& G = gs[g];
             if (!G.iterating) continue;
-            const bool serial = deterministic() || (G.n <= tune.serial_repair && G.prev <= tune.serial_conflicts);
+            const bool serial = deterministic() || (G.n <= tune.serial_repair && G.prev <= tune.serial_conflicts &&
+                                                    G.st.t1_iterations >= 2);
             part[serial][fresh_level(g)].push_back(g);
         }
         for (int serial = 0; serial < 2 && e == hipSuccess; serial++)
@@ -1194,7 +1195,7 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
         B->any_history |= gs[g].history != nullptr;
         B->all_settled &= gs[g].settled;
     }
-    B->sweeps = c.t1_sweeps ? std::min(c.t1_sweeps, max_iters) : max_iters;
+    B->sweeps = std::min(c.t1_sweeps ? c.t1_sweeps : 4u, max_iters);
     // Tier-2 passes captured per round (host placement or no dirty tracking: none, the host repairs)
     uint32_t max_passes = 0;
     for (uint32_t g = 0; g < ng; g++)

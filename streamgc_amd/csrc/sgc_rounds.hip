@@ -83,8 +83,8 @@ __global__ __launch_bounds__(CTL_THREADS) void k_round_begin(uint32_t* __restric
 // Between Tier-1 sweeps.  after: fold the last sweep's conflict count into
 // each iterating graph's state (stop on zero, cap or plateau).  before: pick
 // the next sweep of each graph still iterating -- serial (vertex order, one
-// wavefront) when deterministic or when the graph is small and its last sweep
-// left few conflicts, else parallel -- and clear its conflict counter.
+// wavefront) when deterministic or when the graph is small and two parallel
+// sweeps have left few conflicts, else parallel -- and clear its counter.
 __global__ __launch_bounds__(CTL_THREADS) void k_t1_control(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
                                                             uint32_t ng, uint32_t* __restrict__ fl,
                                                             RoundState* __restrict__ st,
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t1_control(uint32_t* __restrict
             *flag(fl, ng, F_ITER, g) = it;
         }
         const bool run = before && it;
-        const bool serial = deterministic || (rg[g].n <= serial_repair && s.prev <= serial_conflicts);
+        const bool serial = deterministic || (rg[g].n <= serial_repair && s.prev <= serial_conflicts && s.iters >= 2);
         *flag(fl, ng, F_PAR, g) = run && !serial;
         *flag(fl, ng, F_SER, g) = run && serial;
         if (run) c[C_CONFLICTS] = 0;
