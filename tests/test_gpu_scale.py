@@ -2,9 +2,10 @@
 
 C1 (Delaunay 2^17, 100,000 churn updates): the deterministic replay is
 bit-exact with the oracle -- every word, slot and epoch statistic -- and its
-colour count is the reference's.  C2 (Chung-Lu com-LiveJournal shape): the
-parallel engine publishes proper snapshots and its colour count stays close
-to the sequential replay's (exact histograms).
+colour count is the reference's.  C2 (Chung-Lu com-LiveJournal shape) and C3
+(R-MAT scale 20, dense growth inside the top k-cores): the parallel engine
+publishes proper snapshots and its colour count stays close to the
+sequential replay's (exact histograms).
 """
 import json
 import os
@@ -75,3 +76,36 @@ def test_c2_parallel_colouring_is_proper_and_close_to_the_replay():
     final = epochs[-1]["max_color"]
     assert final <= ref["final_max_color"] * 1.06, (final, ref["final_max_color"])
     assert final < REF["c2/reference"]["final_max_color"]
+
+
+def _parallel_run(g, stream, ref, tol):
+    gpu = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True))
+    init = S.generate_init_stream(g)
+    ds = S.DeviceStream(init)
+    for k in range(0, len(init), E):           # launches of 256, epochs every E (as bench.py)
+        for j in range(k, min(k + E, len(init)), 256):
+            gpu.apply_device(ds.at(j), min(256, len(init) - j))
+        if k + E <= len(init):
+            gpu.epoch()
+    ds.close()
+    assert gpu.finish()["t2_remaining"] == 0
+    gpu.set_mode(repair_every_epoch=True)
+    epochs_n = len(stream) // E
+    gpu.history_reserve(epochs_n)
+    epochs = gpu.process_stream(stream[:epochs_n * E])
+    assert len(epochs) == epochs_n and all(e["t2_remaining"] == 0 for e in epochs)
+    bad = gpu.validate_history(S.edge_intervals(g, stream), [E * (k + 1) for k in range(epochs_n)])
+    assert bad.tolist() == [0] * epochs_n
+    final = epochs[-1]["max_color"]
+    assert final <= ref * tol, (final, ref)
+    return final
+
+
+def test_c3_dense_growth_is_proper_and_within_two_percent_of_the_replay():
+    g = S.gen_rmat(20, 48.0, seed=42)
+    ref = REF["c3/exact"]
+    assert (g.n, g.num_edges // 2) == (ref["vertices"], ref["edges"])
+    stream = S.dense_growth_stream(g, ref["stream"], core_pct=90, ins_per_del=2, seed=42)
+    # the replay's last publish follows the 31st epoch: compare the 30 full epochs' last colour count
+    final = _parallel_run(g, stream, ref["final_max_color"], 1.02)
+    assert final < REF["c3/reference"]["final_max_color"]
