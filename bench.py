@@ -63,9 +63,19 @@ def make_graph(S, w):
     return S.gen_delaunay(w["points"], seed=42)
 
 
-def make_stream(S, w, g, size):
-    if w["kind"] == "rmat":   # dense growth inside the dense core
+def make_stream(S, w, g, size, kind=None, held=None):
+    """The workload's stream, or one of the README's other stream types."""
+    kind = kind or ("dense_growth" if w["kind"] == "rmat" else "churn")
+    if kind == "dense_growth":   # inside the 90th-percentile k-core, two inserts per delete
         return S.dense_growth_stream(g, size, core_pct=90, ins_per_del=2, seed=42)
+    if kind == "dense":
+        return S.dense_stream(g, size, core_pct=90, seed=42)
+    if kind == "uniform":
+        return S.uniform_stream(g, size, seed=42)
+    if kind == "holdout":
+        if len(held) < size:
+            raise SystemExit(f"the hold-out stream has {len(held)} inserts, fewer than {size}")
+        return held[:size]
     return S.generate_bench_stream_edges(g, size, 50, 42)
 
 
@@ -154,6 +164,10 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="served workload: requests per phase")
     ap.add_argument("--load", type=float, default=0.5, help="served workload: offered load / served capacity")
     ap.add_argument("--max-delay-us", type=int, default=2000, help="served workload: publish deadline")
+    ap.add_argument("--stream-type", default=None,
+                    choices=["churn", "uniform", "holdout", "dense", "dense_growth"],
+                    help="update stream (default: churn; dense_growth for c3); holdout removes 10%% of the edges "
+                         "from the loaded graph and streams them back as inserts")
     ap.add_argument("--autotune-rounds", type=int, default=1,
                     help="batched workload: rounds each launch-shape candidate is timed on (0: no tuning)")
     ap.add_argument("--graphs", type=int, default=256, help="batched workload: graphs in the pool")
@@ -180,10 +194,13 @@ def main():
     K, W = args.steps, args.warmup
     t = time.perf_counter()
     g = make_graph(S, w)
+    held = None
+    if args.stream_type == "holdout":
+        g, held = S.holdout_stream(g, 10, 42)
     log(f"graph {g.n} vertices {g.num_edges // 2} edges max deg {int(g.degree.max())} ({time.perf_counter() - t:.1f}s)")
     t = time.perf_counter()
     init = S.generate_init_stream(g)
-    bench = make_stream(S, w, g, (W + K) * E)
+    bench = make_stream(S, w, g, (W + K) * E, args.stream_type, held)
     log(f"streams: init {len(init)}, churn {len(bench)} ({time.perf_counter() - t:.1f}s)")
 
     eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True,
@@ -278,7 +295,8 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "updates/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(elapsed * 1e3 / K, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": w["name"], "vertices": g.n, "edges": g.num_edges // 2, "epoch_size": E,
+        "config": {"workload": w["name"] + (f"; {args.stream_type} stream instead" if args.stream_type else ""),
+                   "vertices": g.n, "edges": g.num_edges // 2, "epoch_size": E,
                    "updates_timed": K * E,
                    "init": f"init stream ({len(init)} inserts, untimed) applied in launches of "
                            f"{min(args.init_batch or E, E)} updates, reference-order epochs every {E}, then repaired",

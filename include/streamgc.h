@@ -57,7 +57,17 @@ typedef struct sgc_edge_update {
 #define SGC_EXACT_HISTOGRAM (1u << 2)    /* hub histograms recounted wherever bitmaps are rebuilt */
 #define SGC_FULL_SWEEPS (1u << 3)        /* maintenance sweeps visit every vertex (no dirty tracking); same results */
 #define SGC_HOST_TIER2 (1u << 4)         /* Tier-2 repair on the host (the reference's placement); same results */
-#define SGC_ALL_FLAGS 0x1Fu
+/* The reference's compile-time ablations (Makefile:58-78), here runtime flags: */
+#define SGC_ABL_FORCE_BITMAP (1u << 5)    /* STREAMGC_ABLATION_FORCE_BITMAP: every vertex selects from its bitmap
+                                             (A/include/color_select.cuh:98-100) */
+#define SGC_ABL_FORCE_HISTOGRAM (1u << 6) /* STREAMGC_ABLATION_FORCE_HISTOGRAM: the default dispatch, as in the
+                                             reference (:101-107) */
+#define SGC_ABL_NO_OVERFLOW (1u << 7)     /* STREAMGC_ABLATION_NO_OVERFLOW: a full row drops the edge, nothing is
+                                             buffered or counted (A/src/kernels/local_update.cu:28-38) */
+#define SGC_ABL_LIVE_DEGREE (1u << 8)     /* STREAMGC_ABLATION_LIVE_DEG_PRIORITY: Tier-1 takes a vertex's own
+                                             priority from its live row count, its neighbours' from deg0
+                                             (A/src/kernels/conflict_repair.cu:46-58) */
+#define SGC_ALL_FLAGS 0x1FFu
 
 /* Engine configuration; the defaults (sgc_default_config) are the reference's. */
 typedef struct sgc_config {
@@ -450,6 +460,18 @@ int sgc_core_numbers(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_id
 int sgc_dense_growth_stream(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const uint32_t* core,
                             uint32_t core_min, uint64_t size, uint32_t ins_per_del, uint64_t seed,
                             sgc_edge_update* out);
+
+/* The other update streams of the reference's README.md:83-91 (no code in the
+ * reference; specified by that table): "Uniform" inserts `size` uniform
+ * non-adjacent vertex pairs, each new; "Hold-out" removes round(m * pct / 100)
+ * uniformly drawn edges from the graph (*base_out, same n) and returns them
+ * as inserts in random order (two calls: with out = base_out = NULL it only
+ * sets *count).  The "Dense" stream is sgc_dense_growth_stream with
+ * ins_per_del = 1. */
+int sgc_uniform_stream(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, uint64_t size, uint64_t seed,
+                       sgc_edge_update* out);
+int sgc_holdout_stream(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, uint32_t pct, uint64_t seed,
+                       sgc_csr** base_out, sgc_edge_update* out, uint64_t* count);
 
 /* = generate_init_stream (A/src/host/stream_gen.cpp:30-61): row_ptr[n]/2 records. */
 int sgc_init_stream(uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx, const uint32_t* deg,

@@ -117,6 +117,10 @@ struct Config {              // same field order as sgc_config in include/stream
 };
 constexpr uint32_t FLAG_REPAIR_EVERY_EPOCH = 1u << 1;
 constexpr uint32_t FLAG_EXACT_HISTOGRAM = 1u << 2;
+// The reference's compile-time ablations (Makefile:58-78) as flags.
+constexpr uint32_t FLAG_ABL_FORCE_BITMAP = 1u << 5;     // A/include/color_select.cuh:98-100
+constexpr uint32_t FLAG_ABL_NO_OVERFLOW = 1u << 7;      // A/src/kernels/local_update.cu:28-38
+constexpr uint32_t FLAG_ABL_LIVE_DEGREE = 1u << 8;      // A/src/kernels/conflict_repair.cu:46-58
 
 struct Stats {               // same layout as sgc_epoch_stats in include/streamgc.h
     uint32_t epoch_index;
@@ -179,7 +183,10 @@ struct Engine {
         return (uint16_t)(mi + 1);
     }
     // A/include/color_select.cuh:93-115
-    uint16_t fas(uint32_t v) const { return hd_map[v] != INVALID ? fas_hist(v) : fas_bitmap(v); }
+    uint16_t fas(uint32_t v) const {
+        if (cfg.flags & FLAG_ABL_FORCE_BITMAP) return fas_bitmap(v);
+        return hd_map[v] != INVALID ? fas_hist(v) : fas_bitmap(v);
+    }
 
     // A/include/color_select.cuh:160-183
     void on_new_neighbor(uint32_t v, uint16_t c) {
@@ -209,6 +216,7 @@ struct Engine {
             if (col[j] == INVALID) { col[j] = nb; return; }
             if (col[j] == nb) return;
         }
+        if (cfg.flags & FLAG_ABL_NO_OVERFLOW) return;   // the ablation drops the edge
         uint32_t pos = overflow_count++;
         if (pos < cfg.overflow_capacity) overflow[pos] = {v, nb};
     }
@@ -347,6 +355,11 @@ struct Engine {
                 continue;
             }
             uint64_t my_pi = pi(v);
+            if (cfg.flags & FLAG_ABL_LIVE_DEGREE) {   // own priority from the live row, neighbours' from deg0
+                uint64_t live = 0;
+                for (uint32_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) live += col[j] != INVALID;
+                my_pi = (live << 32) | v;
+            }
             bool need = false;
             for (uint32_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) {
                 uint32_t nb = col[j];

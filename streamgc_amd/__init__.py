@@ -19,7 +19,8 @@ from ._lib import EDGE_UPDATE, SgcError, check, lib
 
 __all__ = ["StreamGCConfig", "default_config", "CSRGraph", "GPUPartition", "DeviceStream", "EDGE_UPDATE",
            "load_graph_csr", "csr_from_edges", "csr_from_edge_arrays", "gen_chung_lu", "gen_rmat",
-           "gen_delaunay", "core_numbers", "dense_growth_stream", "generate_init_stream", "generate_bench_stream_edges",
+           "gen_delaunay", "core_numbers", "dense_growth_stream", "dense_stream", "uniform_stream", "holdout_stream",
+           "generate_init_stream", "generate_bench_stream_edges",
            "static_greedy_coloring", "check_coloring_valid", "edge_intervals", "make_updates", "SgcError",
            "SgcNoMemory", "device_count", "GraphPool", "GraphServer", "tag_graph", "write_coloring",
            "write_coloring_meta", "read_coloring"]
@@ -47,6 +48,8 @@ class StreamGCConfig:
     exact_histogram: bool = False
     full_sweeps: bool = False
     host_tier2: bool = False       # Tier-2 on the host, as the reference places it (same results)
+    ablations: tuple = ()          # the reference's ablations: "force_bitmap", "force_histogram",
+                                   # "no_overflow", "live_degree"
 
     def to_c(self):
         flags = (_lib.SGC_DETERMINISTIC if self.deterministic else 0) | \
@@ -54,6 +57,8 @@ class StreamGCConfig:
                 (_lib.SGC_EXACT_HISTOGRAM if self.exact_histogram else 0) | \
                 (_lib.SGC_FULL_SWEEPS if self.full_sweeps else 0) | \
                 (_lib.SGC_HOST_TIER2 if self.host_tier2 else 0)
+        for a in self.ablations:
+            flags |= _lib.ABLATIONS[a]
         return _lib.Config(self.epoch_size, self.batch_size, self.histogram_threshold, self.slack_pct,
                            self.min_row_capacity, self.overflow_capacity, self.max_repair_iters,
                            self.plateau_iters, self.tier2_passes, flags)
@@ -189,6 +194,29 @@ def dense_growth_stream(g, size, core_pct=90, ins_per_del=2, seed=42, core=None)
     check(lib().sgc_dense_growth_stream(g.n, g.row_ptr, g.col_idx, np.ascontiguousarray(core, np.uint32),
                                         core_min, size, ins_per_del, seed, out.ctypes.data))
     return out[:size]
+
+
+def dense_stream(g, size, core_pct=90, seed=42, core=None):
+    """The README's dense stream: dense growth with one insert per delete."""
+    return dense_growth_stream(g, size, core_pct=core_pct, ins_per_del=1, seed=seed, core=core)
+
+
+def uniform_stream(g, size, seed=42):
+    """The README's uniform stream: inserts of uniform non-adjacent vertex pairs, each new."""
+    out = np.zeros(max(size, 1), dtype=EDGE_UPDATE)
+    check(lib().sgc_uniform_stream(g.n, g.row_ptr, g.col_idx, size, seed, out.ctypes.data))
+    return out[:size]
+
+
+def holdout_stream(g, pct=10, seed=42):
+    """The README's hold-out stream: (base graph without pct% of the edges, those edges as inserts)."""
+    cnt = C.c_uint64()
+    check(lib().sgc_holdout_stream(g.n, g.row_ptr, g.col_idx, pct, seed, None, None, C.byref(cnt)))
+    out = np.zeros(max(cnt.value, 1), dtype=EDGE_UPDATE)
+    h = C.c_void_p()
+    check(lib().sgc_holdout_stream(g.n, g.row_ptr, g.col_idx, pct, seed, C.byref(h), out.ctypes.data,
+                                   C.byref(cnt)))
+    return _from_handle(h), out[:cnt.value]
 
 
 def make_updates(u, v, t=None, graph=0):

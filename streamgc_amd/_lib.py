@@ -18,6 +18,12 @@ SGC_REPAIR_EVERY_EPOCH = 1 << 1
 SGC_EXACT_HISTOGRAM = 1 << 2
 SGC_FULL_SWEEPS = 1 << 3
 SGC_HOST_TIER2 = 1 << 4
+SGC_ABL_FORCE_BITMAP = 1 << 5
+SGC_ABL_FORCE_HISTOGRAM = 1 << 6
+SGC_ABL_NO_OVERFLOW = 1 << 7
+SGC_ABL_LIVE_DEGREE = 1 << 8
+ABLATIONS = {"force_bitmap": SGC_ABL_FORCE_BITMAP, "force_histogram": SGC_ABL_FORCE_HISTOGRAM,
+             "no_overflow": SGC_ABL_NO_OVERFLOW, "live_degree": SGC_ABL_LIVE_DEGREE}
 
 # numpy view of the 16-byte update record (A/include/types.h:16-21)
 EDGE_UPDATE = np.dtype([("u", "<u4"), ("v", "<u4"), ("type", "u1"), ("pad", "u1", 3), ("graph", "<u4")])
@@ -231,6 +237,9 @@ SIGNATURES = {
     "sgc_core_numbers": [C.c_uint32, _u32p, _u32p, _u32p],
     "sgc_dense_growth_stream": [C.c_uint32, _u32p, _u32p, _u32p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64,
                                 _vp],
+    "sgc_uniform_stream": [C.c_uint32, _u32p, _u32p, C.c_uint64, C.c_uint64, _vp],
+    "sgc_holdout_stream": [C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint64, C.POINTER(_vp), _vp,
+                           C.POINTER(C.c_uint64)],
     "sgc_init_stream": [C.c_uint32, _u32p, _u32p, _u32p, _vp, C.POINTER(C.c_uint64)],
     "sgc_churn_stream": [C.c_uint32, _u32p, _u32p, C.c_uint64, C.c_uint32, C.c_uint64, _vp],
     "sgc_static_greedy": [C.c_uint32, _u32p, _u32p, C.POINTER(C.c_uint32)],

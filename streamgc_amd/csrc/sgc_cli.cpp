@@ -9,7 +9,9 @@
 //            bench time = stream + that repair.
 //   --validate checks the live colouring against the rows; --output-dir
 //   writes .coloring, .coloring.txt and .coloring.meta.
-// Extra flags: --repair-every-epoch, --exact-histogram, --deterministic.
+// Extra flags: --repair-every-epoch, --exact-histogram, --deterministic, and
+// --ablation force-bitmap|force-histogram|no-overflow|live-deg (the
+// reference builds each ablation as its own binary).
 // Unknown options are ignored, as the reference ignores them; --log-dir,
 // --profile-hw and --init-only are accepted and, as there, change nothing.
 // One GPU: --num-gpus other than 1 is an error.
@@ -85,6 +87,17 @@ bool parse(int argc, char** argv, Args& a) {
         else if (!strcmp(s, "--repair-every-epoch")) a.flags |= SGC_REPAIR_EVERY_EPOCH;
         else if (!strcmp(s, "--exact-histogram")) a.flags |= SGC_EXACT_HISTOGRAM;
         else if (!strcmp(s, "--deterministic")) a.flags |= SGC_DETERMINISTIC;
+        else if (!strcmp(s, "--ablation") && val) {   // the reference's abl-* builds (Makefile:58-78)
+            const char* k = argv[++i];
+            if (!strcmp(k, "force-bitmap")) a.flags |= SGC_ABL_FORCE_BITMAP;
+            else if (!strcmp(k, "force-histogram")) a.flags |= SGC_ABL_FORCE_HISTOGRAM;
+            else if (!strcmp(k, "no-overflow")) a.flags |= SGC_ABL_NO_OVERFLOW;
+            else if (!strcmp(k, "live-deg")) a.flags |= SGC_ABL_LIVE_DEGREE;
+            else {
+                fprintf(stderr, "unknown ablation: %s\n", k);
+                return false;
+            }
+        }
     }
     return a.graph != nullptr;
 }

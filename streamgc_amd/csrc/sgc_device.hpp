@@ -28,6 +28,8 @@ constexpr uint32_t WAVE = 64;
 // long row costs ROW_UNROLL times fewer memory round trips.
 constexpr uint32_t ROW_UNROLL = 4;
 constexpr int MAX_CAS_RETRIES = 64;          // recolour CAS retries before leaving it to repair
+// Ablation flags carried by the descriptors (the values of SGC_ABL_* in streamgc.h).
+constexpr uint32_t ABL_FORCE_BITMAP = 1u << 5, ABL_NO_OVERFLOW = 1u << 7, ABL_LIVE_DEGREE = 1u << 8;
 
 // Per-graph u32 counters (device).
 enum : uint32_t {
@@ -92,6 +94,7 @@ struct GraphDev {
     uint32_t ovf_cap;
     uint32_t nhd;
     uint32_t dstride;         // n rounded up to a multiple of 4
+    uint32_t flags;           // SGC_ABL_* ablation flags of the pool
 };
 
 // ---------------------------------------------------------------- wave helpers
@@ -261,9 +264,13 @@ __device__ __forceinline__ uint16_t fas_hist(const GraphDev& g, uint32_t hd) {
 }
 
 // A/include/color_select.cuh:93-115
+// Histogram selection for a hub unless every vertex is forced to its bitmap
+// (ablation, A/include/color_select.cuh:98-100).
+__device__ __forceinline__ bool select_by_hist(const GraphDev& g, uint32_t x) {
+    return gp(g.hd_map)[x] != INVALID && !(g.flags & ABL_FORCE_BITMAP);
+}
 __device__ __forceinline__ uint16_t find_color(const GraphDev& g, uint32_t x) {
-    uint32_t hd = gp(g.hd_map)[x];
-    return hd != INVALID ? fas_hist(g, hd) : fas_bitmap(g, x);
+    return select_by_hist(g, x) ? fas_hist(g, gp(g.hd_map)[x]) : fas_bitmap(g, x);
 }
 
 // A/include/color_select.cuh:160-183 (lane 0 only: one neighbour)
@@ -351,11 +358,13 @@ __device__ __forceinline__ uint32_t row_insert(const GraphDev& g, uint32_t x, ui
     }
     if (use_overflow) {
         if (lane() == 0) {
-            uint32_t pos = add32(&g.ctr[C_OVF], 1u);
             add32(&g.ctr[C_ASYM], 1u);
-            if (pos < g.ovf_cap) {
-                st(&g.ovf[2 * (size_t)pos], x);
-                st(&g.ovf[2 * (size_t)pos + 1], nb);
+            if (!(g.flags & ABL_NO_OVERFLOW)) {   // the ablation drops the edge unbuffered and uncounted
+                uint32_t pos = add32(&g.ctr[C_OVF], 1u);
+                if (pos < g.ovf_cap) {
+                    st(&g.ovf[2 * (size_t)pos], x);
+                    st(&g.ovf[2 * (size_t)pos + 1], nb);
+                }
             }
             // the two rows of this edge may now disagree: always maintain both
             mark_sticky(g, x);

@@ -59,6 +59,7 @@ struct PendingTiming {
 };
 
 uint32_t mark_stride(uint32_t n) { return (n + 3u) & ~3u; }
+constexpr uint32_t ABL_MASK = SGC_ABL_FORCE_BITMAP | SGC_ABL_FORCE_HISTOGRAM | SGC_ABL_NO_OVERFLOW | SGC_ABL_LIVE_DEGREE;
 
 // Bytes one graph occupies on the device (admission control).
 uint64_t graph_bytes(uint32_t n, uint64_t S, uint32_t nhd, uint32_t ovf_cap) {
@@ -421,6 +422,15 @@ struct sgc_pool {
     }
 
     hipError_t upload_descriptor(uint32_t gid) { return graphs.upload(stream, gid, 1); }
+    // The ablation flags travel in every descriptor.
+    hipError_t refresh_flags() {
+        hipError_t e = graphs.writable();
+        for (uint32_t g = 0; g < ng && e == hipSuccess; g++)
+            if (gs[g].used) graphs.h[g].flags = flags & ABL_MASK;
+        if (e == hipSuccess && ng) e = graphs.upload(stream, 0, ng);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        return e;
+    }
 
     // ------------------------------------------------------------ graphs
     int add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
@@ -573,6 +583,7 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     d.n = n;
     d.ovf_cap = cfg->overflow_capacity;
     d.nhd = nhd;
+    d.flags = flags & ABL_MASK;
     G.used = true;
     used_bytes += bytes;
     e = upload_descriptor(gid);
@@ -1878,10 +1889,9 @@ int sgc_set_flags(sgc_engine* e, uint32_t flags) {
     if (flags != e->flags) e->batch_drop();
     e->flags = flags;
     e->gs[0].cfg.flags = flags;
-    if (e->tracking() != was) {
-        hipError_t r = e->set_tracking(e->tracking());
-        if (r != hipSuccess) return e->hip_fail(r, "set flags");
-    }
+    hipError_t r = e->refresh_flags();
+    if (r == hipSuccess && e->tracking() != was) r = e->set_tracking(e->tracking());
+    if (r != hipSuccess) return e->hip_fail(r, "set flags");
     return SGC_OK;
 }
 
@@ -2245,10 +2255,9 @@ int sgc_pool_set_flags(sgc_pool* p, uint32_t flags) {
     if (flags != p->flags) p->batch_drop();   // captured rounds follow the flags they were made with
     p->flags = flags;
     for (auto& G : p->gs) G.cfg.flags = flags;
-    if (p->tracking() != was) {
-        hipError_t r = p->set_tracking(p->tracking());
-        if (r != hipSuccess) return p->hip_fail(r, "set flags");
-    }
+    hipError_t r = p->refresh_flags();
+    if (r == hipSuccess && p->tracking() != was) r = p->set_tracking(p->tracking());
+    if (r != hipSuccess) return p->hip_fail(r, "set flags");
     return SGC_OK;
 }
 int sgc_pool_apply(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, int on_device) {

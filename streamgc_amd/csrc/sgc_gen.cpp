@@ -342,21 +342,17 @@ int sgc_core_numbers(uint32_t n, const uint32_t* rp, const uint32_t* col, uint32
     return SGC_OK;
 }
 
-// Dense-growth stream over the dense region H = {v : core(v) >= core_min}:
-// each event inserts with probability ins/(ins+1) (a uniform non-adjacent
-// pair in H, redrawn until new) or deletes a uniform live edge inside H
-// (swap-remove).  mt19937-free: counter-based draws of (seed, event).
-int sgc_dense_growth_stream(uint32_t n, const uint32_t* rp, const uint32_t* col, const uint32_t* core,
-                            uint32_t core_min, uint64_t size, uint32_t ins_per_del, uint64_t seed,
-                            sgc_edge_update* out) {
-    if (!rp || !col || !core || (!out && size)) return SGC_ERR_INVALID;
+// Random-pair streams over a region H of the vertices: each event inserts
+// with probability ins/(ins+1) -- always, for an insert-only stream -- a
+// uniform non-adjacent pair in H, redrawn until new, or deletes a uniform
+// live edge inside H (swap-remove).  mt19937-free: counter-based draws of
+// (seed, stream id, event).
+static int region_stream(uint32_t n, const uint32_t* rp, const uint32_t* col, const std::vector<uint8_t>& inH,
+                         uint64_t size, uint32_t ins_per_del, bool inserts_only, uint64_t seed, uint64_t sid,
+                         sgc_edge_update* out) {
     std::vector<uint32_t> H;
-    std::vector<uint8_t> inH(n, 0);
     for (uint32_t v = 0; v < n; v++)
-        if (core[v] >= core_min) {
-            H.push_back(v);
-            inH[v] = 1;
-        }
+        if (inH[v]) H.push_back(v);
     if (H.size() < 2) return SGC_ERR_RANGE;
     std::vector<uint64_t> live_list;
     // all live edges (membership), and the deletable list inside H
@@ -365,7 +361,7 @@ int sgc_dense_growth_stream(uint32_t n, const uint32_t* rp, const uint32_t* col,
         for (uint32_t j = rp[u]; j < rp[u + 1]; j++)
             if (u < col[j]) {
                 const uint64_t k = ((uint64_t)u << 32) | col[j];
-                if (inH[u] && inH[col[j]]) live_list.push_back(k);
+                if (!inserts_only && inH[u] && inH[col[j]]) live_list.push_back(k);
                 all_keys.push_back(k);
             }
     // open-addressing set with backward-shift delete
@@ -395,11 +391,11 @@ int sgc_dense_growth_stream(uint32_t n, const uint32_t* rp, const uint32_t* col,
         slot[hole] = ~0ULL;
     };
     uint64_t draw_k = 0;
-    auto next = [&]() { return draw(seed, 4, draw_k++); };
+    auto next = [&]() { return draw(seed, sid, draw_k++); };
     auto below = [&](uint64_t r) { return (uint64_t)(((unsigned __int128)next() * r) >> 64); };
     const uint64_t hs = H.size();
     for (uint64_t i = 0; i < size; i++) {
-        const bool ins = live_list.empty() || below(ins_per_del + 1) < ins_per_del;
+        const bool ins = inserts_only || live_list.empty() || below(ins_per_del + 1) < ins_per_del;
         std::memset(&out[i], 0, sizeof out[i]);
         if (ins) {
             for (int tries = 0;; tries++) {
@@ -412,7 +408,7 @@ int sgc_dense_growth_stream(uint32_t n, const uint32_t* rp, const uint32_t* col,
                     continue;
                 }
                 slot[s] = k;
-                live_list.push_back(k);
+                if (!inserts_only) live_list.push_back(k);
                 out[i].u = (uint32_t)(k >> 32);
                 out[i].v = (uint32_t)k;
                 out[i].type = 0;
@@ -428,6 +424,55 @@ int sgc_dense_growth_stream(uint32_t n, const uint32_t* rp, const uint32_t* col,
             out[i].v = (uint32_t)k;
             out[i].type = 1;
         }
+    }
+    return SGC_OK;
+}
+
+// Dense-growth stream over the dense region H = {v : core(v) >= core_min}.
+int sgc_dense_growth_stream(uint32_t n, const uint32_t* rp, const uint32_t* col, const uint32_t* core,
+                            uint32_t core_min, uint64_t size, uint32_t ins_per_del, uint64_t seed,
+                            sgc_edge_update* out) {
+    if (!rp || !col || !core || (!out && size)) return SGC_ERR_INVALID;
+    std::vector<uint8_t> inH(n, 0);
+    for (uint32_t v = 0; v < n; v++) inH[v] = core[v] >= core_min;
+    return region_stream(n, rp, col, inH, size, ins_per_del, false, seed, 4, out);
+}
+
+// Uniform stream (README.md "Uniform"): inserts of uniform non-adjacent
+// vertex pairs, each new.
+int sgc_uniform_stream(uint32_t n, const uint32_t* rp, const uint32_t* col, uint64_t size, uint64_t seed,
+                       sgc_edge_update* out) {
+    if (!rp || !col || (!out && size)) return SGC_ERR_INVALID;
+    return region_stream(n, rp, col, std::vector<uint8_t>(n, 1), size, 0, true, seed, 6, out);
+}
+
+// Hold-out stream (README.md "Hold-out"): round(m * pct / 100) of the m
+// edges, drawn uniformly (a partial Fisher-Yates shuffle of the edge list),
+// leave the base graph and come back as inserts in the drawn order.
+int sgc_holdout_stream(uint32_t n, const uint32_t* rp, const uint32_t* col, uint32_t pct, uint64_t seed,
+                       sgc_csr** base_out, sgc_edge_update* out, uint64_t* count) {
+    if (!rp || !col || !count || pct > 100) return SGC_ERR_INVALID;
+    std::vector<uint64_t> keys;
+    for (uint32_t u = 0; u < n; u++)
+        for (uint32_t j = rp[u]; j < rp[u + 1]; j++)
+            if (u < col[j] && col[j] < n) keys.push_back(((uint64_t)u << 32) | col[j]);
+    const uint64_t m = keys.size(), h = (m * pct + 50) / 100;
+    *count = h;
+    if (!out && !base_out) return SGC_OK;
+    for (uint64_t i = 0; i < h; i++) {
+        const uint64_t j = i + (uint64_t)(((unsigned __int128)draw(seed, 7, i) * (m - i)) >> 64);
+        std::swap(keys[i], keys[j]);
+    }
+    if (out)
+        for (uint64_t i = 0; i < h; i++) {
+            std::memset(&out[i], 0, sizeof out[i]);
+            out[i].u = (uint32_t)(keys[i] >> 32);
+            out[i].v = (uint32_t)keys[i];
+            out[i].type = 0;
+        }
+    if (base_out) {
+        std::vector<uint64_t> kept(keys.begin() + h, keys.end());
+        *base_out = csr_from_keys(n, kept);
     }
     return SGC_OK;
 }

@@ -729,11 +729,22 @@ __device__ __forceinline__ uint16_t fresh_pick(const GraphDev& g, uint32_t v, bo
 // reference's selection.  pres: this wavefront's PRESENCE_WORDS of LDS.
 // fresh: 0 = the reference's selection, 1 = exact colours for bitmap-mode
 // rows, 2 = for histogram-mode rows too.
+// Live slots of v's row (the ablation's priority degree,
+// A/src/kernels/conflict_repair.cu:46-58): one wavefront, or the whole block.
+template <bool kBlock>
+__device__ __forceinline__ uint32_t live_row_count(const GraphDev& g, uint32_t v) {
+    const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
+    const uint32_t width = kBlock ? BLOCK : WAVE, me = kBlock ? threadIdx.x : lane();
+    uint32_t k = 0;
+    for (uint32_t j = s + me; j < e; j += width) k += ld(&g.col[j]) != INVALID;
+    return kBlock ? block_sum32(k) : wave_sum32(k);
+}
+
 __device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v, uint32_t* pres, int fresh_level) {
     const uint32_t w = bcast0(ld(&g.word[v]));
     const uint16_t mine = color_of(w);
     const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
-    const bool hub = gp(g.hd_map)[v] != INVALID;
+    const bool hub = select_by_hist(g, v);
     const bool fresh = fresh_level >= (hub ? 2 : 1);
     ColourSet cs;
     if (fresh && hub) {
@@ -770,7 +781,7 @@ __device__ __forceinline__ uint32_t tier1_vertex(const GraphDev& g, uint32_t v, 
         mark_neighbors(g, v, 0, c);
         return 1;
     }
-    const uint32_t my_deg = gp(g.deg0)[v];
+    const uint32_t my_deg = (g.flags & ABL_LIVE_DEGREE) ? live_row_count<false>(g, v) : gp(g.deg0)[v];
     bool need = false;
     for (uint32_t base = s; base < e && (fresh || !need); base += ROW_UNROLL * WAVE) {
         uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
@@ -820,8 +831,8 @@ __device__ __forceinline__ uint32_t tier1_vertex_block(const GraphDev& g, uint32
     const uint32_t w = ld(&g.word[v]);
     const uint16_t mine = color_of(w);
     const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
-    const uint32_t my_deg = gp(g.deg0)[v];
-    const bool hub = gp(g.hd_map)[v] != INVALID;
+    const uint32_t my_deg = (g.flags & ABL_LIVE_DEGREE) ? live_row_count<true>(g, v) : gp(g.deg0)[v];
+    const bool hub = select_by_hist(g, v);
     const bool fresh = fresh_level >= (hub ? 2 : 1);   // see tier1_vertex
     ColourSet cs;
     uint32_t hit = 0;

@@ -37,6 +37,10 @@ class Stats(C.Structure):
 FLAG_DETERMINISTIC = 1 << 0
 FLAG_REPAIR_EVERY_EPOCH = 1 << 1
 FLAG_EXACT_HISTOGRAM = 1 << 2
+FLAG_ABL_FORCE_BITMAP = 1 << 5
+FLAG_ABL_FORCE_HISTOGRAM = 1 << 6
+FLAG_ABL_NO_OVERFLOW = 1 << 7
+FLAG_ABL_LIVE_DEGREE = 1 << 8
 
 
 def default_config(**kw):

@@ -27,7 +27,8 @@ def _pair(g, **cfg):
                           max_repair_iters=sc.max_repair_iters, plateau_iters=sc.plateau_iters,
                           tier2_passes=sc.tier2_passes,
                           flags=(O.FLAG_REPAIR_EVERY_EPOCH if sc.repair_every_epoch else 0)
-                          | (O.FLAG_EXACT_HISTOGRAM if sc.exact_histogram else 0))
+                          | (O.FLAG_EXACT_HISTOGRAM if sc.exact_histogram else 0)
+                          | sum(S._lib.ABLATIONS[a] for a in sc.ablations))
     og = O.Csr(g.n, g.row_ptr, g.col_idx, g.degree)
     return S.GPUPartition(g, sc), O.Engine(og, oc), og
 
@@ -391,3 +392,44 @@ def test_parallel_selection_modes_repair_to_proper(name, fresh, graph_path):
     orc.finish()
     orc.submit(bench["u"], bench["v"], bench["type"])
     assert gpu.validate()[2] <= orc.finish()["max_color"] * 1.25 + 2
+
+
+@pytest.mark.parametrize("kind", ["uniform", "holdout", "dense", "dense_growth"])
+def test_other_stream_types_bit_exact(kind, graph_path):
+    # the README's other stream types through the deterministic engine, against the oracle
+    g = S.load_graph_csr(graph_path("rmat_s10" if kind.startswith("dense") else "er2k"))
+    if kind == "holdout":
+        g, stream = S.holdout_stream(g, 10, 5)
+    elif kind == "uniform":
+        stream = S.uniform_stream(g, 3000, seed=5)
+    elif kind == "dense":
+        stream = S.dense_stream(g, 3000, seed=5)
+    else:
+        stream = S.dense_growth_stream(g, 3000, seed=5)
+    gpu, orc, og = _pair(g, deterministic=True, epoch_size=128, batch_size=128, repair_every_epoch=True)
+    init = S.generate_init_stream(g)
+    assert gpu.process_stream(init) == orc.submit(init["u"], init["v"], init["type"])
+    assert gpu.finish() == orc.finish()
+    assert gpu.process_stream(stream) == orc.submit(stream["u"], stream["v"], stream["type"])
+    assert gpu.finish() == orc.finish()
+    _same_state(gpu, orc, kind)
+    assert gpu.validate(snapshot=True)[:2] == (0, 0)
+
+
+@pytest.mark.parametrize("repair", [False, True])
+@pytest.mark.parametrize("ablation", ["force_bitmap", "force_histogram", "no_overflow", "live_degree"])
+def test_ablations_bit_exact(ablation, repair, graph_path):
+    # the reference's compile-time ablations as runtime flags, against the oracle's
+    overflow = dict(slack_pct=0, min_row_capacity=2, overflow_capacity=64) if ablation == "no_overflow" else {}
+    g = S.load_graph_csr(graph_path("er2k" if overflow else "chunglu4k"))
+    gpu, orc, og = _pair(g, deterministic=True, epoch_size=128, batch_size=128, repair_every_epoch=repair,
+                         exact_histogram=repair, ablations=(ablation,), **overflow)
+    init = S.generate_init_stream(g)
+    assert gpu.process_stream(init) == orc.submit(init["u"], init["v"], init["type"])
+    assert gpu.finish() == orc.finish()
+    bench = S.generate_bench_stream_edges(g, 2000, 70 if overflow else 50, 5)
+    assert gpu.process_stream(bench) == orc.submit(bench["u"], bench["v"], bench["type"])
+    assert gpu.finish() == orc.finish()
+    _same_state(gpu, orc, ablation)
+    if overflow:
+        assert gpu.overflow_count() == 0 and gpu.totals()["overflow_events"] > 0

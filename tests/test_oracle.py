@@ -380,3 +380,53 @@ def test_static_greedy_is_first_fit(graph_path):
     for name in ["k4", "c100", "er2k", "rmat_s10", "delaunay1k"]:
         g = O.load_mtx(graph_path(name))
         assert O.static_greedy(g) == _first_fit(g)
+
+
+# ---------------------------------------------------------------- the reference's ablations (Makefile:58-78)
+def test_ablation_force_bitmap_ignores_the_histogram():
+    # A/include/color_select.cuh:98-100: a hub selects from its bitmap.  The
+    # histogram's first zero is colour 2, the bitmap's first clear bit colour 4.
+    g, e = _engine(2, [(0, 1)], histogram_threshold=0)
+    h = e.hist()
+    h[:] = 0
+    h[0, 0] = 1
+    e.set_hist(h)
+    b = e.bitmaps()
+    b[0] = 0b111
+    e.set_bitmaps(b)
+    assert e.find_color(0) == 2
+    e.set_flags(O.FLAG_ABL_FORCE_BITMAP)
+    assert e.find_color(0) == 4
+    e.set_flags(O.FLAG_ABL_FORCE_HISTOGRAM)            # the reference's default dispatch
+    assert e.find_color(0) == 2
+
+
+def test_ablation_no_overflow_drops_unbuffered():
+    # A/src/kernels/local_update.cu:28-38: a full row drops the edge; nothing is buffered or counted
+    g = O.Csr.from_edges(4, [])
+    e = O.Engine(g, O.default_config(slack_pct=0, min_row_capacity=1, overflow_capacity=8,
+                                     flags=O.FLAG_ABL_NO_OVERFLOW))
+    e.apply(np.array([0, 0, 0]), np.array([1, 2, 3]))
+    assert e.overflow_count() == 0
+    rp, col = e.csr()
+    assert col[rp[0]:rp[1]].tolist() == [1]                 # row 0 kept its first neighbour only
+    assert col[rp[2]:rp[3]].tolist() == [0] and col[rp[3]:rp[4]].tolist() == [0]
+    st = e.finish()
+    assert st["overflow_count"] == 0 and st["overflow_drained"] == 0
+
+
+def test_ablation_live_degree_priority_picks_the_other_loser():
+    # A/src/kernels/conflict_repair.cu:46-58: a vertex's own priority comes from
+    # its live row, its neighbours' from the loaded degree.  Vertex 0 (deg0 3,
+    # live 1 after two deletes) and vertex 1 (deg0 2) share colour 1: frozen
+    # priorities make 1 the loser, the live degree makes 0 the loser.
+    edges = [(0, 1), (0, 2), (0, 3), (1, 4)]
+    for flags, loser in ((0, 1), (O.FLAG_ABL_LIVE_DEGREE, 0)):
+        g, e = _engine(5, edges, flags=flags)
+        e.apply(np.array([0, 0]), np.array([2, 3]), np.array([1, 1]))    # deletes: row 0 keeps vertex 1
+        w = np.array([1, 1, 2, 3, 2], np.uint32)                        # version 0, colours
+        e.set_words(w)
+        e.tier1()
+        after = e.words()
+        assert (after[loser] & 0xFFFF) != 1 and (after[loser] >> 16) == 1
+        assert after[1 - loser] == w[1 - loser]
