@@ -349,6 +349,12 @@ int sgc_pool_batch_prepare(sgc_pool* p, const sgc_edge_update* updates, uint64_t
                            uint32_t* rounds);
 int sgc_pool_batch_run(sgc_pool* p, sgc_batch_stats* out);
 int sgc_pool_batch_release(sgc_pool* p);
+/* The same for a single-graph engine: its epochs of E updates replayed as
+ * captured rounds (records need no graph id). */
+int sgc_batch_prepare(sgc_engine* e, const sgc_edge_update* updates, uint64_t count, const sgc_batch_config* cfg,
+                      uint32_t* rounds);
+int sgc_batch_run(sgc_engine* e, sgc_batch_stats* out);
+int sgc_batch_release(sgc_engine* e);
 
 /* ---- served pattern: a persistent request loop over a pool ----
  * No reference counterpart.  Any thread submits update batches and queries

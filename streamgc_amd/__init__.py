@@ -333,6 +333,25 @@ class GPUPartition:
         self._ck(lib().sgc_submit(self.h, ptr, count, 1, st if max_epochs else None, max_epochs, C.byref(ne)))
         return [st[i].as_dict() for i in range(min(ne.value, max_epochs))]
 
+    def batch_prepare(self, records, use_graph=True, t1_sweeps=0, autotune_rounds=0, t2_passes=0):
+        """The stream's epochs as captured rounds (see GraphPool.batch_prepare); returns the rounds."""
+        records = np.ascontiguousarray(records, dtype=EDGE_UPDATE)
+        cfg = _lib.BatchConfig()
+        lib().sgc_default_batch_config(C.byref(cfg))
+        cfg.use_graph, cfg.t1_sweeps, cfg.autotune_rounds, cfg.t2_passes = (1 if use_graph else 0, t1_sweeps,
+                                                                             autotune_rounds, t2_passes)
+        r = C.c_uint32()
+        self._ck(lib().sgc_batch_prepare(self.h, records.ctypes.data, len(records), C.byref(cfg), C.byref(r)))
+        return r.value
+
+    def batch_run(self):
+        st = _lib.BatchStats()
+        self._ck(lib().sgc_batch_run(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def batch_release(self):
+        self._ck(lib().sgc_batch_release(self.h))
+
     def apply(self, records):
         """= one local_update_kernel launch per batch, no epoch step."""
         if isinstance(records, DeviceStream):

@@ -208,6 +208,9 @@ SIGNATURES = {
     "sgc_pool_batch_prepare": [_vp, _vp, C.c_uint64, C.POINTER(BatchConfig), C.POINTER(C.c_uint32)],
     "sgc_pool_batch_run": [_vp, C.POINTER(BatchStats)],
     "sgc_pool_batch_release": [_vp],
+    "sgc_batch_prepare": [_vp, _vp, C.c_uint64, C.POINTER(BatchConfig), C.POINTER(C.c_uint32)],
+    "sgc_batch_run": [_vp, C.POINTER(BatchStats)],
+    "sgc_batch_release": [_vp],
     "sgc_pool_kernel_times": [_vp, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                               C.c_uint32, C.POINTER(C.c_uint32)],
     "sgc_default_server_config": [C.POINTER(ServerConfig)],

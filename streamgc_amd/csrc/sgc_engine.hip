@@ -2431,6 +2431,12 @@ int sgc_pool_batch_run(sgc_pool* p, sgc_batch_stats* out) {
     if (!p) return SGC_ERR_INVALID;
     return p->batch_run(out);
 }
+int sgc_batch_prepare(sgc_engine* e, const sgc_edge_update* updates, uint64_t count, const sgc_batch_config* cfg,
+                      uint32_t* rounds) {
+    return sgc_pool_batch_prepare(e, updates, count, cfg, rounds);
+}
+int sgc_batch_run(sgc_engine* e, sgc_batch_stats* out) { return sgc_pool_batch_run(e, out); }
+int sgc_batch_release(sgc_engine* e) { return sgc_pool_batch_release(e); }
 int sgc_pool_batch_release(sgc_pool* p) {
     if (!p) return SGC_ERR_INVALID;
     p->batch_drop();

@@ -176,3 +176,29 @@ def test_batch_refuses_rounds_over_the_batch_capacity(graph_path):
     pool, graphs, gid, churn, mixed = _pool(graph_path, False, batch_capacity=256)
     with pytest.raises(S.SgcError):
         pool.batch_prepare(mixed)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("name", ["er2k", "rmat_s10"])
+def test_single_engine_batch_equals_the_oracle(name, use_graph, graph_path):
+    # a plain engine's epochs replayed as captured rounds: the same words,
+    # slots and snapshot as the oracle's epoch-by-epoch run
+    g = S.load_graph_csr(graph_path(name))
+    cfg = S.StreamGCConfig(epoch_size=128, batch_size=128, deterministic=True, repair_every_epoch=True,
+                           exact_histogram=True)
+    gpu = S.GPUPartition(g, cfg)
+    gpu.process_stream(S.generate_init_stream(g))
+    gpu.finish()
+    churn = S.generate_bench_stream_edges(g, 1000, 50, 5)
+    rounds = gpu.batch_prepare(churn, use_graph=use_graph)
+    assert rounds == -(-len(churn) // 128)
+    st = gpu.batch_run()
+    assert st["rounds"] == rounds and st["updates"] == len(churn)
+    gpu.batch_release()
+    orc = _oracle(g, cfg, churn)
+    assert np.array_equal(gpu.words(), orc.words())
+    assert np.array_equal(gpu.snapshot(), orc.snapshot())
+    grp, gcol = gpu.csr()
+    orp, ocol = orc.csr()
+    assert np.array_equal(grp, orp) and np.array_equal(gcol, ocol)
+    assert gpu.validate(snapshot=True)[:2] == (0, 0)
