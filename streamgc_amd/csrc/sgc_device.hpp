@@ -75,6 +75,12 @@ enum : uint32_t {
     T_SWEEP_SLOTS = T_SWEEP_ROWS + 4,  // 4 counters: row slots those sweeps read
     T_COUNT = T_SWEEP_SLOTS + 4
 };
+// The totals are striped: T_STRIPES copies of the counters, T_STRIDE u64
+// apart (two cache lines), a block adding into copy blockIdx % T_STRIPES;
+// the host sums the copies.  Thousands of waves adding into one address
+// serialise on it and hold up every other atomic of its memory channel.
+constexpr uint32_t T_STRIPES = 256, T_STRIDE = 32;
+static_assert(T_COUNT <= T_STRIDE, "totals stripe too small");
 
 struct GraphDev {
     uint32_t* word;           // n packed (version << 16) | colour
@@ -159,6 +165,9 @@ __device__ __forceinline__ uint32_t add32(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ void add64(uint64_t* p, uint64_t v) {
     __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void tot_add(uint64_t* totals, uint32_t idx, uint64_t v) {
+    add64(&totals[(size_t)(blockIdx.x % T_STRIPES) * T_STRIDE + idx], v);
+}
 __device__ __forceinline__ uint32_t max32(uint32_t* p, uint32_t v) {
     return __hip_atomic_fetch_max(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -195,13 +204,13 @@ struct WaveAcc {
     uint64_t updates = 0, scan = 0, hd = 0, recolor = 0, rc_cap = 0, rc_live = 0, ovf = 0;
     __device__ __forceinline__ void flush(uint64_t* totals) {
         if (!totals || lane() != 0) return;
-        if (updates) add64(&totals[T_UPDATES], updates);
-        if (scan) add64(&totals[T_SCAN], scan);
-        if (hd) add64(&totals[T_HD], hd);
-        if (recolor) add64(&totals[T_RECOLOR], recolor);
-        if (rc_cap) add64(&totals[T_RC_CAP], rc_cap);
-        if (rc_live) add64(&totals[T_RC_LIVE], rc_live);
-        if (ovf) add64(&totals[T_OVF_EVENTS], ovf);
+        if (updates) tot_add(totals, T_UPDATES, updates);
+        if (scan) tot_add(totals, T_SCAN, scan);
+        if (hd) tot_add(totals, T_HD, hd);
+        if (recolor) tot_add(totals, T_RECOLOR, recolor);
+        if (rc_cap) tot_add(totals, T_RC_CAP, rc_cap);
+        if (rc_live) tot_add(totals, T_RC_LIVE, rc_live);
+        if (ovf) tot_add(totals, T_OVF_EVENTS, ovf);
     }
 };
 
@@ -273,17 +282,15 @@ __device__ __forceinline__ uint16_t find_color(const GraphDev& g, uint32_t x) {
     return select_by_hist(g, x) ? fas_hist(g, gp(g.hd_map)[x]) : fas_bitmap(g, x);
 }
 
-// A/include/color_select.cuh:160-183 (lane 0 only: one neighbour)
-__device__ __forceinline__ void on_new_neighbor(const GraphDev& g, uint32_t x, uint16_t c) {
+// A/include/color_select.cuh:160-183 (lane 0 only: one neighbour); hd = hd_map[x]
+__device__ __forceinline__ void on_new_neighbor(const GraphDev& g, uint32_t x, uint32_t hd, uint16_t c) {
     if (c == 0 || lane() != 0) return;
     if (c <= 64) or64(&g.bitmap[x], 1ull << (c - 1));
-    uint32_t hd = gp(g.hd_map)[x];
     if (hd != INVALID && c <= HIST_BINS) add32(&g.hist[(size_t)hd * HIST_BINS + c - 1], 1u);
 }
 // A/include/color_select.cuh:188-209 (bitmap bits stay set until a rebuild)
-__device__ __forceinline__ void on_removed_neighbor(const GraphDev& g, uint32_t x, uint16_t c) {
+__device__ __forceinline__ void on_removed_neighbor(const GraphDev& g, uint32_t x, uint32_t hd, uint16_t c) {
     if (c == 0 || lane() != 0) return;
-    uint32_t hd = gp(g.hd_map)[x];
     if (hd != INVALID && c <= HIST_BINS) add32(&g.hist[(size_t)hd * HIST_BINS + c - 1], 0xFFFFFFFFu);
 }
 // A/include/color_select.cuh:120-157 (per lane: each lane owns one neighbour)
@@ -400,30 +407,138 @@ __device__ __forceinline__ void mark_neighbors(const GraphDev& g, uint32_t v, ui
     }
 }
 
-// A/src/kernels/local_update.cu:42-56: the first slot holding nb becomes free.
-__device__ __forceinline__ uint32_t row_remove(const GraphDev& g, uint32_t x, uint32_t nb) {
-    const uint32_t s = gp(g.row_ptr)[x], e = s + ld(&g.fill[x]);
-    for (uint32_t base = s; base < e; base += ROW_UNROLL * WAVE) {
-        uint32_t val[ROW_UNROLL];
+// Both rows of one update, scanned in the same memory round trips: an insert
+// looks for the first slot that is free or already holds the neighbour
+// (A/src/kernels/local_update.cu:12-39), a delete for the first slot holding
+// it (A/src/kernels/local_update.cu:42-56), row 0 and row 1 each on its own.
+// A round trip loads 4 x 64 slots of each row still searching (8 x 64 when
+// one row is left).  The rows' matches are then claimed together, lane 0
+// for row 0 and lane 1 for row 1: one CAS each, and for an insert the raise
+// of the row's fill to the claimed position (harmless when the CAS loses:
+// the slot's winner raised it as far).  A row whose CAS lost resumes its scan
+// after that slot.  The two rows must differ; a single wave applying a
+// stream in order claims exactly the slots of the two scans one after the
+// other.
+// Out: pos = 1-based position of the row's slot (0: none before the end);
+// pf = the fill before an insert's raise when this wave wrote the slot, else 0.
+template <bool INSERT>
+__device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t x0, uint32_t nb0, uint32_t s0,
+                                          uint32_t end0, bool run1, uint32_t x1, uint32_t nb1, uint32_t s1,
+                                          uint32_t end1, uint32_t& pos0, uint32_t& pos1, uint32_t& pf0,
+                                          uint32_t& pf1) {
+    constexpr uint32_t CH = 2 * ROW_UNROLL;   // 64-slot chunks per round trip
+    uint32_t b0 = s0, b1 = s1;
+    pos0 = pos1 = pf0 = pf1 = 0;
+    run0 = run0 && b0 < end0;
+    run1 = run1 && b1 < end1;
+    while (run0 || run1) {
+        const uint32_t per0 = run0 ? (run1 ? ROW_UNROLL : CH) : 0;   // chunks of row 0; the rest are row 1's
+        uint32_t val[CH];
 #pragma unroll
-        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
-            const uint32_t j = base + u * WAVE + lane();
-            val[u] = j < e ? ld(&g.col[j]) : INVALID;
+        for (uint32_t q = 0; q < CH; q++) {
+            const bool r1 = q >= per0;
+            const uint32_t j = (r1 ? b1 + (q - per0) * WAVE : b0 + q * WAVE) + lane();
+            val[q] = j < (r1 ? end1 : end0) ? ld(&g.col[j]) : 0u;
         }
+        uint32_t c0 = INVALID, c1 = INVALID;   // first match of each row in this trip
+        bool eq0 = false, eq1 = false;         // it already holds the neighbour (insert)
 #pragma unroll
-        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
-            const uint32_t b = base + u * WAVE;
-            uint64_t m = ballot(b + lane() < e && val[u] == nb);
-            while (m) {
+        for (uint32_t q = 0; q < CH; q++) {
+            const bool r1 = q >= per0;
+            const uint32_t b = r1 ? b1 + (q - per0) * WAVE : b0 + q * WAVE;
+            const uint32_t want = r1 ? nb1 : nb0;
+            const bool in = b + lane() < (r1 ? end1 : end0);
+            const uint64_t m = ballot(in && (val[q] == want || (INSERT && val[q] == INVALID)));
+            if (m && (r1 ? c1 : c0) == INVALID) {
                 const uint32_t k = (uint32_t)__builtin_ctzll(m);
-                uint32_t old = 0;
-                if (lane() == 0) old = cas32(&g.col[b + k], nb, INVALID);
-                old = bcast0(old);
-                if (old == nb) return b + k - s + 1;
-                m &= m - 1;
+                const bool eq = __builtin_amdgcn_readlane(val[q], k) == want;
+                if (r1) {
+                    c1 = b + k;
+                    eq1 = eq;
+                } else {
+                    c0 = b + k;
+                    eq0 = eq;
+                }
+            }
+        }
+        if (run0 && c0 == INVALID) {
+            b0 += per0 * WAVE;
+            run0 = b0 < end0;
+        }
+        if (run1 && c1 == INVALID) {
+            b1 += (CH - per0) * WAVE;
+            run1 = b1 < end1;
+        }
+        if (INSERT && c0 != INVALID && eq0) {   // the neighbour is there already
+            pos0 = c0 - s0 + 1;
+            run0 = false;
+            c0 = INVALID;
+        }
+        if (INSERT && c1 != INVALID && eq1) {
+            pos1 = c1 - s1 + 1;
+            run1 = false;
+            c1 = INVALID;
+        }
+        if (c0 == INVALID && c1 == INVALID) continue;
+        uint32_t old = 0, pf = 0;
+        const bool mine = lane() == 0 ? c0 != INVALID : (lane() == 1 && c1 != INVALID);
+        if (mine) {
+            const uint32_t at = lane() == 0 ? c0 : c1;
+            const uint32_t want = lane() == 0 ? nb0 : nb1;
+            if (INSERT) {
+                old = cas32(&g.col[at], INVALID, want);
+                pf = max32(&g.fill[lane() == 0 ? x0 : x1], at - (lane() == 0 ? s0 : s1) + 1);
+            } else {
+                old = cas32(&g.col[at], want, INVALID);
+            }
+        }
+        const uint32_t old0 = __builtin_amdgcn_readlane(old, 0), old1 = __builtin_amdgcn_readlane(old, 1);
+        const uint32_t f0 = __builtin_amdgcn_readlane(pf, 0), f1 = __builtin_amdgcn_readlane(pf, 1);
+        if (c0 != INVALID) {
+            if (old0 == nb0 || (INSERT && old0 == INVALID)) {
+                pos0 = c0 - s0 + 1;
+                pf0 = INSERT && old0 == INVALID ? f0 : 0u;
+                run0 = false;
+            } else {   // another wave took the slot: go on after it
+                b0 = c0 + 1;
+                run0 = b0 < end0;
+            }
+        }
+        if (c1 != INVALID) {
+            if (old1 == nb1 || (INSERT && old1 == INVALID)) {
+                pos1 = c1 - s1 + 1;
+                pf1 = INSERT && old1 == INVALID ? f1 : 0u;
+                run1 = false;
+            } else {
+                b1 = c1 + 1;
+                run1 = b1 < end1;
             }
         }
     }
+}
+
+// An insert into row x (slots [s, e)) after its scan: a slot written in a hole
+// may duplicate a copy further along; no slot at all is an overflow event.
+// Returns the slots scanned (the position found, or the whole row).
+__device__ __forceinline__ uint32_t finish_insert(const GraphDev& g, uint32_t x, uint32_t nb, uint32_t s, uint32_t e,
+                                                  uint32_t pos, uint32_t pf, WaveAcc& acc) {
+    if (pos) {
+        if (pf > pos && nb < g.n) flag_duplicate(g, x, nb, s + pos, s + pf);
+        return pos;
+    }
+    if (lane() == 0) {
+        add32(&g.ctr[C_ASYM], 1u);
+        if (!(g.flags & ABL_NO_OVERFLOW)) {   // the ablation drops the edge unbuffered and uncounted
+            uint32_t at = add32(&g.ctr[C_OVF], 1u);
+            if (at < g.ovf_cap) {
+                st(&g.ovf[2 * (size_t)at], x);
+                st(&g.ovf[2 * (size_t)at + 1], nb);
+            }
+        }
+        mark_sticky(g, x);
+        mark_sticky(g, nb);
+    }
+    acc.ovf++;
     return e - s;
 }
 
@@ -481,9 +596,9 @@ __device__ __forceinline__ bool recolor(const GraphDev& g, uint32_t x, WaveAcc& 
     return false;
 }
 
-// A/src/kernels/recolor.cu:145-163
-__device__ __forceinline__ void try_reduce(const GraphDev& g, uint32_t x, uint16_t freed, WaveAcc& acc) {
-    const uint16_t cur = color_of(bcast0(ld(&g.word[x])));
+// A/src/kernels/recolor.cu:145-163; cur = x's colour (read by the caller
+// after the removal).
+__device__ __forceinline__ void try_reduce(const GraphDev& g, uint32_t x, uint16_t freed, uint16_t cur, WaveAcc& acc) {
     if (cur == 0 || freed >= cur) return;
     if (find_color(g, x) < cur) recolor(g, x, acc);
 }
@@ -496,48 +611,64 @@ __device__ __forceinline__ void color_fresh(const GraphDev& g, uint32_t x, uint3
     mark_neighbors(g, x, 0, c);
 }
 
-// One update, A/src/kernels/local_update.cu:71-146.
+// One update, A/src/kernels/local_update.cu:71-146.  The reads each step
+// needs are issued together (row bounds, histogram rows, words), and both rows
+// are scanned and claimed in the same round trips (scan_pair).  A vertex word
+// changes only by a recolour, so a colour read before the row writes equals
+// the one read after them unless another wave recoloured in between, which
+// is one more interleaving the parallel mode allows anyway.
 __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint32_t v, uint32_t type, WaveAcc& acc) {
-    if (u >= g.n || v >= g.n) return;  // hash miss: skipped (:77)
-    if (type <= 1 && lane() == 0) {
+    if (u >= g.n || v >= g.n || type > 1) return;  // hash miss: skipped (:77); other kinds do nothing
+    const uint32_t su = gp(g.row_ptr)[u], eu = gp(g.row_ptr)[u + 1];
+    const uint32_t sv = gp(g.row_ptr)[v], ev = gp(g.row_ptr)[v + 1];
+    const uint32_t hu = gp(g.hd_map)[u], hv = gp(g.hd_map)[v];
+    if (lane() == 0) {
         // an insert fills the first free slot (the row stays packed); a delete leaves a hole
         const uint32_t bits = type == 1 ? D_ALL : (D_BITS | D_STATUS);
         mark(g, u, bits);
         mark(g, v, bits);
     }
+    const bool two = u != v;   // a self-loop's two rows are one: scanned one after the other
+    uint32_t pu, pv, fu, fv, d0, d1;
     if (type == 0) {
-        uint32_t p = row_insert(g, u, v, true, nullptr, &acc);
-        p += row_insert(g, v, u, true, nullptr, &acc);
-        uint32_t wu = bcast0(ld(&g.word[u]));
-        if (color_of(wu) == 0) color_fresh(g, u, wu);
-        const uint32_t wv = bcast0(ld(&g.word[v]));
-        if (color_of(wv) == 0) {
-            on_new_neighbor(g, v, color_of(bcast0(ld(&g.word[u]))));
-            color_fresh(g, v, wv);
+        scan_pair<true>(g, true, u, v, su, eu, two, v, u, sv, ev, pu, pv, fu, fv);
+        if (!two) scan_pair<true>(g, false, u, v, su, eu, true, v, u, sv, ev, d0, pv, d1, fv);
+        uint32_t p = finish_insert(g, u, v, su, eu, pu, fu, acc);
+        p += finish_insert(g, v, u, sv, ev, pv, fv, acc);
+        const uint32_t ru = ld(&g.word[u]), rv = ld(&g.word[v]);
+        uint16_t cu = color_of(bcast0(ru)), cv = color_of(bcast0(rv));
+        if (cu == 0 || cv == 0) {   // first colouring of an endpoint (:90-101)
+            const uint32_t wu = bcast0(ld(&g.word[u]));
+            if (color_of(wu) == 0) color_fresh(g, u, wu);
+            const uint32_t wv = bcast0(ld(&g.word[v]));
+            if (color_of(wv) == 0) {
+                on_new_neighbor(g, v, hv, color_of(bcast0(ld(&g.word[u]))));
+                color_fresh(g, v, wv);
+            }
+            cu = color_of(bcast0(ld(&g.word[u])));
+            cv = color_of(bcast0(ld(&g.word[v])));
         }
-        const uint16_t cu = color_of(bcast0(ld(&g.word[u])));
-        const uint16_t cv = color_of(bcast0(ld(&g.word[v])));
-        on_new_neighbor(g, u, cv);
-        on_new_neighbor(g, v, cu);
-        const uint16_t cu2 = color_of(bcast0(ld(&g.word[u])));
-        const uint16_t cv2 = color_of(bcast0(ld(&g.word[v])));
-        if (ballot(cu2 == cv2 && cu2 != 0)) recolor(g, pi_less(g, u, v) ? u : v, acc);
-        acc.updates++;
+        on_new_neighbor(g, u, hu, cv);
+        on_new_neighbor(g, v, hv, cu);
+        const uint32_t qu = ld(&g.word[u]), qv = ld(&g.word[v]);
+        const uint16_t cu2 = color_of(bcast0(qu)), cv2 = color_of(bcast0(qv));
+        if (cu2 == cv2 && cu2 != 0) recolor(g, pi_less(g, u, v) ? u : v, acc);
         acc.scan += p;
-        acc.hd += (gp(g.hd_map)[u] != INVALID) + (gp(g.hd_map)[v] != INVALID);
-    } else if (type == 1) {
-        const uint16_t fu = color_of(bcast0(ld(&g.word[v])));
-        const uint16_t fv = color_of(bcast0(ld(&g.word[u])));
-        uint32_t p = row_remove(g, u, v);
-        p += row_remove(g, v, u);
-        on_removed_neighbor(g, u, fu);
-        on_removed_neighbor(g, v, fv);
-        try_reduce(g, u, fu, acc);
-        try_reduce(g, v, fv, acc);
-        acc.updates++;
-        acc.scan += p;
-        acc.hd += (gp(g.hd_map)[u] != INVALID) + (gp(g.hd_map)[v] != INVALID);
+    } else {
+        const uint32_t ru = ld(&g.word[u]), rv = ld(&g.word[v]);
+        const uint32_t lu = ld(&g.fill[u]), lv = ld(&g.fill[v]);
+        const uint16_t cu = color_of(bcast0(ru)), cv = color_of(bcast0(rv));   // freed at v, freed at u
+        const uint32_t tu = su + bcast0(lu), tv = sv + bcast0(lv);
+        scan_pair<false>(g, true, u, v, su, tu, two, v, u, sv, tv, pu, pv, fu, fv);
+        if (!two) scan_pair<false>(g, false, u, v, su, tu, true, v, u, sv, tv, d0, pv, d1, fv);
+        acc.scan += (pu ? pu : tu - su) + (pv ? pv : tv - sv);
+        on_removed_neighbor(g, u, hu, cv);
+        on_removed_neighbor(g, v, hv, cu);
+        try_reduce(g, u, cv, cu, acc);
+        try_reduce(g, v, cu, cv, acc);
     }
+    acc.updates++;
+    acc.hd += (hu != INVALID) + (hv != INVALID);
 }
 
 }  // namespace sgc

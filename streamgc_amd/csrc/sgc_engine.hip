@@ -1615,9 +1615,9 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipMalloc(&P->d_ctr, (size_t)max_graphs * C_COUNT * 4);
     if (e == hipSuccess) e = hipMemsetAsync(P->d_ctr, 0, (size_t)max_graphs * C_COUNT * 4, P->stream);
     if (e == hipSuccess) e = hipHostMalloc(&P->h_ctr, (size_t)max_graphs * C_COUNT * 4);
-    if (e == hipSuccess) e = hipMalloc(&P->d_totals, T_COUNT * 8);
-    if (e == hipSuccess) e = hipMemsetAsync(P->d_totals, 0, T_COUNT * 8, P->stream);
-    if (e == hipSuccess) e = hipHostMalloc(&P->h_totals, T_COUNT * 8);
+    if (e == hipSuccess) e = hipMalloc(&P->d_totals, (size_t)T_STRIPES * T_STRIDE * 8);
+    if (e == hipSuccess) e = hipMemsetAsync(P->d_totals, 0, (size_t)T_STRIPES * T_STRIDE * 8, P->stream);
+    if (e == hipSuccess) e = hipHostMalloc(&P->h_totals, (size_t)T_STRIPES * T_STRIDE * 8);
     if (e == hipSuccess) e = hipMalloc(&P->d_batch, (size_t)batch_cap * sizeof(sgc_edge_update));
     if (e == hipSuccess) e = hipMalloc(&P->d_tmp16, 64);
     if (e == hipSuccess) e = hipMalloc(&P->d_list_count, 8);
@@ -2016,8 +2016,10 @@ int sgc_overflow_count(sgc_engine* e, uint32_t* c) {
 }
 
 static int totals_impl(sgc_pool* P, sgc_totals* out) {
-    int r = download(P, P->h_totals, P->d_totals, T_COUNT * 8);
+    int r = download(P, P->h_totals, P->d_totals, (size_t)T_STRIPES * T_STRIDE * 8);
     if (r) return r;
+    for (uint32_t k = 0; k < T_COUNT; k++)   // the stripes' sum lands in stripe 0
+        for (uint32_t q = 1; q < T_STRIPES; q++) P->h_totals[k] += P->h_totals[(size_t)q * T_STRIDE + k];
     memset(out, 0, sizeof *out);
     out->updates = P->h_totals[T_UPDATES];
     out->scan_slots = P->h_totals[T_SCAN];

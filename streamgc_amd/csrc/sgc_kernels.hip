@@ -69,7 +69,7 @@ __device__ __forceinline__ void block_add64(uint64_t* totals, uint32_t idx, uint
     if (threadIdx.x == 0 && totals) {
         uint64_t t = 0;
         for (uint32_t k = 0; k < BLOCK / WAVE; k++) t += part[k];
-        if (t) add64(&totals[idx], t);
+        if (t) tot_add(totals, idx, t);
     }
     __syncthreads();
 }
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__
         }
     }
     acc.flush(totals);
-    if (grouped && totals && lane() == 0) add64(&totals[T_GROUPED], grouped);
+    if (grouped && totals && lane() == 0) tot_add(totals, T_GROUPED, grouped);
 }
 
 // One wavefront per segment, the segment's updates in order; positions go
@@ -995,9 +995,9 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
                 }
             }
         }
-        if (lane() == 0) add32(&g.ctr[C_CONFLICTS], cnt);
+        if (lane() == 0 && cnt) add32(&g.ctr[C_CONFLICTS], cnt);
     }
-    if (lane() == 0 && totals && rows) add64(&totals[T_SWEEP_ROWS + 2], rows);
+    if (lane() == 0 && totals && rows) tot_add(totals, T_SWEEP_ROWS + 2, rows);
 }
 
 // Vertices the host Tier-2 recoloured: their own status and their
