@@ -420,15 +420,17 @@ __device__ __forceinline__ void mark_neighbors(const GraphDev& g, uint32_t v, ui
 // stream in order claims exactly the slots of the two scans one after the
 // other.
 // Out: pos = 1-based position of the row's slot (0: none before the end);
-// pf = the fill before an insert's raise when this wave wrote the slot, else 0.
+// pf = the fill before an insert's raise when this wave wrote the slot, else 0;
+// had = an insert found the neighbour in the row already.
 template <bool INSERT>
 __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t x0, uint32_t nb0, uint32_t s0,
                                           uint32_t end0, bool run1, uint32_t x1, uint32_t nb1, uint32_t s1,
                                           uint32_t end1, uint32_t& pos0, uint32_t& pos1, uint32_t& pf0,
-                                          uint32_t& pf1) {
+                                          uint32_t& pf1, bool& had0, bool& had1) {
     constexpr uint32_t CH = 2 * ROW_UNROLL;   // 64-slot chunks per round trip
     uint32_t b0 = s0, b1 = s1;
     pos0 = pos1 = pf0 = pf1 = 0;
+    had0 = had1 = false;
     run0 = run0 && b0 < end0;
     run1 = run1 && b1 < end1;
     while (run0 || run1) {
@@ -471,11 +473,13 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
         }
         if (INSERT && c0 != INVALID && eq0) {   // the neighbour is there already
             pos0 = c0 - s0 + 1;
+            had0 = true;
             run0 = false;
             c0 = INVALID;
         }
         if (INSERT && c1 != INVALID && eq1) {
             pos1 = c1 - s1 + 1;
+            had1 = true;
             run1 = false;
             c1 = INVALID;
         }
@@ -498,6 +502,7 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
             if (old0 == nb0 || (INSERT && old0 == INVALID)) {
                 pos0 = c0 - s0 + 1;
                 pf0 = INSERT && old0 == INVALID ? f0 : 0u;
+                had0 = INSERT && old0 == nb0;
                 run0 = false;
             } else {   // another wave took the slot: go on after it
                 b0 = c0 + 1;
@@ -508,6 +513,7 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
             if (old1 == nb1 || (INSERT && old1 == INVALID)) {
                 pos1 = c1 - s1 + 1;
                 pf1 = INSERT && old1 == INVALID ? f1 : 0u;
+                had1 = INSERT && old1 == nb1;
                 run1 = false;
             } else {
                 b1 = c1 + 1;
@@ -617,27 +623,49 @@ __device__ __forceinline__ void color_fresh(const GraphDev& g, uint32_t x, uint3
 // changes only by a recolour, so a colour read before the row writes equals
 // the one read after them unless another wave recoloured in between, which
 // is one more interleaving the parallel mode allows anyway.
+//
+// Marks.  A delete leaves a hole (ROW) and a stale bitmap bit (BITS); it
+// cannot give either endpoint a conflict.  An insert that stored the edge in
+// both rows and then read the colours leaves both bitmaps and histograms
+// exact (a recolour of an endpoint after those reads sees the new slot and
+// tells the other endpoint itself), and once it has resolved a same-colour
+// edge on the spot neither endpoint's Tier-1 verdict changed.  So an insert
+// marks only what it leaves inexact or unresolved: BITS where the row held the
+// edge already (its histogram counts it twice), everything where an endpoint
+// was uncoloured, and CHECK/T2 where the endpoints still share a colour.
+// With the live-degree ablation a row change moves the vertex's own
+// priority, so both kinds mark CHECK/T2 too.
 __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint32_t v, uint32_t type, WaveAcc& acc) {
     if (u >= g.n || v >= g.n || type > 1) return;  // hash miss: skipped (:77); other kinds do nothing
     const uint32_t su = gp(g.row_ptr)[u], eu = gp(g.row_ptr)[u + 1];
     const uint32_t sv = gp(g.row_ptr)[v], ev = gp(g.row_ptr)[v + 1];
     const uint32_t hu = gp(g.hd_map)[u], hv = gp(g.hd_map)[v];
     if (lane() == 0) {
-        // an insert fills the first free slot (the row stays packed); a delete leaves a hole
-        const uint32_t bits = type == 1 ? D_ALL : (D_BITS | D_STATUS);
-        mark(g, u, bits);
-        mark(g, v, bits);
+        const uint32_t bits = (type == 1 ? D_ROW | D_BITS : 0u) | ((g.flags & ABL_LIVE_DEGREE) ? D_STATUS : 0u);
+        if (bits) {
+            mark(g, u, bits);
+            mark(g, v, bits);
+        }
     }
     const bool two = u != v;   // a self-loop's two rows are one: scanned one after the other
     uint32_t pu, pv, fu, fv, d0, d1;
+    bool hadu, hadv, h0;
     if (type == 0) {
-        scan_pair<true>(g, true, u, v, su, eu, two, v, u, sv, ev, pu, pv, fu, fv);
-        if (!two) scan_pair<true>(g, false, u, v, su, eu, true, v, u, sv, ev, d0, pv, d1, fv);
+        scan_pair<true>(g, true, u, v, su, eu, two, v, u, sv, ev, pu, pv, fu, fv, hadu, hadv);
+        if (!two) scan_pair<true>(g, false, u, v, su, eu, true, v, u, sv, ev, d0, pv, d1, fv, h0, hadv);
         uint32_t p = finish_insert(g, u, v, su, eu, pu, fu, acc);
         p += finish_insert(g, v, u, sv, ev, pv, fv, acc);
+        if (lane() == 0) {
+            if (hadu) mark(g, u, D_BITS);
+            if (hadv) mark(g, v, D_BITS);
+        }
         const uint32_t ru = ld(&g.word[u]), rv = ld(&g.word[v]);
         uint16_t cu = color_of(bcast0(ru)), cv = color_of(bcast0(rv));
         if (cu == 0 || cv == 0) {   // first colouring of an endpoint (:90-101)
+            if (lane() == 0) {
+                mark(g, u, D_BITS | D_STATUS);
+                mark(g, v, D_BITS | D_STATUS);
+            }
             const uint32_t wu = bcast0(ld(&g.word[u]));
             if (color_of(wu) == 0) color_fresh(g, u, wu);
             const uint32_t wv = bcast0(ld(&g.word[v]));
@@ -652,15 +680,21 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
         on_new_neighbor(g, v, hv, cu);
         const uint32_t qu = ld(&g.word[u]), qv = ld(&g.word[v]);
         const uint16_t cu2 = color_of(bcast0(qu)), cv2 = color_of(bcast0(qv));
-        if (cu2 == cv2 && cu2 != 0) recolor(g, pi_less(g, u, v) ? u : v, acc);
+        if (cu2 == cv2) {   // same colour (or both uncoloured): the sweeps look again whatever happens here
+            if (lane() == 0) {
+                mark(g, u, D_STATUS);
+                mark(g, v, D_STATUS);
+            }
+            if (cu2 != 0) recolor(g, pi_less(g, u, v) ? u : v, acc);
+        }
         acc.scan += p;
     } else {
         const uint32_t ru = ld(&g.word[u]), rv = ld(&g.word[v]);
         const uint32_t lu = ld(&g.fill[u]), lv = ld(&g.fill[v]);
         const uint16_t cu = color_of(bcast0(ru)), cv = color_of(bcast0(rv));   // freed at v, freed at u
         const uint32_t tu = su + bcast0(lu), tv = sv + bcast0(lv);
-        scan_pair<false>(g, true, u, v, su, tu, two, v, u, sv, tv, pu, pv, fu, fv);
-        if (!two) scan_pair<false>(g, false, u, v, su, tu, true, v, u, sv, tv, d0, pv, d1, fv);
+        scan_pair<false>(g, true, u, v, su, tu, two, v, u, sv, tv, pu, pv, fu, fv, hadu, hadv);
+        if (!two) scan_pair<false>(g, false, u, v, su, tu, true, v, u, sv, tv, d0, pv, d1, fv, h0, hadv);
         acc.scan += (pu ? pu : tu - su) + (pv ? pv : tv - sv);
         on_removed_neighbor(g, u, hu, cv);
         on_removed_neighbor(g, v, hv, cu);

@@ -63,7 +63,8 @@ constexpr uint32_t ABL_MASK = SGC_ABL_FORCE_BITMAP | SGC_ABL_FORCE_HISTOGRAM | S
 
 // Bytes one graph occupies on the device (admission control).
 uint64_t graph_bytes(uint32_t n, uint64_t S, uint32_t nhd, uint32_t ovf_cap) {
-    return 32ull * n + 4 + (uint64_t)M_ARRAYS * mark_stride(n) + 4 * S + 4ull * nhd + 4096ull * nhd +
+    return 28ull * n + 4ull * mark_stride(n) + 4 + (uint64_t)M_ARRAYS * mark_stride(n) + 4 * S + 4ull * nhd +
+           4096ull * nhd +
            8ull * (ovf_cap ? ovf_cap : 1);
 }
 
@@ -525,7 +526,7 @@ int sgc_pool::add_graph(const sgc_config* cfg, uint32_t n, const uint32_t* row_p
     A((void**)&G.deg0, (size_t)n * 4);
     A((void**)&G.row_ptr, ((size_t)n + 1) * 4);
     A((void**)&G.col, S * 4);
-    A((void**)&G.fill, (size_t)n * 4);
+    A((void**)&G.fill, (size_t)mark_stride(n) * 4);   // padded: sweeps read fill 4 vertices at a time
     A((void**)&G.bitmap, (size_t)n * 8);
     A((void**)&G.hd_map, (size_t)n * 4);
     A((void**)&G.hubs, (size_t)nhd * 4);
@@ -1090,7 +1091,8 @@ void sgc_pool::release() {
     perm.release();
     seg.release();
     for (void* p : {(void*)d_ctr, (void*)d_totals, (void*)d_batch, (void*)d_tmp16, (void*)d_qids, (void*)d_qout,
-                    (void*)eg.keys, (void*)eg.head, (void*)eg.next, (void*)d_list, (void*)d_list_count,
+                    (void*)eg.keys, (void*)eg.head, (void*)eg.next, (void*)eg.heads, (void*)eg.nheads,
+                    (void*)d_list, (void*)d_list_count,
                     (void*)d_t2_items, d_rev, (void*)d_rev_count})
         if (p) (void)hipFree(p);
     for (void* p : {(void*)h_ctr, (void*)h_totals})
@@ -1628,6 +1630,8 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipMalloc(&P->eg.keys, (size_t)P->eg.size * 8);
     if (e == hipSuccess) e = hipMalloc(&P->eg.head, (size_t)P->eg.size * 4);
     if (e == hipSuccess) e = hipMalloc(&P->eg.next, (size_t)batch_cap * 4);
+    if (e == hipSuccess) e = hipMalloc(&P->eg.heads, (size_t)batch_cap * 4);
+    if (e == hipSuccess) e = hipMalloc(&P->eg.nheads, 4);
     if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
     if (e != hipSuccess) return P->hip_fail(e, "pool create");
     return SGC_OK;

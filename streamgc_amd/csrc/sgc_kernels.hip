@@ -176,6 +176,104 @@ __global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__
     }
 }
 
+// The same worklist for a launch over one graph: each thread reads the marks
+// of 4 vertices as one word (mark arrays are padded to a multiple of 4), with
+// no per-vertex branches, so a wavefront keeps all its loads in flight.  Each
+// block takes a contiguous run of words, counts its hits, reserves its place
+// in each list end with one atomic, then lists and clears them.
+constexpr uint32_t LIST_WORDS = 4;   // mark words per thread per tile
+
+struct ListTile {
+    uint32_t small[LIST_WORDS], big[LIST_WORDS];   // 4-bit vertex masks per word
+};
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
+    return ((x & 0xFFu) != 0) | (((x >> 8) & 0xFFu) != 0) << 1 | (((x >> 16) & 0xFFu) != 0) << 2 |
+           (((x >> 24) & 0xFFu) != 0) << 3;
+}
+__device__ __forceinline__ void list_tile(const GraphDev& g, uint32_t kind, uint32_t long_row, uint32_t tile,
+                                          uint32_t words, bool clear, ListTile& out) {
+    const SGC_G uint32_t* mk = (const SGC_G uint32_t*)(gp(g.dirty) + (size_t)kind * g.dstride);
+    const SGC_G uint32_t* sk = (const SGC_G uint32_t*)(gp(g.dirty) + (size_t)M_STICKY * g.dstride);
+    uint32_t m[LIST_WORDS], h[LIST_WORDS];
+#pragma unroll
+    for (uint32_t k = 0; k < LIST_WORDS; k++) {
+        const uint32_t w = tile * (BLOCK * LIST_WORDS) + k * BLOCK + threadIdx.x;
+        m[k] = w < words ? mk[w] : 0u;
+        h[k] = w < words ? sk[w] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < LIST_WORDS; k++) {
+        const uint32_t w = tile * (BLOCK * LIST_WORDS) + k * BLOCK + threadIdx.x;
+        uint32_t hit = nonzero_bytes(m[k] | h[k]);
+        if (4 * w + 4 > g.n) hit &= (1u << (g.n > 4 * w ? g.n - 4 * w : 0u)) - 1u;   // padding bytes
+        uint32_t big = 0;
+        if (hit) {   // fill is allocated to a multiple of 4 vertices
+            const uint4 f = *reinterpret_cast<const uint4*>(g.fill + 4 * (size_t)w);
+            big = ((f.x > long_row) | (f.y > long_row) << 1 | (f.z > long_row) << 2 | (f.w > long_row) << 3) & hit;
+        }
+        out.small[k] = hit & ~big;
+        out.big[k] = big;
+        if (clear && m[k]) ((SGC_G uint32_t*)mk)[w] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_build_list1(const GraphDev* __restrict__ graphs, Space sp, uint32_t kind,
+                                                     uint32_t long_row, uint32_t* __restrict__ list, uint32_t cap,
+                                                     uint32_t* __restrict__ count,
+                                                     const uint32_t* __restrict__ active,
+                                                     const uint32_t* __restrict__ gate) {
+    __shared__ uint32_t block_at[2];
+    if (gate && *gate == 0) return;
+    const uint32_t gid = sp.gid ? sp.gid[0] : 0u;
+    const uint32_t off = sp.base ? sp.base[0] : 0u;   // virtual index of the graph's vertex 0
+    if (active && active[gid] == 0) return;
+    const GraphDev g = graphs[gid];
+    const uint32_t words = (g.n + 3) / 4;
+    const uint32_t tiles = (words + BLOCK * LIST_WORDS - 1) / (BLOCK * LIST_WORDS);
+    const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t t0 = blockIdx.x * per, t1 = t0 + per < tiles ? t0 + per : tiles;
+    if (t0 >= t1) return;
+    ListTile lt;
+    uint32_t ns = 0, nb = 0;
+    for (uint32_t t = t0; t < t1; t++) {
+        list_tile(g, kind, long_row, t, words, false, lt);
+#pragma unroll
+        for (uint32_t k = 0; k < LIST_WORDS; k++) {
+            ns += __builtin_popcount(lt.small[k]);
+            nb += __builtin_popcount(lt.big[k]);
+        }
+    }
+    const uint32_t ts = block_sum32(ns), tb = block_sum32(nb);
+    if (threadIdx.x == 0) {
+        block_at[0] = ts ? add32(&count[0], ts) : 0u;
+        block_at[1] = tb ? add32(&count[1], tb) : 0u;
+    }
+    __syncthreads();
+    uint32_t at = block_at[0], bt = block_at[1];
+    for (uint32_t t = t0; t < t1; t++) {
+        list_tile(g, kind, long_row, t, words, true, lt);
+        uint32_t cs = 0, cb = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < LIST_WORDS; k++) {
+            cs += __builtin_popcount(lt.small[k]);
+            cb += __builtin_popcount(lt.big[k]);
+        }
+        uint32_t tot_s, tot_b;
+        uint32_t ps = at + block_scan32(cs, &tot_s);
+        uint32_t pb = bt + block_scan32(cb, &tot_b);
+#pragma unroll
+        for (uint32_t k = 0; k < LIST_WORDS; k++) {
+            const uint32_t v0 = off + 4 * (t * (BLOCK * LIST_WORDS) + k * BLOCK + threadIdx.x);
+            for (uint32_t q = 0; q < 4; q++) {
+                if ((lt.small[k] >> q) & 1u) list[ps++] = v0 + q;
+                if ((lt.big[k] >> q) & 1u) list[cap - 1 - pb++] = v0 + q;
+            }
+        }
+        at += tot_s;
+        bt += tot_b;
+    }
+}
+
 __global__ void k_zero_list_count(uint32_t* p) {
     p[0] = 0;
     p[1] = 0;
@@ -209,54 +307,74 @@ __device__ __forceinline__ uint32_t key_hash(uint64_t k, uint32_t mask) {
 }
 
 __global__ __launch_bounds__(256) void k_group_clear(uint64_t* __restrict__ keys, uint32_t* __restrict__ head,
-                                                     uint32_t size) {
+                                                     uint32_t size, uint32_t* __restrict__ nheads) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nheads = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += gridDim.x * blockDim.x) {
         gp(keys)[i] = KEY_EMPTY;
         gp(head)[i] = NONE;
     }
 }
 
-// Slot of `key` (inserted when absent).  The table holds at least twice the
-// batch, so a free slot always exists and the probe ends.
-__device__ __forceinline__ uint32_t group_slot(uint64_t* __restrict__ keys, uint64_t key, uint32_t mask) {
+// Slot of `key` (inserted when absent; *created tells whether this call
+// inserted it).  The table holds at least twice the batch, so a free slot
+// always exists and the probe ends.
+__device__ __forceinline__ uint32_t group_slot(uint64_t* __restrict__ keys, uint64_t key, uint32_t mask,
+                                               bool* created) {
     uint32_t s = key_hash(key, mask);
     for (uint32_t probes = 0; probes <= mask; s = (s + 1) & mask, probes++) {
         uint64_t found = KEY_EMPTY;
         __hip_atomic_compare_exchange_strong(gp(&keys[s]), &found, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
+        *created = found == KEY_EMPTY;
         if (found == KEY_EMPTY || found == key) return s;
     }
     return NONE;
 }
-__device__ __forceinline__ uint32_t group_find(const uint64_t* __restrict__ keys, uint64_t key, uint32_t mask) {
-    uint32_t s = key_hash(key, mask);
-    for (uint32_t probes = 0; probes <= mask; s = (s + 1) & mask, probes++) {
-        const uint64_t k = gp(keys)[s];
-        if (k == key) return s;
-        if (k == KEY_EMPTY) return NONE;
-    }
-    return NONE;
-}
+
+// Links every update of the batch into its edge's list, and lists each edge
+// once (the update that inserted the key adds its slot; an update whose key
+// found no slot is listed alone, position | GROUP_SINGLE): the per-update
+// kernel then runs one wavefront per edge.  Records that change nothing
+// (out-of-range vertex, other kinds) are not listed.  Each block appends its
+// entries with one atomic.
+constexpr uint32_t GROUP_SINGLE = 0x80000000u;
 
 __global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__ graphs,
                                                     const uint32_t* __restrict__ gbase,
                                                     const uint4* __restrict__ upd, uint32_t count, int multi,
                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ head,
                                                     uint32_t* __restrict__ next, uint32_t mask,
+                                                    uint32_t* __restrict__ heads, uint32_t* __restrict__ nheads,
                                                     const uint32_t* __restrict__ dyn) {
+    __shared__ uint32_t block_at;
     if (dyn) {   // the batch is a slice of a device-resident stream: [offset, count] read here
         upd += dyn[0];
         count = dyn[1];
     }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
-        uint32_t prev = NONE;
-        if (key != KEY_EMPTY) {
-            const uint32_t s = group_slot(keys, key, mask);
-            if (s != NONE)
-                prev = __hip_atomic_exchange(gp(&head[s]), i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < count; b0 += gridDim.x * blockDim.x) {
+        const uint32_t i = b0 + threadIdx.x;
+        uint32_t entry = NONE;
+        if (i < count) {
+            const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
+            uint32_t prev = NONE;
+            if (key != KEY_EMPTY) {
+                bool created = false;
+                const uint32_t s = group_slot(keys, key, mask, &created);
+                if (s != NONE) {
+                    prev = __hip_atomic_exchange(gp(&head[s]), i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (created) entry = s;
+                } else {
+                    entry = i | GROUP_SINGLE;
+                }
+            }
+            gp(next)[i] = prev;
         }
-        gp(next)[i] = prev;
+        uint32_t total;
+        const uint32_t at = block_scan32(entry != NONE ? 1u : 0u, &total);
+        if (threadIdx.x == 0) block_at = total ? add32(nheads, total) : 0u;
+        __syncthreads();
+        if (entry != NONE) gp(heads)[block_at + at] = entry;
+        __syncthreads();
     }
 }
 
@@ -271,47 +389,45 @@ __device__ __forceinline__ void apply_record(const GraphDev* __restrict__ graphs
 
 __global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__ graphs,
                                                     const uint4* __restrict__ upd, uint32_t count, int multi,
-                                                    const uint32_t* __restrict__ gbase,
-                                                    const uint64_t* __restrict__ gkeys,
                                                     const uint32_t* __restrict__ ghead,
-                                                    const uint32_t* __restrict__ gnext, uint32_t gmask,
-                                                    uint64_t* totals, const uint32_t* __restrict__ dyn) {
+                                                    const uint32_t* __restrict__ gnext,
+                                                    const uint32_t* __restrict__ heads,
+                                                    const uint32_t* __restrict__ nheads, uint64_t* totals,
+                                                    const uint32_t* __restrict__ dyn) {
     WaveAcc acc;
     uint64_t grouped = 0;
     if (dyn) {
         upd += dyn[0];
         count = dyn[1];
     }
-    for (uint32_t i = wave_id(); i < count; i += wave_count()) {
-        // the updates this wave applies for position i, in stream order: i
-        // alone, or the whole group of i's edge when this wave owns its list
-        uint32_t h = NONE;
-        if (gkeys) {
-            const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
-            const uint32_t s = key == KEY_EMPTY ? NONE : group_find(gkeys, key, gmask);
-            if (s != NONE) {
-                h = gp(ghead)[s];
-                if (h != i) continue;          // the list head's wave applies this group
-            }
+    if (!heads) {   // no grouping: one wavefront per position
+        for (uint32_t i = wave_id(); i < count; i += wave_count()) apply_record(graphs, upd, i, multi, acc);
+        acc.flush(totals);
+        return;
+    }
+    // One wavefront per edge of the batch: its updates in stream order.  The
+    // edge's list (head, then next links) holds its positions; take
+    // repeatedly the smallest position above the last one applied (groups
+    // are short).
+    const uint32_t n = *nheads;
+    for (uint32_t j = wave_id(); j < n; j += wave_count()) {
+        const uint32_t e = heads[j];
+        if (e & GROUP_SINGLE) {
+            apply_record(graphs, upd, e & ~GROUP_SINGLE, multi, acc);
+            continue;
         }
-        // Take repeatedly the smallest position above the last one applied
-        // (groups are short).
-        uint32_t last = 0;
-        bool first = true;
+        const uint32_t h = gp(ghead)[e];
+        uint32_t last = 0, applied = 0;
         for (;;) {
             uint32_t best = NONE;
-            if (h == NONE) {
-                best = first ? i : NONE;
-            } else {
-                for (uint32_t k = h; k != NONE; k = gp(gnext)[k])
-                    if ((first || k > last) && (best == NONE || k < best)) best = k;
-            }
+            for (uint32_t k = h; k != NONE; k = gp(gnext)[k])
+                if ((applied == 0 || k > last) && (best == NONE || k < best)) best = k;
             if (best == NONE) break;
             apply_record(graphs, upd, best, multi, acc);
-            grouped += best != i;
             last = best;
-            first = false;
+            applied++;
         }
+        grouped += applied - 1;
     }
     acc.flush(totals);
     if (grouped && totals && lane() == 0) tot_add(totals, T_GROUPED, grouped);
@@ -1407,19 +1523,18 @@ static inline uint32_t grid_for_waves(uint64_t waves, uint32_t waves_per_block, 
 
 hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg) {
     if (count == 0) return hipSuccess;
-    const uint64_t* keys = nullptr;
-    const uint32_t *head = nullptr, *next = nullptr;
-    uint32_t mask = 0;
+    const uint32_t *head = nullptr, *next = nullptr, *heads = nullptr, *nheads = nullptr;
     if (eg && eg->keys && count > 1) {
-        if ((uint64_t)eg->size < 2ull * count || eg->next_cap < count) return hipErrorInvalidValue;
-        keys = eg->keys;
+        if ((uint64_t)eg->size < 2ull * count || eg->next_cap < count || !eg->heads || !eg->nheads)
+            return hipErrorInvalidValue;
         head = eg->head;
         next = eg->next;
-        mask = eg->size - 1;
+        heads = eg->heads;
+        nheads = eg->nheads;
     }
     uint32_t grid = grid_for_waves(count, 4, L.max_blocks);
-    k_apply_elem<<<grid, 256, 0, L.stream>>>(L.graphs, (const uint4*)upd, count, multi, L.gbase, keys, head, next,
-                                              mask, L.totals, L.dyn);
+    k_apply_elem<<<grid, 256, 0, L.stream>>>(L.graphs, (const uint4*)upd, count, multi, head, next, heads, nheads,
+                                              L.totals, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg) {
@@ -1427,10 +1542,10 @@ hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int
     if ((uint64_t)eg->size < 2ull * count || eg->next_cap < count) return hipErrorInvalidValue;
     const uint32_t mask = eg->size - 1;
     uint32_t cg = grid_for_waves(eg->size, 4 * WAVE, 1024);
-    k_group_clear<<<cg, 256, 0, L.stream>>>(eg->keys, eg->head, eg->size);
+    k_group_clear<<<cg, 256, 0, L.stream>>>(eg->keys, eg->head, eg->size, eg->nheads);
     uint32_t mg = grid_for_waves(count, 4 * WAVE, 1024);
     k_group_link<<<mg, 256, 0, L.stream>>>(L.graphs, L.gbase, (const uint4*)upd, count, multi, eg->keys, eg->head,
-                                           eg->next, mask, L.dyn);
+                                           eg->next, mask, eg->heads, eg->nheads, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg,
@@ -1464,6 +1579,12 @@ static inline SweepList sweep_list(const LaunchCtx& L) { return SweepList{L.list
 hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count) {
     if (kind >= M_STICKY || cap < L.V) return hipErrorInvalidValue;
     k_zero_list_count<<<1, 1, 0, L.stream>>>(count);
+    if (L.ns == 1) {   // one graph: vertices of its own index space, 4 per mark word
+        const uint32_t tiles = (L.V / 4 + BLOCK * LIST_WORDS) / (BLOCK * LIST_WORDS);
+        k_build_list1<<<tiles < 512 ? tiles : 512, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, list,
+                                                                      cap, count, L.active, L.gate);
+        return hipGetLastError();
+    }
     uint32_t grid = grid_for_waves((L.V + WAVE * LIST_TILE - 1) / (WAVE * LIST_TILE), BLOCK / WAVE, 4096);
     k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, list, cap, count, L.active,
                                              L.gate);

@@ -34,11 +34,14 @@ struct LaunchCtx {
 };
 
 // Per-edge grouping of a parallel batch: hash table (size: power of two >=
-// 2 x batch) of edge keys with list heads, and a next link per update.
+// 2 x batch) of edge keys with list heads, a next link per update, and the
+// list of the batch's edges.
 struct EdgeGroups {
     uint64_t* keys = nullptr;
     uint32_t* head = nullptr;
     uint32_t* next = nullptr;
+    uint32_t* heads = nullptr;    // next_cap entries: one per edge of the batch (slot, or position | 1 << 31)
+    uint32_t* nheads = nullptr;   // device word: entries in heads
     uint32_t size = 0;
     uint32_t next_cap = 0;
 };
