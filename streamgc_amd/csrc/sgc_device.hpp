@@ -624,8 +624,10 @@ __device__ __forceinline__ void color_fresh(const GraphDev& g, uint32_t x, uint3
 // the one read after them unless another wave recoloured in between, which
 // is one more interleaving the parallel mode allows anyway.
 //
-// Marks.  A delete leaves a hole (ROW) and a stale bitmap bit (BITS); it
-// cannot give either endpoint a conflict.  An insert that stored the edge in
+// Marks.  A delete that removed the neighbour leaves a hole (ROW) and, in a
+// bitmap row, a stale bit (BITS); a histogram row stays exact unless the edge
+// was not in it (its count then drops for a colour nobody left, BITS).  A
+// delete cannot give either endpoint a conflict.  An insert that stored the edge in
 // both rows and then read the colours leaves both bitmaps and histograms
 // exact (a recolour of an endpoint after those reads sees the new slot and
 // tells the other endpoint itself), and once it has resolved a same-colour
@@ -640,12 +642,9 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
     const uint32_t su = gp(g.row_ptr)[u], eu = gp(g.row_ptr)[u + 1];
     const uint32_t sv = gp(g.row_ptr)[v], ev = gp(g.row_ptr)[v + 1];
     const uint32_t hu = gp(g.hd_map)[u], hv = gp(g.hd_map)[v];
-    if (lane() == 0) {
-        const uint32_t bits = (type == 1 ? D_ROW | D_BITS : 0u) | ((g.flags & ABL_LIVE_DEGREE) ? D_STATUS : 0u);
-        if (bits) {
-            mark(g, u, bits);
-            mark(g, v, bits);
-        }
+    if ((g.flags & ABL_LIVE_DEGREE) && lane() == 0) {
+        mark(g, u, D_STATUS);
+        mark(g, v, D_STATUS);
     }
     const bool two = u != v;   // a self-loop's two rows are one: scanned one after the other
     uint32_t pu, pv, fu, fv, d0, d1;
@@ -696,6 +695,10 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
         scan_pair<false>(g, true, u, v, su, tu, two, v, u, sv, tv, pu, pv, fu, fv, hadu, hadv);
         if (!two) scan_pair<false>(g, false, u, v, su, tu, true, v, u, sv, tv, d0, pv, d1, fv, h0, hadv);
         acc.scan += (pu ? pu : tu - su) + (pv ? pv : tv - sv);
+        if (lane() == 0) {
+            mark(g, u, (pu ? D_ROW : 0u) | ((hu == INVALID) == (pu != 0) ? D_BITS : 0u));
+            mark(g, v, (pv ? D_ROW : 0u) | ((hv == INVALID) == (pv != 0) ? D_BITS : 0u));
+        }
         on_removed_neighbor(g, u, hu, cv);
         on_removed_neighbor(g, v, hv, cu);
         try_reduce(g, u, cv, cu, acc);
