@@ -1631,7 +1631,7 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipMalloc(&P->eg.head, (size_t)P->eg.size * 4);
     if (e == hipSuccess) e = hipMalloc(&P->eg.next, (size_t)batch_cap * 4);
     if (e == hipSuccess) e = hipMalloc(&P->eg.heads, (size_t)batch_cap * 4);
-    if (e == hipSuccess) e = hipMalloc(&P->eg.nheads, 4);
+    if (e == hipSuccess) e = hipMalloc(&P->eg.nheads, 8);
     if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
     if (e != hipSuccess) return P->hip_fail(e, "pool create");
     return SGC_OK;

@@ -308,7 +308,7 @@ __device__ __forceinline__ uint32_t key_hash(uint64_t k, uint32_t mask) {
 
 __global__ __launch_bounds__(256) void k_group_clear(uint64_t* __restrict__ keys, uint32_t* __restrict__ head,
                                                      uint32_t size, uint32_t* __restrict__ nheads) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *nheads = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 2) nheads[threadIdx.x] = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += gridDim.x * blockDim.x) {
         gp(keys)[i] = KEY_EMPTY;
         gp(head)[i] = NONE;
@@ -335,18 +335,22 @@ __device__ __forceinline__ uint32_t group_slot(uint64_t* __restrict__ keys, uint
 // once (the update that inserted the key adds its slot; an update whose key
 // found no slot is listed alone, position | GROUP_SINGLE): the per-update
 // kernel then runs one wavefront per edge.  Records that change nothing
-// (out-of-range vertex, other kinds) are not listed.  Each block appends its
-// entries with one atomic.
+// (out-of-range vertex, other kinds) are not listed.  Edges with an endpoint
+// row filled past HEAVY_FILL slots (the long scans) go to the front of the
+// list, the others to the back, so that the long ones start first and the
+// short ones fill in behind them.  Each block appends its entries with one
+// atomic per list end; nheads = [front, back] counts.
 constexpr uint32_t GROUP_SINGLE = 0x80000000u;
+constexpr uint32_t HEAVY_FILL = 1024;
 
 __global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__ graphs,
                                                     const uint32_t* __restrict__ gbase,
                                                     const uint4* __restrict__ upd, uint32_t count, int multi,
                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ head,
                                                     uint32_t* __restrict__ next, uint32_t mask,
-                                                    uint32_t* __restrict__ heads, uint32_t* __restrict__ nheads,
-                                                    const uint32_t* __restrict__ dyn) {
-    __shared__ uint32_t block_at;
+                                                    uint32_t* __restrict__ heads, uint32_t cap,
+                                                    uint32_t* __restrict__ nheads, const uint32_t* __restrict__ dyn) {
+    __shared__ uint32_t block_at[2];
     if (dyn) {   // the batch is a slice of a device-resident stream: [offset, count] read here
         upd += dyn[0];
         count = dyn[1];
@@ -354,8 +358,10 @@ __global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < count; b0 += gridDim.x * blockDim.x) {
         const uint32_t i = b0 + threadIdx.x;
         uint32_t entry = NONE;
+        bool heavy = false;
         if (i < count) {
-            const uint64_t key = edge_key(graphs, gbase, upd[i], multi);
+            const uint4 r = upd[i];
+            const uint64_t key = edge_key(graphs, gbase, r, multi);
             uint32_t prev = NONE;
             if (key != KEY_EMPTY) {
                 bool created = false;
@@ -366,14 +372,22 @@ __global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__
                 } else {
                     entry = i | GROUP_SINGLE;
                 }
+                if (entry != NONE) {
+                    const GraphDev& g = graphs[multi ? r.w : 0u];
+                    heavy = gp(g.fill)[r.x] > HEAVY_FILL || gp(g.fill)[r.y] > HEAVY_FILL;
+                }
             }
             gp(next)[i] = prev;
         }
-        uint32_t total;
-        const uint32_t at = block_scan32(entry != NONE ? 1u : 0u, &total);
-        if (threadIdx.x == 0) block_at = total ? add32(nheads, total) : 0u;
+        uint32_t nf, nb;
+        const uint32_t af = block_scan32(entry != NONE && heavy ? 1u : 0u, &nf);
+        const uint32_t ab = block_scan32(entry != NONE && !heavy ? 1u : 0u, &nb);
+        if (threadIdx.x == 0) {
+            block_at[0] = nf ? add32(&nheads[0], nf) : 0u;
+            block_at[1] = nb ? add32(&nheads[1], nb) : 0u;
+        }
         __syncthreads();
-        if (entry != NONE) gp(heads)[block_at + at] = entry;
+        if (entry != NONE) gp(heads)[heavy ? block_at[0] + af : cap - 1 - (block_at[1] + ab)] = entry;
         __syncthreads();
     }
 }
@@ -387,11 +401,11 @@ __device__ __forceinline__ void apply_record(const GraphDev* __restrict__ graphs
     apply_update(g, r.x, r.y, r.z & 0xFFu, acc);
 }
 
-__global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__ graphs,
+__global__ __launch_bounds__(64) void k_apply_elem(const GraphDev* __restrict__ graphs,
                                                     const uint4* __restrict__ upd, uint32_t count, int multi,
                                                     const uint32_t* __restrict__ ghead,
                                                     const uint32_t* __restrict__ gnext,
-                                                    const uint32_t* __restrict__ heads,
+                                                    const uint32_t* __restrict__ heads, uint32_t cap,
                                                     const uint32_t* __restrict__ nheads, uint64_t* totals,
                                                     const uint32_t* __restrict__ dyn) {
     WaveAcc acc;
@@ -409,9 +423,9 @@ __global__ __launch_bounds__(256) void k_apply_elem(const GraphDev* __restrict__
     // edge's list (head, then next links) holds its positions; take
     // repeatedly the smallest position above the last one applied (groups
     // are short).
-    const uint32_t n = *nheads;
+    const uint32_t nf = nheads[0], n = nf + nheads[1];
     for (uint32_t j = wave_id(); j < n; j += wave_count()) {
-        const uint32_t e = heads[j];
+        const uint32_t e = j < nf ? heads[j] : heads[cap - 1 - (j - nf)];   // long scans first
         if (e & GROUP_SINGLE) {
             apply_record(graphs, upd, e & ~GROUP_SINGLE, multi, acc);
             continue;
@@ -1532,9 +1546,10 @@ hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int
         heads = eg->heads;
         nheads = eg->nheads;
     }
-    uint32_t grid = grid_for_waves(count, 4, L.max_blocks);
-    k_apply_elem<<<grid, 256, 0, L.stream>>>(L.graphs, (const uint4*)upd, count, multi, head, next, heads, nheads,
-                                              L.totals, L.dyn);
+    // one-wavefront blocks: a wavefront's slot is refilled as soon as it ends
+    uint32_t grid = grid_for_waves(count, 1, 4 * L.max_blocks);
+    k_apply_elem<<<grid, WAVE, 0, L.stream>>>(L.graphs, (const uint4*)upd, count, multi, head, next, heads,
+                                               eg ? eg->next_cap : 0u, nheads, L.totals, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg) {
@@ -1545,7 +1560,7 @@ hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int
     k_group_clear<<<cg, 256, 0, L.stream>>>(eg->keys, eg->head, eg->size, eg->nheads);
     uint32_t mg = grid_for_waves(count, 4 * WAVE, 1024);
     k_group_link<<<mg, 256, 0, L.stream>>>(L.graphs, L.gbase, (const uint4*)upd, count, multi, eg->keys, eg->head,
-                                           eg->next, mask, eg->heads, eg->nheads, L.dyn);
+                                           eg->next, mask, eg->heads, eg->next_cap, eg->nheads, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg,

@@ -40,8 +40,9 @@ struct EdgeGroups {
     uint64_t* keys = nullptr;
     uint32_t* head = nullptr;
     uint32_t* next = nullptr;
-    uint32_t* heads = nullptr;    // next_cap entries: one per edge of the batch (slot, or position | 1 << 31)
-    uint32_t* nheads = nullptr;   // device word: entries in heads
+    uint32_t* heads = nullptr;    // next_cap entries: one per edge of the batch (slot, or position | 1 << 31),
+                                  // long scans from the front, the others from the back
+    uint32_t* nheads = nullptr;   // device words: entries at the front, at the back
     uint32_t size = 0;
     uint32_t next_cap = 0;
 };
