@@ -318,10 +318,19 @@ __device__ __forceinline__ void on_neighbor_recolor(const GraphDev& g, uint32_t 
 // overflowed edge's (sticky, and counted in C_ASYM).
 __device__ __forceinline__ void flag_duplicate(const GraphDev& g, uint32_t x, uint32_t nb, uint32_t from,
                                                uint32_t end) {
+    constexpr uint32_t CH = 2 * ROW_UNROLL;   // 64-slot chunks per round trip
     bool dup = false;
-    for (uint32_t base = from; base < end && !dup; base += WAVE) {
-        const uint32_t j = base + lane();
-        dup = ballot(j < end && ld(&g.col[j]) == nb) != 0;
+    for (uint32_t base = from; base < end && !dup; base += CH * WAVE) {
+        uint32_t val[CH];
+#pragma unroll
+        for (uint32_t q = 0; q < CH; q++) {
+            const uint32_t j = base + q * WAVE + lane();
+            val[q] = j < end ? ld(&g.col[j]) : INVALID;
+        }
+        bool hit = false;
+#pragma unroll
+        for (uint32_t q = 0; q < CH; q++) hit |= val[q] == nb;
+        dup = ballot(hit) != 0;
     }
     if (dup && lane() == 0) {
         add32(&g.ctr[C_ASYM], 1u);
@@ -526,10 +535,11 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
 // An insert into row x (slots [s, e)) after its scan: a slot written in a hole
 // may duplicate a copy further along; no slot at all is an overflow event.
 // Returns the slots scanned (the position found, or the whole row).
+// sticky: both endpoints are sticky already (a duplicate would change nothing).
 __device__ __forceinline__ uint32_t finish_insert(const GraphDev& g, uint32_t x, uint32_t nb, uint32_t s, uint32_t e,
-                                                  uint32_t pos, uint32_t pf, WaveAcc& acc) {
+                                                  uint32_t pos, uint32_t pf, bool sticky, WaveAcc& acc) {
     if (pos) {
-        if (pf > pos && nb < g.n) flag_duplicate(g, x, nb, s + pos, s + pf);
+        if (pf > pos && nb < g.n && !sticky) flag_duplicate(g, x, nb, s + pos, s + pf);
         return pos;
     }
     if (lane() == 0) {
@@ -642,6 +652,8 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
     const uint32_t su = gp(g.row_ptr)[u], eu = gp(g.row_ptr)[u + 1];
     const uint32_t sv = gp(g.row_ptr)[v], ev = gp(g.row_ptr)[v + 1];
     const uint32_t hu = gp(g.hd_map)[u], hv = gp(g.hd_map)[v];
+    const SGC_G uint8_t* sticky = g.dirty ? gp(g.dirty) + (size_t)M_STICKY * g.dstride : nullptr;
+    const bool stu = sticky && sticky[u], stv = sticky && sticky[v];
     if ((g.flags & ABL_LIVE_DEGREE) && lane() == 0) {
         mark(g, u, D_STATUS);
         mark(g, v, D_STATUS);
@@ -652,8 +664,9 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
     if (type == 0) {
         scan_pair<true>(g, true, u, v, su, eu, two, v, u, sv, ev, pu, pv, fu, fv, hadu, hadv);
         if (!two) scan_pair<true>(g, false, u, v, su, eu, true, v, u, sv, ev, d0, pv, d1, fv, h0, hadv);
-        uint32_t p = finish_insert(g, u, v, su, eu, pu, fu, acc);
-        p += finish_insert(g, v, u, sv, ev, pv, fv, acc);
+        const bool both_sticky = stu && stv;
+        uint32_t p = finish_insert(g, u, v, su, eu, pu, fu, both_sticky, acc);
+        p += finish_insert(g, v, u, sv, ev, pv, fv, both_sticky, acc);
         if (lane() == 0) {
             if (hadu) mark(g, u, D_BITS);
             if (hadv) mark(g, v, D_BITS);
