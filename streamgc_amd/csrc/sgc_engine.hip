@@ -388,7 +388,7 @@ struct sgc_pool {
             if (d_list) (void)hipFree(d_list);
             d_list = nullptr;
             list_cap = 0;
-            if (e == hipSuccess) e = hipMalloc((void**)&d_list, (size_t)V * 4);
+            if (e == hipSuccess) e = hipMalloc((void**)&d_list, list_words(V) * 4);
             if (e == hipSuccess) list_cap = V;
         }
         return e;

@@ -176,12 +176,18 @@ __global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__
     }
 }
 
-// The same worklist for a launch over one graph: each thread reads the marks
-// of 4 vertices as one word (mark arrays are padded to a multiple of 4), with
-// no per-vertex branches, so a wavefront keeps all its loads in flight.  Each
-// block takes a contiguous run of words, counts its hits, reserves its place
-// in each list end with one atomic, then lists and clears them.
-constexpr uint32_t LIST_WORDS = 4;   // mark words per thread per tile
+// The same worklist for a launch over one graph, in two passes with no
+// shared counter.  Each thread reads the marks of 4 vertices as one word
+// (mark arrays are padded to a multiple of 4) with no per-vertex branches.
+// Pass 1: each block takes a tile of LIST_TILE_V vertices, lists its hits in
+// the tile's own region of a scratch array (short rows up from the start,
+// long rows down from the end), clears their marks and stores its two
+// counts.  Pass 2: each block adds up the counts of the tiles before it and
+// copies its hits to their places in the worklist; the last block stores the
+// totals.
+constexpr uint32_t LIST_WORDS = 4;                          // mark words per thread
+constexpr uint32_t LIST_TILE_V = BLOCK * LIST_WORDS * 4;    // vertices per tile
+static_assert(LIST_TILE_V == 4096, "list_words() sizes the scratch for 4096-vertex tiles");
 
 struct ListTile {
     uint32_t small[LIST_WORDS], big[LIST_WORDS];   // 4-bit vertex masks per word
@@ -191,8 +197,8 @@ __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
            (((x >> 24) & 0xFFu) != 0) << 3;
 }
 __device__ __forceinline__ void list_tile(const GraphDev& g, uint32_t kind, uint32_t long_row, uint32_t tile,
-                                          uint32_t words, bool clear, ListTile& out) {
-    const SGC_G uint32_t* mk = (const SGC_G uint32_t*)(gp(g.dirty) + (size_t)kind * g.dstride);
+                                          uint32_t words, ListTile& out) {
+    SGC_G uint32_t* mk = (SGC_G uint32_t*)(gp(g.dirty) + (size_t)kind * g.dstride);
     const SGC_G uint32_t* sk = (const SGC_G uint32_t*)(gp(g.dirty) + (size_t)M_STICKY * g.dstride);
     uint32_t m[LIST_WORDS], h[LIST_WORDS];
 #pragma unroll
@@ -213,64 +219,74 @@ __device__ __forceinline__ void list_tile(const GraphDev& g, uint32_t kind, uint
         }
         out.small[k] = hit & ~big;
         out.big[k] = big;
-        if (clear && m[k]) ((SGC_G uint32_t*)mk)[w] = 0u;
+        if (m[k]) mk[w] = 0u;
     }
 }
 
-__global__ __launch_bounds__(256) void k_build_list1(const GraphDev* __restrict__ graphs, Space sp, uint32_t kind,
-                                                     uint32_t long_row, uint32_t* __restrict__ list, uint32_t cap,
-                                                     uint32_t* __restrict__ count,
-                                                     const uint32_t* __restrict__ active,
-                                                     const uint32_t* __restrict__ gate) {
-    __shared__ uint32_t block_at[2];
+__global__ __launch_bounds__(256) void k_list_tiles(const GraphDev* __restrict__ graphs, Space sp, uint32_t kind,
+                                                    uint32_t long_row, uint32_t* __restrict__ tmp,
+                                                    uint32_t* __restrict__ tcount,
+                                                    const uint32_t* __restrict__ active,
+                                                    const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0) return;
     const uint32_t gid = sp.gid ? sp.gid[0] : 0u;
     const uint32_t off = sp.base ? sp.base[0] : 0u;   // virtual index of the graph's vertex 0
     if (active && active[gid] == 0) return;
     const GraphDev g = graphs[gid];
-    const uint32_t words = (g.n + 3) / 4;
-    const uint32_t tiles = (words + BLOCK * LIST_WORDS - 1) / (BLOCK * LIST_WORDS);
-    const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
-    const uint32_t t0 = blockIdx.x * per, t1 = t0 + per < tiles ? t0 + per : tiles;
-    if (t0 >= t1) return;
+    const uint32_t words = (g.n + 3) / 4, t = blockIdx.x;
     ListTile lt;
-    uint32_t ns = 0, nb = 0;
-    for (uint32_t t = t0; t < t1; t++) {
-        list_tile(g, kind, long_row, t, words, false, lt);
+    list_tile(g, kind, long_row, t, words, lt);
+    uint32_t cs = 0, cb = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < LIST_WORDS; k++) {
-            ns += __builtin_popcount(lt.small[k]);
-            nb += __builtin_popcount(lt.big[k]);
+    for (uint32_t k = 0; k < LIST_WORDS; k++) {
+        cs += __builtin_popcount(lt.small[k]);
+        cb += __builtin_popcount(lt.big[k]);
+    }
+    uint32_t ts, tb;
+    uint32_t ps = block_scan32(cs, &ts);
+    uint32_t pb = block_scan32(cb, &tb);
+    SGC_G uint32_t* region = gp(tmp) + (size_t)t * LIST_TILE_V;
+#pragma unroll
+    for (uint32_t k = 0; k < LIST_WORDS; k++) {
+        const uint32_t v0 = off + 4 * (t * (BLOCK * LIST_WORDS) + k * BLOCK + threadIdx.x);
+        for (uint32_t q = 0; q < 4; q++) {
+            if ((lt.small[k] >> q) & 1u) region[ps++] = v0 + q;
+            if ((lt.big[k] >> q) & 1u) region[LIST_TILE_V - 1 - pb++] = v0 + q;
         }
     }
-    const uint32_t ts = block_sum32(ns), tb = block_sum32(nb);
     if (threadIdx.x == 0) {
-        block_at[0] = ts ? add32(&count[0], ts) : 0u;
-        block_at[1] = tb ? add32(&count[1], tb) : 0u;
+        gp(tcount)[2 * t] = ts;
+        gp(tcount)[2 * t + 1] = tb;
     }
-    __syncthreads();
-    uint32_t at = block_at[0], bt = block_at[1];
-    for (uint32_t t = t0; t < t1; t++) {
-        list_tile(g, kind, long_row, t, words, true, lt);
-        uint32_t cs = 0, cb = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < LIST_WORDS; k++) {
-            cs += __builtin_popcount(lt.small[k]);
-            cb += __builtin_popcount(lt.big[k]);
-        }
-        uint32_t tot_s, tot_b;
-        uint32_t ps = at + block_scan32(cs, &tot_s);
-        uint32_t pb = bt + block_scan32(cb, &tot_b);
-#pragma unroll
-        for (uint32_t k = 0; k < LIST_WORDS; k++) {
-            const uint32_t v0 = off + 4 * (t * (BLOCK * LIST_WORDS) + k * BLOCK + threadIdx.x);
-            for (uint32_t q = 0; q < 4; q++) {
-                if ((lt.small[k] >> q) & 1u) list[ps++] = v0 + q;
-                if ((lt.big[k] >> q) & 1u) list[cap - 1 - pb++] = v0 + q;
-            }
-        }
-        at += tot_s;
-        bt += tot_b;
+}
+
+__global__ __launch_bounds__(256) void k_list_gather(const GraphDev* __restrict__ graphs, Space sp,
+                                                     const uint32_t* __restrict__ tmp,
+                                                     const uint32_t* __restrict__ tcount,
+                                                     uint32_t* __restrict__ list, uint32_t cap,
+                                                     uint32_t* __restrict__ count,
+                                                     const uint32_t* __restrict__ active,
+                                                     const uint32_t* __restrict__ gate) {
+    const uint32_t gid = sp.gid ? sp.gid[0] : 0u;
+    if ((gate && *gate == 0) || (active && active[gid] == 0)) {   // nothing listed
+        if (blockIdx.x == 0 && threadIdx.x < 2) count[threadIdx.x] = 0;
+        return;
+    }
+    const uint32_t t = blockIdx.x;
+    uint32_t bs = 0, bb = 0;
+    for (uint32_t k = threadIdx.x; k < t; k += BLOCK) {
+        bs += gp(tcount)[2 * k];
+        bb += gp(tcount)[2 * k + 1];
+    }
+    bs = block_sum32(bs);
+    bb = block_sum32(bb);
+    const uint32_t ns = gp(tcount)[2 * t], nb = gp(tcount)[2 * t + 1];
+    const SGC_G uint32_t* region = gp(tmp) + (size_t)t * LIST_TILE_V;
+    for (uint32_t k = threadIdx.x; k < ns; k += BLOCK) gp(list)[bs + k] = region[k];
+    for (uint32_t k = threadIdx.x; k < nb; k += BLOCK) gp(list)[cap - 1 - (bb + k)] = region[LIST_TILE_V - 1 - k];
+    if (t == gridDim.x - 1 && threadIdx.x == 0) {
+        count[0] = bs + ns;
+        count[1] = bb + nb;
     }
 }
 
@@ -1593,13 +1609,17 @@ static inline uint32_t sweep_grid(const LaunchCtx& L) {
 static inline SweepList sweep_list(const LaunchCtx& L) { return SweepList{L.list, L.list_count, L.list_cap}; }
 hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count) {
     if (kind >= M_STICKY || cap < L.V) return hipErrorInvalidValue;
-    k_zero_list_count<<<1, 1, 0, L.stream>>>(count);
-    if (L.ns == 1) {   // one graph: vertices of its own index space, 4 per mark word
-        const uint32_t tiles = (L.V / 4 + BLOCK * LIST_WORDS) / (BLOCK * LIST_WORDS);
-        k_build_list1<<<tiles < 512 ? tiles : 512, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, list,
-                                                                      cap, count, L.active, L.gate);
+    if (L.ns == 1) {   // one graph: two passes over tiles of its own vertices (scratch behind the list)
+        const uint32_t tiles = (L.V + LIST_TILE_V - 1) / LIST_TILE_V;
+        uint32_t* tmp = list + cap;
+        uint32_t* tcount = tmp + (size_t)tiles * LIST_TILE_V;
+        k_list_tiles<<<tiles, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, tmp, tcount, L.active,
+                                                  L.gate);
+        k_list_gather<<<tiles, 256, 0, L.stream>>>(L.graphs, space(L), tmp, tcount, list, cap, count, L.active,
+                                                   L.gate);
         return hipGetLastError();
     }
+    k_zero_list_count<<<1, 1, 0, L.stream>>>(count);
     uint32_t grid = grid_for_waves((L.V + WAVE * LIST_TILE - 1) / (WAVE * LIST_TILE), BLOCK / WAVE, 4096);
     k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, list, cap, count, L.active,
                                              L.gate);

@@ -57,7 +57,13 @@ hipError_t launch_max_color(const LaunchCtx& L);
 hipError_t launch_snapshot(const LaunchCtx& L);
 hipError_t launch_compact(const LaunchCtx& L);
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
+// Worklist of the vertices marked `kind` (or sticky).  `list` holds
+// list_words(cap) u32: the list, then scratch of the one-graph builder.
 hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count);
+inline size_t list_words(uint32_t cap) {
+    const size_t tiles = ((size_t)cap + 4095) / 4096;   // tiles of the one-graph builder
+    return (size_t)cap + tiles * 4096 + 2 * tiles;
+}
 hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist);
 // serial: vertex order, one wavefront per graph; fresh: parallel-mode colour
 // selection from the exact neighbour colours (0 none, 1 bitmap-mode rows,
