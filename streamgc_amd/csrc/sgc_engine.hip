@@ -177,6 +177,7 @@ struct BatchRun {
     uint32_t rounds = 0, ng = 0, max_count = 0, max_n = 0, sweeps = 0, ovf_cap = 0, t2_passes = 0;
     uint64_t updates = 0;
     bool multi = false, any_history = false, all_settled = true;
+    bool lean = false;                         // captures leave the Tier-2 check and repair out
     std::vector<uint32_t> tab;                 // per round: first record, record count
     std::vector<uint8_t> inround, full;        // rounds x ng: takes part; reaches its epoch boundary
     std::vector<uint32_t> counts;              // rounds x ng: records
@@ -1214,6 +1215,18 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
     for (uint32_t g = 0; g < ng; g++)
         if (gs[g].used) max_passes = std::max(max_passes, gs[g].cfg.tier2_passes);
     B->t2_passes = (flags & SGC_HOST_TIER2) ? 0 : std::min(c.t2_passes ? c.t2_passes : 4u, max_passes);
+    // No graph has ever had an edge in one row only: the Tier-2 check can only
+    // be needed after a Tier-1 loop that ended unclean or a new overflow, so
+    // the rounds are captured without it and stop for the host when it is.
+    {
+        std::vector<uint32_t> hc((size_t)ng * C_COUNT);
+        hipError_t ec = hipMemcpyAsync(hc.data(), d_ctr, hc.size() * 4, hipMemcpyDeviceToHost, stream);
+        if (ec == hipSuccess) ec = hipStreamSynchronize(stream);
+        if (ec != hipSuccess) return hip_fail(ec, "batch: counters");
+        B->lean = true;
+        for (uint32_t g = 0; g < ng; g++)
+            if (gs[g].used && hc[(size_t)g * C_COUNT + C_ASYM] != 0) B->lean = false;
+    }
     hipError_t e = hipMalloc((void**)&B->d_stream, std::max<size_t>(ordered.size(), 1) * sizeof(sgc_edge_update));
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_tab, std::max<size_t>(B->tab.size(), 1) * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_seg, std::max<size_t>(seg.size(), 1) * 4);
@@ -1287,6 +1300,8 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
     // graphs not yet repaired once take the reference's selection (fresh_level)
     const int fresh = B.all_settled ? (int)t.fresh_repair : 0;
     const bool rebuild_sweep = det || fresh < 2;
+    bool serial_ok = false;   // some graph is small enough for serial sweeps
+    for (uint32_t g = 0; g < ng; g++) serial_ok |= gs[g].used && gs[g].n <= t.serial_repair;
     hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "batch capture");
     e = launch_round_begin(stream, B.d_ctl, B.d_tab, B.d_inround, ng, B.d_fl, B.d_state, B.d_rg);
@@ -1322,7 +1337,7 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
             e = launch_build_list(Lp, M_CHECK, d_list, list_cap, d_list_count);
             if (e == hipSuccess) e = launch_tier1(Lp, false, fresh);
         }
-        if (e == hipSuccess) {
+        if (e == hipSuccess && (det || (it >= 2 && serial_ok))) {   // parallel runs: serial after two sweeps
             LaunchCtx Ls = base;
             Ls.active = F(F_SER);
             e = launch_tier1(Ls, true, det ? 0 : fresh);
@@ -1331,12 +1346,12 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
     if (e == hipSuccess)
         e = launch_t1_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.d_rg, true, false, det,
                               t.serial_repair, t.serial_conflicts);
-    if (e == hipSuccess) e = launch_t2_prep(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state);
+    if (e == hipSuccess) e = launch_t2_prep(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.lean);
     LaunchCtx Lt = base;
     Lt.active = F(F_T2);
-    if (e == hipSuccess) e = launch_build_list(Lt, M_T2, d_list, list_cap, d_list_count);
-    if (e == hipSuccess) e = launch_t2_check(Lt);
-    if (B.t2_passes) {   // Tier-2 on the device for the graphs whose check found bad slots
+    if (e == hipSuccess && !B.lean) e = launch_build_list(Lt, M_T2, d_list, list_cap, d_list_count);
+    if (e == hipSuccess && !B.lean) e = launch_t2_check(Lt);
+    if (B.t2_passes && !B.lean) {   // Tier-2 on the device for the graphs whose check found bad slots
         LaunchCtx L2 = base;
         L2.active = F(F_T2RUN);
         L2.gate = B.d_ctl + RC_T2_ANY;
@@ -1536,9 +1551,22 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
                 S.host_rounds++;
                 rc = batch_host_round(B, s);
                 if (rc) break;
+                if (B.h_ctl[RC_T2_MISSING] && B.lean) {   // the Tier-2 check is needed now: capture it
+                    B.lean = false;
+                    for (uint32_t k = 0; k < C && rc == SGC_OK; k++) {
+                        hipGraphExec_t x = nullptr;
+                        rc = batch_capture(B.cands[k], &x);
+                        if (rc == SGC_OK) {
+                            (void)hipGraphExecDestroy(B.execs[k]);
+                            B.execs[k] = x;
+                        }
+                    }
+                    if (rc) break;
+                }
                 r = s + 1;
                 B.h_ctl[RC_ROUND] = r;
                 B.h_ctl[RC_STOP] = 0;
+                B.h_ctl[RC_T2_MISSING] = 0;
                 e = batch_push(B);
             } else {
                 r = end;

@@ -97,6 +97,7 @@ enum : uint32_t {
     RC_SEG,           //   offset of its segment table (deterministic replay)
     RC_T2_ANY,        // some graph runs another device Tier-2 pass
     RC_T2_RAN,        // some graph ran the device Tier-2 this round
+    RC_T2_MISSING,    // the round stopped for a Tier-2 check the capture leaves out
     RC_WORDS = 16
 };
 // Per-graph flag arrays (ng u32 each): taking part in the round, Tier-1
@@ -118,8 +119,10 @@ hipError_t launch_round_begin(hipStream_t s, uint32_t* ctl, const uint32_t* tab,
 hipError_t launch_t1_control(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
                              RoundState* st, const RoundGraph* rg, bool after, bool before, bool deterministic,
                              uint32_t serial_repair, uint32_t serial_conflicts);
+// lean: the capture has no Tier-2 section, so a graph that needs the check
+// stops the round too (RC_T2_MISSING).
 hipError_t launch_t2_prep(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
-                          const RoundState* st);
+                          const RoundState* st, bool lean);
 hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl);
 // Device Tier-2 control of a round: 0 after the check (graphs with bad slots
 // run it), 1 after the reverse-only list (a graph whose list overflowed stops

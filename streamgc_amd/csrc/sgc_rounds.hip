@@ -131,11 +131,12 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t1_control(uint32_t* __restrict
 
 // After the captured Tier-1 sweeps: a graph still iterating stops the round
 // (the host finishes its repair).  The others need the Tier-2 check unless
-// their last sweep was clean and no overflow ever left an edge in one row.
+// their last sweep was clean and no overflow ever left an edge in one row; in
+// a capture without the Tier-2 section (lean) such a graph stops the round.
 __global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
                                                          uint32_t ng, uint32_t* __restrict__ fl,
-                                                         const RoundState* __restrict__ st) {
-    uint32_t stop = 0;
+                                                         const RoundState* __restrict__ st, int lean) {
+    uint32_t stop = 0, missing = 0;
     for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
         uint32_t need = 0;
         if (*flag(fl, ng, F_ROUND, g)) {
@@ -145,14 +146,17 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ 
                 SGC_G uint32_t* c = gp(graphs[g].ctr);
                 need = !(st[g].clean && c[C_ASYM] == 0);
                 if (need) c[C_T2BAD] = 0;
+                missing |= need && lean;
             }
         }
         *flag(fl, ng, F_T2, g) = need;
     }
     stop = block_any(stop);
-    if (threadIdx.x == 0 && stop && ctl[RC_STOP] == 0) {
+    missing = block_any(missing);
+    if (threadIdx.x == 0 && (stop || missing) && ctl[RC_STOP] == 0) {
         ctl[RC_STOP] = 1;
         ctl[RC_STOP_ROUND] = ctl[RC_ROUND];
+        ctl[RC_T2_MISSING] = missing;
     }
 }
 
@@ -270,8 +274,8 @@ hipError_t launch_t1_control(hipStream_t s, uint32_t* ctl, const GraphDev* graph
     return hipGetLastError();
 }
 hipError_t launch_t2_prep(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
-                          const RoundState* st) {
-    k_t2_prep<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, st);
+                          const RoundState* st, bool lean) {
+    k_t2_prep<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, st, lean ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl) {
