@@ -430,16 +430,19 @@ __device__ __forceinline__ void mark_neighbors(const GraphDev& g, uint32_t v, ui
 // other.
 // Out: pos = 1-based position of the row's slot (0: none before the end);
 // pf = the fill before an insert's raise when this wave wrote the slot, else 0;
-// had = an insert found the neighbour in the row already.
+// had = an insert found the neighbour in the row already; free = the first
+// free slot a delete's scan passed (absolute index, INVALID: none).
 template <bool INSERT>
 __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t x0, uint32_t nb0, uint32_t s0,
                                           uint32_t end0, bool run1, uint32_t x1, uint32_t nb1, uint32_t s1,
                                           uint32_t end1, uint32_t& pos0, uint32_t& pos1, uint32_t& pf0,
-                                          uint32_t& pf1, bool& had0, bool& had1) {
+                                          uint32_t& pf1, bool& had0, bool& had1, uint32_t& free0,
+                                          uint32_t& free1) {
     constexpr uint32_t CH = 2 * ROW_UNROLL;   // 64-slot chunks per round trip
     uint32_t b0 = s0, b1 = s1;
     pos0 = pos1 = pf0 = pf1 = 0;
     had0 = had1 = false;
+    free0 = free1 = INVALID;
     run0 = run0 && b0 < end0;
     run1 = run1 && b1 < end1;
     while (run0 || run1) {
@@ -460,6 +463,13 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
             const uint32_t want = r1 ? nb1 : nb0;
             const bool in = b + lane() < (r1 ? end1 : end0);
             const uint64_t m = ballot(in && (val[q] == want || (INSERT && val[q] == INVALID)));
+            if (!INSERT) {
+                const uint64_t mf = ballot(in && val[q] == INVALID);
+                if (mf && (r1 ? free1 : free0) == INVALID) {
+                    if (r1) free1 = b + (uint32_t)__builtin_ctzll(mf);
+                    else free0 = b + (uint32_t)__builtin_ctzll(mf);
+                }
+            }
             if (m && (r1 ? c1 : c0) == INVALID) {
                 const uint32_t k = (uint32_t)__builtin_ctzll(m);
                 const bool eq = __builtin_amdgcn_readlane(val[q], k) == want;
@@ -647,23 +657,73 @@ __device__ __forceinline__ void color_fresh(const GraphDev& g, uint32_t x, uint3
 // was uncoloured, and CHECK/T2 where the endpoints still share a colour.
 // With the live-degree ablation a row change moves the vertex's own
 // priority, so both kinds mark CHECK/T2 too.
-__device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint32_t v, uint32_t type, WaveAcc& acc) {
-    if (u >= g.n || v >= g.n || type > 1) return;  // hash miss: skipped (:77); other kinds do nothing
-    const uint32_t su = gp(g.row_ptr)[u], eu = gp(g.row_ptr)[u + 1];
-    const uint32_t sv = gp(g.row_ptr)[v], ev = gp(g.row_ptr)[v + 1];
-    const uint32_t hu = gp(g.hd_map)[u], hv = gp(g.hd_map)[v];
-    const SGC_G uint8_t* sticky = g.dirty ? gp(g.dirty) + (size_t)M_STICKY * g.dstride : nullptr;
-    const bool stu = sticky && sticky[u], stv = sticky && sticky[v];
+//
+// A wavefront applying one edge's updates in order passes a RowHint from a
+// delete to the next update: if that is the same edge's re-insert, the rows'
+// bounds are known and the first free-or-equal slot of each row is the first
+// free slot the delete's scan passed, or the one it freed (the delete found
+// the first copy there, so no copy lies before it); the insert claims those
+// slots directly and scans only when another wavefront took one.
+struct RowHint {
+    uint32_t u, v, su, eu, sv, ev, hu, hv, fu, fv;
+    bool stu, stv, valid;
+};
+__device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint32_t v, uint32_t type, WaveAcc& acc,
+                                             RowHint* hint = nullptr) {
+    if (u >= g.n || v >= g.n || type > 1) {  // hash miss: skipped (:77); other kinds do nothing
+        if (hint) hint->valid = false;
+        return;
+    }
+    const bool hinted = type == 0 && hint && hint->valid && hint->u == u && hint->v == v;
+    uint32_t su, eu, sv, ev, hu, hv, su0 = 0, sv0 = 0;
+    bool stu, stv;
+    if (hinted) {
+        su = hint->su, eu = hint->eu, sv = hint->sv, ev = hint->ev, hu = hint->hu, hv = hint->hv;
+        stu = hint->stu, stv = hint->stv;
+        su0 = hint->fu, sv0 = hint->fv;
+    } else {
+        su = gp(g.row_ptr)[u], eu = gp(g.row_ptr)[u + 1];
+        sv = gp(g.row_ptr)[v], ev = gp(g.row_ptr)[v + 1];
+        hu = gp(g.hd_map)[u], hv = gp(g.hd_map)[v];
+        const SGC_G uint8_t* sticky = g.dirty ? gp(g.dirty) + (size_t)M_STICKY * g.dstride : nullptr;
+        stu = sticky && sticky[u], stv = sticky && sticky[v];
+    }
+    if (hint) hint->valid = false;
     if ((g.flags & ABL_LIVE_DEGREE) && lane() == 0) {
         mark(g, u, D_STATUS);
         mark(g, v, D_STATUS);
     }
     const bool two = u != v;   // a self-loop's two rows are one: scanned one after the other
-    uint32_t pu, pv, fu, fv, d0, d1;
-    bool hadu, hadv, h0;
+    uint32_t pu = 0, pv = 0, fu = 0, fv = 0, d0, d1, xu, xv;
+    bool hadu = false, hadv = false, h0;
     if (type == 0) {
-        scan_pair<true>(g, true, u, v, su, eu, two, v, u, sv, ev, pu, pv, fu, fv, hadu, hadv);
-        if (!two) scan_pair<true>(g, false, u, v, su, eu, true, v, u, sv, ev, d0, pv, d1, fv, h0, hadv);
+        bool scan_u = true, scan_v = true;
+        if (hinted) {   // claim the slots the delete left (u != v: a hint needs two rows)
+            uint32_t old = 0, pf = 0;
+            if (lane() < 2) {
+                const uint32_t at = lane() == 0 ? su0 : sv0;
+                old = cas32(&g.col[at], INVALID, lane() == 0 ? v : u);
+                pf = max32(&g.fill[lane() == 0 ? u : v], at - (lane() == 0 ? su : sv) + 1);
+            }
+            if (__builtin_amdgcn_readlane(old, 0) == INVALID) {
+                pu = su0 - su + 1;
+                fu = __builtin_amdgcn_readlane(pf, 0);
+                scan_u = false;
+            }
+            if (__builtin_amdgcn_readlane(old, 1) == INVALID) {
+                pv = sv0 - sv + 1;
+                fv = __builtin_amdgcn_readlane(pf, 1);
+                scan_v = false;
+            }
+        }
+        if (scan_u || scan_v) {
+            uint32_t qu, qv, gu, gv;
+            bool au, av;
+            scan_pair<true>(g, scan_u, u, v, su, eu, two && scan_v, v, u, sv, ev, qu, qv, gu, gv, au, av, xu, xv);
+            if (scan_u) pu = qu, fu = gu, hadu = au;
+            if (scan_v && two) pv = qv, fv = gv, hadv = av;
+            if (!two) scan_pair<true>(g, false, u, v, su, eu, true, v, u, sv, ev, d0, pv, d1, fv, h0, hadv, xu, xv);
+        }
         const bool both_sticky = stu && stv;
         uint32_t p = finish_insert(g, u, v, su, eu, pu, fu, both_sticky, acc);
         p += finish_insert(g, v, u, sv, ev, pv, fv, both_sticky, acc);
@@ -690,14 +750,15 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
         }
         on_new_neighbor(g, u, hu, cv);
         on_new_neighbor(g, v, hv, cu);
-        const uint32_t qu = ld(&g.word[u]), qv = ld(&g.word[v]);
-        const uint16_t cu2 = color_of(bcast0(qu)), cv2 = color_of(bcast0(qv));
-        if (cu2 == cv2) {   // same colour (or both uncoloured): the sweeps look again whatever happens here
+        // The colours were read after both slots were written: a recolour of
+        // an endpoint after that read sees the new slot and marks the other
+        // endpoint itself, so no second read is needed.
+        if (cu == cv) {   // same colour (or both uncoloured): the sweeps look again whatever happens here
             if (lane() == 0) {
                 mark(g, u, D_STATUS);
                 mark(g, v, D_STATUS);
             }
-            if (cu2 != 0) recolor(g, pi_less(g, u, v) ? u : v, acc);
+            if (cu != 0) recolor(g, pi_less(g, u, v) ? u : v, acc);
         }
         acc.scan += p;
     } else {
@@ -705,8 +766,12 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
         const uint32_t lu = ld(&g.fill[u]), lv = ld(&g.fill[v]);
         const uint16_t cu = color_of(bcast0(ru)), cv = color_of(bcast0(rv));   // freed at v, freed at u
         const uint32_t tu = su + bcast0(lu), tv = sv + bcast0(lv);
-        scan_pair<false>(g, true, u, v, su, tu, two, v, u, sv, tv, pu, pv, fu, fv, hadu, hadv);
-        if (!two) scan_pair<false>(g, false, u, v, su, tu, true, v, u, sv, tv, d0, pv, d1, fv, h0, hadv);
+        scan_pair<false>(g, true, u, v, su, tu, two, v, u, sv, tv, pu, pv, fu, fv, hadu, hadv, xu, xv);
+        if (!two) scan_pair<false>(g, false, u, v, su, tu, true, v, u, sv, tv, d0, pv, d1, fv, h0, hadv, d0, xv);
+        if (hint && two && pu && pv) {   // both copies removed: the re-insert's slots
+            const uint32_t ou = su + pu - 1, ov = sv + pv - 1;
+            *hint = RowHint{u, v, su, eu, sv, ev, hu, hv, xu < ou ? xu : ou, xv < ov ? xv : ov, stu, stv, true};
+        }
         acc.scan += (pu ? pu : tu - su) + (pv ? pv : tv - sv);
         if (lane() == 0) {
             mark(g, u, (pu ? D_ROW : 0u) | ((hu == INVALID) == (pu != 0) ? D_BITS : 0u));

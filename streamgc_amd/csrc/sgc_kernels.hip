@@ -410,11 +410,11 @@ __global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__
 
 // ---------------------------------------------------------------- updates
 __device__ __forceinline__ void apply_record(const GraphDev* __restrict__ graphs, const uint4* __restrict__ upd,
-                                             uint32_t i, int multi, WaveAcc& acc) {
+                                             uint32_t i, int multi, WaveAcc& acc, RowHint* hint = nullptr) {
     const uint4 r = upd[i];
     const uint32_t gid = multi ? r.w : 0u;
     const GraphDev g = graphs[gid];
-    apply_update(g, r.x, r.y, r.z & 0xFFu, acc);
+    apply_update(g, r.x, r.y, r.z & 0xFFu, acc, hint);
 }
 
 __global__ __launch_bounds__(64) void k_apply_elem(const GraphDev* __restrict__ graphs,
@@ -448,12 +448,14 @@ __global__ __launch_bounds__(64) void k_apply_elem(const GraphDev* __restrict__ 
         }
         const uint32_t h = gp(ghead)[e];
         uint32_t last = 0, applied = 0;
+        RowHint hint;   // a delete's slots for the same edge's re-insert
+        hint.valid = false;
         for (;;) {
             uint32_t best = NONE;
             for (uint32_t k = h; k != NONE; k = gp(gnext)[k])
                 if ((applied == 0 || k > last) && (best == NONE || k < best)) best = k;
             if (best == NONE) break;
-            apply_record(graphs, upd, best, multi, acc);
+            apply_record(graphs, upd, best, multi, acc, &hint);
             last = best;
             applied++;
         }

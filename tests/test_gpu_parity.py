@@ -433,3 +433,65 @@ def test_ablations_bit_exact(ablation, repair, graph_path):
     _same_state(gpu, orc, ablation)
     if overflow:
         assert gpu.overflow_count() == 0 and gpu.totals()["overflow_events"] > 0
+
+
+def test_parallel_same_edge_groups_keep_the_oracles_slots(graph_path):
+    # One parallel launch applies each edge's updates in stream order on one
+    # wavefront; a delete followed by the same edge's re-insert claims the
+    # slot the delete's scan found first free.  With pairwise disjoint edges
+    # every row's result is independent of the wavefront order, so the rows
+    # must equal the oracle's slot for slot (holes and fill included).
+    g = S.load_graph_csr(graph_path("chunglu4k"))
+    gpu, orc, og = _pair(g, epoch_size=1 << 20, batch_size=1 << 20)
+    rp, col = g.row_ptr, g.col_idx
+    hub = int(np.argmax(g.degree))
+    nbrs = [int(x) for x in col[rp[hub]:rp[hub + 1]]]
+    assert len(nbrs) >= 40
+    # holes early in the hub's row
+    holes = S.make_updates([hub] * 3, [nbrs[2], nbrs[5], nbrs[9]], [1, 1, 1])
+    gpu.apply(holes)
+    orc.submit(holes["u"], holes["v"], holes["type"])
+    used = {hub, nbrs[2], nbrs[5], nbrs[9]}
+    # groups: the hub's edge deep in its row (re-insert goes to the first hole),
+    # other edges deleted and re-inserted, an edge inserted then deleted, and
+    # an edge deleted, re-inserted and deleted again
+    groups = [[(hub, nbrs[30], 1), (hub, nbrs[30], 0)]]
+    used |= {nbrs[30]}
+    rng = np.random.default_rng(7)
+    order = rng.permutation(g.n)
+    for u in order.tolist():
+        if len(groups) >= 60:
+            break
+        row = [int(x) for x in col[rp[u]:rp[u + 1]]]
+        free = [w for w in row if w not in used and w != u]
+        if u in used or not free:
+            continue
+        w = free[len(free) // 2]
+        kind = len(groups) % 3
+        if kind == 0:
+            groups.append([(u, w, 1), (u, w, 0)])
+        elif kind == 1:
+            groups.append([(u, w, 1), (u, w, 0), (u, w, 1)])
+        else:
+            x = next((y for y in order.tolist() if y not in used and y not in (u, w) and y not in row), None)
+            if x is None:
+                continue
+            groups.append([(u, x, 0), (u, x, 1)])
+            used.add(x)
+        used |= {u, w}
+    # interleave the groups at random, keeping each group's order
+    owner = np.concatenate([np.full(len(q), k) for k, q in enumerate(groups)])
+    rng.shuffle(owner)
+    pos = [0] * len(groups)
+    recs = []
+    for k in owner.tolist():
+        recs.append(groups[k][pos[k]])
+        pos[k] += 1
+    u, v, t = zip(*recs)
+    batch = S.make_updates(list(u), list(v), list(t))
+    gpu.apply(batch)
+    orc.submit(batch["u"], batch["v"], batch["type"])
+    grp, gcol = gpu.csr()
+    orp, ocol = orc.csr()
+    assert np.array_equal(grp, orp)
+    assert np.array_equal(gcol, ocol), np.nonzero(gcol != ocol)[0][:8]
