@@ -664,23 +664,39 @@ __device__ __forceinline__ void color_fresh(const GraphDev& g, uint32_t x, uint3
 // free slot the delete's scan passed, or the one it freed (the delete found
 // the first copy there, so no copy lies before it); the insert claims those
 // slots directly and scans only when another wavefront took one.
+// The delete's marks wait in the hint: a re-insert into the very slot the
+// delete freed leaves nothing to compact, and one that reads the neighbour's
+// colour as the delete did leaves the bitmap bit right.  Any other next
+// update, or the end of the edge's updates, stores them as they were.
 struct RowHint {
     uint32_t u, v, su, eu, sv, ev, hu, hv, fu, fv;
+    uint32_t ou, ov;   // the slots the delete freed
+    uint32_t mu, mv;   // the delete's marks, held back
+    uint16_t cu, cv;   // the colours the delete read
     bool stu, stv, valid;
 };
+__device__ __forceinline__ void hint_release(const GraphDev& g, RowHint* h) {
+    if (!h || !h->valid) return;
+    if (lane() == 0) {
+        mark(g, h->u, h->mu);
+        mark(g, h->v, h->mv);
+    }
+    h->valid = false;
+}
 __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint32_t v, uint32_t type, WaveAcc& acc,
                                              RowHint* hint = nullptr) {
-    if (u >= g.n || v >= g.n || type > 1) {  // hash miss: skipped (:77); other kinds do nothing
-        if (hint) hint->valid = false;
-        return;
-    }
-    const bool hinted = type == 0 && hint && hint->valid && hint->u == u && hint->v == v;
-    uint32_t su, eu, sv, ev, hu, hv, su0 = 0, sv0 = 0;
+    const bool hinted = type == 0 && hint && hint->valid && hint->u == u && hint->v == v && u < g.n && v < g.n;
+    if (!hinted) hint_release(g, hint);
+    if (u >= g.n || v >= g.n || type > 1) return;  // hash miss: skipped (:77); other kinds do nothing
+    uint32_t su, eu, sv, ev, hu, hv, su0 = 0, sv0 = 0, held_u = 0, held_v = 0;
     bool stu, stv;
     if (hinted) {
         su = hint->su, eu = hint->eu, sv = hint->sv, ev = hint->ev, hu = hint->hu, hv = hint->hv;
         stu = hint->stu, stv = hint->stv;
         su0 = hint->fu, sv0 = hint->fv;
+        held_u = hint->mu, held_v = hint->mv;
+        if (su0 == hint->ou) held_u &= ~D_ROW;   // back in the freed slot, if it can claim it
+        if (sv0 == hint->ov) held_v &= ~D_ROW;
     } else {
         su = gp(g.row_ptr)[u], eu = gp(g.row_ptr)[u + 1];
         sv = gp(g.row_ptr)[v], ev = gp(g.row_ptr)[v + 1];
@@ -716,6 +732,8 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
                 scan_v = false;
             }
         }
+        if (scan_u) held_u |= hinted ? D_ROW : 0u;   // another wavefront took the slot
+        if (scan_v) held_v |= hinted ? D_ROW : 0u;
         if (scan_u || scan_v) {
             uint32_t qu, qv, gu, gv;
             bool au, av;
@@ -733,6 +751,14 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
         }
         const uint32_t ru = ld(&g.word[u]), rv = ld(&g.word[v]);
         uint16_t cu = color_of(bcast0(ru)), cv = color_of(bcast0(rv));
+        if (hinted) {   // the delete's marks, less what the re-insert undid
+            if (cv == hint->cv) held_u &= ~D_BITS;
+            if (cu == hint->cu) held_v &= ~D_BITS;
+            if (lane() == 0) {
+                mark(g, u, held_u);
+                mark(g, v, held_v);
+            }
+        }
         if (cu == 0 || cv == 0) {   // first colouring of an endpoint (:90-101)
             if (lane() == 0) {
                 mark(g, u, D_BITS | D_STATUS);
@@ -768,14 +794,17 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
         const uint32_t tu = su + bcast0(lu), tv = sv + bcast0(lv);
         scan_pair<false>(g, true, u, v, su, tu, two, v, u, sv, tv, pu, pv, fu, fv, hadu, hadv, xu, xv);
         if (!two) scan_pair<false>(g, false, u, v, su, tu, true, v, u, sv, tv, d0, pv, d1, fv, h0, hadv, d0, xv);
-        if (hint && two && pu && pv) {   // both copies removed: the re-insert's slots
-            const uint32_t ou = su + pu - 1, ov = sv + pv - 1;
-            *hint = RowHint{u, v, su, eu, sv, ev, hu, hv, xu < ou ? xu : ou, xv < ov ? xv : ov, stu, stv, true};
-        }
         acc.scan += (pu ? pu : tu - su) + (pv ? pv : tv - sv);
-        if (lane() == 0) {
-            mark(g, u, (pu ? D_ROW : 0u) | ((hu == INVALID) == (pu != 0) ? D_BITS : 0u));
-            mark(g, v, (pv ? D_ROW : 0u) | ((hv == INVALID) == (pv != 0) ? D_BITS : 0u));
+        const uint32_t mu = (pu ? D_ROW : 0u) | ((hu == INVALID) == (pu != 0) ? D_BITS : 0u);
+        const uint32_t mv = (pv ? D_ROW : 0u) | ((hv == INVALID) == (pv != 0) ? D_BITS : 0u);
+        const bool hold = hint && two && pu && pv;   // the same edge's re-insert may follow
+        if (hold) {   // both copies removed: the re-insert's slots; the marks wait
+            const uint32_t ou = su + pu - 1, ov = sv + pv - 1;
+            *hint = RowHint{u, v, su, eu, sv, ev, hu, hv, xu < ou ? xu : ou, xv < ov ? xv : ov, ou, ov, mu, mv,
+                            cu, cv, stu, stv, true};
+        } else if (lane() == 0) {
+            mark(g, u, mu);
+            mark(g, v, mv);
         }
         on_removed_neighbor(g, u, hu, cv);
         on_removed_neighbor(g, v, hv, cu);

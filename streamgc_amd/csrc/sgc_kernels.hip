@@ -459,6 +459,7 @@ __global__ __launch_bounds__(64) void k_apply_elem(const GraphDev* __restrict__ 
             last = best;
             applied++;
         }
+        if (hint.valid) hint_release(graphs[multi ? upd[last].w : 0u], &hint);   // marks of a last delete
         grouped += applied - 1;
     }
     acc.flush(totals);
