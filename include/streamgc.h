@@ -327,8 +327,9 @@ int sgc_pool_set_epoch_size(sgc_pool* p, uint32_t gid, uint32_t epoch_size);
  * while a run is in progress. */
 typedef struct sgc_batch_config {
     uint32_t use_graph;        /* 1: replay a captured hipGraph per round; 0: host-driven launches */
-    uint32_t t1_sweeps;        /* Tier-1 sweeps captured per round (0: 4, at most max_repair_iters); a graph
-                                  still repairing after them makes the host finish the round */
+    uint32_t t1_sweeps;        /* Tier-1 sweeps captured per round (at most max_repair_iters); a graph still
+                                  repairing after them makes the host finish the round.  0: start with 2 and
+                                  double them after each round the host finished for that reason */
     uint32_t autotune_rounds;  /* > 0: rounds given to each launch-shape candidate (hipGraph mode) */
     uint32_t t2_passes;        /* device Tier-2 passes captured per round (0: 4); a graph needing more, or a
                                   Tier-2 on the host (SGC_HOST_TIER2), makes the host finish the round */

@@ -175,6 +175,7 @@ struct Staged {
 struct BatchRun {
     sgc_batch_config cfg{};
     uint32_t rounds = 0, ng = 0, max_count = 0, max_n = 0, sweeps = 0, ovf_cap = 0, t2_passes = 0;
+    uint32_t sweeps_max = 0;                   // most Tier-1 sweeps a capture may hold
     uint64_t updates = 0;
     bool multi = false, any_history = false, all_settled = true;
     bool lean = false;                         // captures leave the Tier-2 check and repair out
@@ -1209,7 +1210,13 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
         B->any_history |= gs[g].history != nullptr;
         B->all_settled &= gs[g].settled;
     }
-    B->sweeps = std::min(c.t1_sweeps ? c.t1_sweeps : 4u, max_iters);
+    // By default the rounds start with two captured sweeps (most rounds need
+    // one or two); each round that needs more is finished by the host and
+    // doubles the sweeps captured for the rounds left, up to the graphs'
+    // iteration cap (a slot costs a few launches, a host-finished round a
+    // synchronisation and a capture).
+    B->sweeps_max = c.t1_sweeps ? std::min(c.t1_sweeps, max_iters) : max_iters;
+    B->sweeps = c.t1_sweeps ? B->sweeps_max : std::min(2u, B->sweeps_max);
     // Tier-2 passes captured per round (host placement or no dirty tracking: none, the host repairs)
     uint32_t max_passes = 0;
     for (uint32_t g = 0; g < ng; g++)
@@ -1551,8 +1558,11 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
                 S.host_rounds++;
                 rc = batch_host_round(B, s);
                 if (rc) break;
-                if (B.h_ctl[RC_T2_MISSING] && B.lean) {   // the Tier-2 check is needed now: capture it
-                    B.lean = false;
+                const bool more_t2 = B.h_ctl[RC_T2_MISSING] && B.lean;   // the Tier-2 check is needed now
+                const bool more_t1 = B.h_ctl[RC_T1_SHORT] && B.sweeps < B.sweeps_max;   // more sweeps too
+                if (more_t2 || more_t1) {   // capture them for the rounds left
+                    if (more_t2) B.lean = false;
+                    if (more_t1) B.sweeps = std::min(2 * B.sweeps, B.sweeps_max);
                     for (uint32_t k = 0; k < C && rc == SGC_OK; k++) {
                         hipGraphExec_t x = nullptr;
                         rc = batch_capture(B.cands[k], &x);
@@ -1567,6 +1577,7 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
                 B.h_ctl[RC_ROUND] = r;
                 B.h_ctl[RC_STOP] = 0;
                 B.h_ctl[RC_T2_MISSING] = 0;
+                B.h_ctl[RC_T1_SHORT] = 0;
                 e = batch_push(B);
             } else {
                 r = end;

@@ -98,6 +98,7 @@ enum : uint32_t {
     RC_T2_ANY,        // some graph runs another device Tier-2 pass
     RC_T2_RAN,        // some graph ran the device Tier-2 this round
     RC_T2_MISSING,    // the round stopped for a Tier-2 check the capture leaves out
+    RC_T1_SHORT,      // the round stopped with a graph still iterating Tier-1
     RC_WORDS = 16
 };
 // Per-graph flag arrays (ng u32 each): taking part in the round, Tier-1

@@ -157,6 +157,7 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ 
         ctl[RC_STOP] = 1;
         ctl[RC_STOP_ROUND] = ctl[RC_ROUND];
         ctl[RC_T2_MISSING] = missing;
+        ctl[RC_T1_SHORT] = stop;
     }
 }
 
