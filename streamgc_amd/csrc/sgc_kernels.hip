@@ -409,13 +409,25 @@ __global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__
 }
 
 // ---------------------------------------------------------------- updates
-__device__ __forceinline__ void apply_record(const GraphDev* __restrict__ graphs, const uint4* __restrict__ upd,
-                                             uint32_t i, int multi, WaveAcc& acc, RowHint* hint = nullptr) {
-    const uint4 r = upd[i];
+__device__ __forceinline__ void apply_rec(const GraphDev* __restrict__ graphs, const uint4& r, int multi, WaveAcc& acc,
+                                          RowHint* hint = nullptr) {
     const uint32_t gid = multi ? r.w : 0u;
     const GraphDev g = graphs[gid];
     apply_update(g, r.x, r.y, r.z & 0xFFu, acc, hint);
 }
+__device__ __forceinline__ void apply_record(const GraphDev* __restrict__ graphs, const uint4* __restrict__ upd,
+                                             uint32_t i, int multi, WaveAcc& acc, RowHint* hint = nullptr) {
+    apply_rec(graphs, upd[i], multi, acc, hint);
+}
+
+// Edge lists of up to 4 positions are applied from registers; NONE (all
+// ones) sorts last.
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
 
 __global__ __launch_bounds__(64) void k_apply_elem(const GraphDev* __restrict__ graphs,
                                                     const uint4* __restrict__ upd, uint32_t count, int multi,
@@ -450,10 +462,27 @@ __global__ __launch_bounds__(64) void k_apply_elem(const GraphDev* __restrict__ 
         uint32_t last = 0, applied = 0;
         RowHint hint;   // a delete's slots for the same edge's re-insert
         hint.valid = false;
+        // Short lists (nearly all: a delete and its re-insert) are read once
+        // into registers and put in stream order by a fixed network.
+        uint32_t p0 = h, p1 = NONE, p2 = NONE, p3 = NONE, k = NONE;
+        if (p0 != NONE) p1 = gp(gnext)[p0];
+        if (p1 != NONE) p2 = gp(gnext)[p1];
+        if (p2 != NONE) p3 = gp(gnext)[p2];
+        if (p3 != NONE) k = gp(gnext)[p3];
+        const bool held = k == NONE;   // the whole list is in p0..p3
+        cswap(p0, p1);                 // sorting network for 4 (NONE sorts last)
+        cswap(p2, p3);
+        cswap(p0, p2);
+        cswap(p1, p3);
+        cswap(p1, p2);
         for (;;) {
             uint32_t best = NONE;
-            for (uint32_t k = h; k != NONE; k = gp(gnext)[k])
-                if ((applied == 0 || k > last) && (best == NONE || k < best)) best = k;
+            if (held) {
+                best = applied == 0 ? p0 : applied == 1 ? p1 : applied == 2 ? p2 : applied == 3 ? p3 : NONE;
+            } else {   // a longer list: the smallest position above the last one applied
+                for (uint32_t k2 = h; k2 != NONE; k2 = gp(gnext)[k2])
+                    if ((applied == 0 || k2 > last) && (best == NONE || k2 < best)) best = k2;
+            }
             if (best == NONE) break;
             apply_record(graphs, upd, best, multi, acc, &hint);
             last = best;
