@@ -742,9 +742,13 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
             if (scan_v && two) pv = qv, fv = gv, hadv = av;
             if (!two) scan_pair<true>(g, false, u, v, su, eu, true, v, u, sv, ev, d0, pv, d1, fv, h0, hadv, xu, xv);
         }
-        const bool both_sticky = stu && stv;
-        uint32_t p = finish_insert(g, u, v, su, eu, pu, fu, both_sticky, acc);
-        p += finish_insert(g, v, u, sv, ev, pv, fv, both_sticky, acc);
+        // No duplicate check when both rows took the slots the delete left:
+        // each row lost one copy of the edge and gained one, so the rows hold
+        // as many copies as before (and if they disagreed, both endpoints are
+        // sticky already).
+        const bool skip_dup = (stu && stv) || (hinted && !scan_u && !scan_v);
+        uint32_t p = finish_insert(g, u, v, su, eu, pu, fu, skip_dup, acc);
+        p += finish_insert(g, v, u, sv, ev, pv, fv, skip_dup, acc);
         if (lane() == 0) {
             if (hadu) mark(g, u, D_BITS);
             if (hadv) mark(g, v, D_BITS);
