@@ -544,6 +544,29 @@ __global__ __launch_bounds__(256) void k_max_color(const GraphDev* __restrict__ 
 
 // ---------------------------------------------------------------- snapshot
 // A/src/kernels/epoch_maintenance.cu:5-16, into the back buffer.
+// One graph: each thread copies the colours of 4 consecutive vertices (one
+// 16-byte read of their words; the snapshot half may start at any even
+// address, so four 2-byte stores).
+__global__ __launch_bounds__(256) void k_snapshot1(const GraphDev* __restrict__ graphs, Space sp,
+                                                   const uint32_t* __restrict__ active) {
+    const uint32_t gid = sp.gid ? sp.gid[0] : 0u;
+    if (active && !active[gid]) return;
+    const GraphDev g = graphs[gid];
+    SGC_G uint16_t* out = gp(g.snap_write);
+    const uint32_t quads = g.n / 4;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+        const uint4 w = *reinterpret_cast<const uint4*>(g.word + 4 * (size_t)q);
+        out[4 * q] = color_of(w.x);
+        out[4 * q + 1] = color_of(w.y);
+        out[4 * q + 2] = color_of(w.z);
+        out[4 * q + 3] = color_of(w.w);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < g.n % 4) {
+        const uint32_t v = 4 * quads + threadIdx.x;
+        out[v] = color_of(gp(g.word)[v]);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ graphs, Space sp,
                                                   const uint32_t* __restrict__ active) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sp.total; i += gridDim.x * blockDim.x) {
@@ -1628,6 +1651,11 @@ hipError_t launch_max_color(const LaunchCtx& L) {
     return hipGetLastError();
 }
 hipError_t launch_snapshot(const LaunchCtx& L) {
+    if (L.ns == 1) {
+        const uint32_t grid = grid_for_waves(L.V / 4 + 1, 4 * WAVE, L.max_blocks);
+        k_snapshot1<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), L.active);
+        return hipGetLastError();
+    }
     uint32_t grid = grid_for_waves(L.V, 4 * WAVE, L.max_blocks);
     k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), L.active);
     return hipGetLastError();
