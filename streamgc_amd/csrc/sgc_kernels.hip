@@ -542,6 +542,35 @@ __global__ __launch_bounds__(256) void k_max_color(const GraphDev* __restrict__ 
     }
 }
 
+// One graph: 4 words per 16-byte read, a block-wide maximum, one atomic per
+// block (thousands of waves folding into one counter serialise on it).
+__global__ __launch_bounds__(256) void k_max_color1(const GraphDev* __restrict__ graphs, Space sp) {
+    __shared__ uint32_t part[BLOCK / WAVE];
+    const uint32_t gid = sp.gid ? sp.gid[0] : 0u;
+    const GraphDev g = graphs[gid];
+    const uint32_t quads = g.n / 4;
+    uint32_t mx = 0;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+        const uint4 w = *reinterpret_cast<const uint4*>(g.word + 4 * (size_t)q);
+        const uint32_t a = color_of(w.x) > color_of(w.y) ? color_of(w.x) : color_of(w.y);
+        const uint32_t b = color_of(w.z) > color_of(w.w) ? color_of(w.z) : color_of(w.w);
+        mx = a > mx ? a : mx;
+        mx = b > mx ? b : mx;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < g.n % 4) {
+        const uint32_t c = color_of(gp(g.word)[4 * quads + threadIdx.x]);
+        mx = c > mx ? c : mx;
+    }
+    mx = wave_max32(mx);
+    if (lane() == 0) part[threadIdx.x / WAVE] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t k = 0; k < BLOCK / WAVE; k++) t = part[k] > t ? part[k] : t;
+        if (t) max32(&g.ctr[C_MAXC], t);
+    }
+}
+
 // ---------------------------------------------------------------- snapshot
 // A/src/kernels/epoch_maintenance.cu:5-16, into the back buffer.
 // One graph: each thread copies the colours of 4 consecutive vertices (one
@@ -1646,6 +1675,10 @@ hipError_t launch_zero_ctr(const LaunchCtx& L, uint32_t idx) {
     return hipGetLastError();
 }
 hipError_t launch_max_color(const LaunchCtx& L) {
+    if (L.ns == 1) {
+        k_max_color1<<<256, 256, 0, L.stream>>>(L.graphs, space(L));
+        return hipGetLastError();
+    }
     uint32_t grid = grid_for_waves(L.V, 4 * WAVE, 1024);
     k_max_color<<<grid, 256, 0, L.stream>>>(L.graphs, space(L));
     return hipGetLastError();
