@@ -99,7 +99,7 @@ struct GraphDev {
     uint32_t n;
     uint32_t ovf_cap;
     uint32_t nhd;
-    uint32_t dstride;         // n rounded up to a multiple of 4
+    uint32_t dstride;         // n rounded up to a multiple of 16
     uint32_t flags;           // SGC_ABL_* ablation flags of the pool
 };
 

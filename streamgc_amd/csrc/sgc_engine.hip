@@ -58,7 +58,9 @@ struct PendingTiming {
     hipEvent_t a, b;
 };
 
-uint32_t mark_stride(uint32_t n) { return (n + 3u) & ~3u; }
+// Mark arrays are padded to a multiple of 16 vertices: sweeps read the marks
+// of 4 vertices per 4-byte word, the serial sweep 16 per lane.
+uint32_t mark_stride(uint32_t n) { return (n + 15u) & ~15u; }
 constexpr uint32_t ABL_MASK = SGC_ABL_FORCE_BITMAP | SGC_ABL_FORCE_HISTOGRAM | SGC_ABL_NO_OVERFLOW | SGC_ABL_LIVE_DEGREE;
 
 // Bytes one graph occupies on the device (admission control).

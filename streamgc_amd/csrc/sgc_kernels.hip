@@ -1201,27 +1201,38 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
             for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex(g, v, presence, fresh);
             rows += g.n;
         } else {
-            // 256 vertices per step: each lane reads the marks of 4 as one dword
+            // 1024 vertices per step: each lane reads the marks of 16 as two
+            // 8-byte words (mark arrays are padded to a multiple of 16), and
+            // again after each vertex it visits (a recolour marks neighbours)
             const size_t cst = (size_t)M_CHECK * g.dstride, sst = (size_t)M_STICKY * g.dstride;
-            for (uint32_t base = 0; base < g.n; base += 4 * WAVE) {
-                const uint32_t v0 = base + 4 * lane();
-                const uint32_t sticky = v0 < g.n ? *gp((const uint32_t*)(g.dirty + sst + v0)) : 0u;
+            for (uint32_t base = 0; base < g.n; base += 16 * WAVE) {
+                const uint32_t v0 = base + 16 * lane();
+                const uint64_t* cw = (const uint64_t*)(g.dirty + cst + v0);
+                const uint64_t* sw = (const uint64_t*)(g.dirty + sst + v0);
+                const uint64_t s0 = v0 < g.n ? gp(sw)[0] : 0ull, s1 = v0 < g.n ? gp(sw)[1] : 0ull;
+                const uint32_t live = v0 >= g.n ? 0u : g.n - v0 >= 16 ? 0xFFFFu : (1u << (g.n - v0)) - 1u;
                 uint32_t from = 0;   // chunk offset of the next vertex the sweep may visit
                 for (;;) {
-                    const uint32_t w = (v0 < g.n ? ld((const uint32_t*)(g.dirty + cst + v0)) : 0u) | sticky;
-                    uint32_t m4 = 0;
+                    const uint64_t w0 = (v0 < g.n ? ld(&cw[0]) : 0ull) | s0;
+                    const uint64_t w1 = (v0 < g.n ? ld(&cw[1]) : 0ull) | s1;
+                    uint32_t m16 = 0;
 #pragma unroll
-                    for (uint32_t q = 0; q < 4; q++)
-                        if (((w >> (8 * q)) & 0xFFu) && v0 + q < g.n && 4 * lane() + q >= from) m4 |= 1u << q;
-                    const uint64_t any = ballot(m4 != 0);
+                    for (uint32_t q = 0; q < 8; q++) {
+                        m16 |= (((w0 >> (8 * q)) & 0xFFull) != 0) << q;
+                        m16 |= (((w1 >> (8 * q)) & 0xFFull) != 0) << (q + 8);
+                    }
+                    m16 &= live;
+                    const uint32_t at = 16 * lane();
+                    m16 &= from <= at ? 0xFFFFu : from >= at + 16 ? 0u : ~((1u << (from - at)) - 1u) & 0xFFFFu;
+                    const uint64_t any = ballot(m16 != 0);
                     if (!any) break;
                     const uint32_t k = (uint32_t)__builtin_ctzll(any);
-                    const uint32_t q = (uint32_t)__builtin_ctz((uint32_t)__shfl((int)m4, (int)k));
-                    const uint32_t u = base + 4 * k + q;
+                    const uint32_t q = (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(m16, k));
+                    const uint32_t u = base + 16 * k + q;
                     if (lane() == 0) gp(g.dirty)[cst + u] = 0;
                     cnt += tier1_vertex(g, u, presence, fresh);
                     rows++;
-                    from = 4 * k + q + 1;
+                    from = 16 * k + q + 1;
                 }
             }
         }
