@@ -42,6 +42,9 @@ WORKLOADS = {
                     "R-MAT s12 ef 48) + com-Orkut-shape Chung-Lu (3,072,441 vertices, 117M edges) resident in one "
                     "pool; churn (50% insert) batches of U[1,8192] updates to uniform tenants; E_g = 4,096",
                kind="served"),
+    "trc": dict(name="T_rc: dependent version-bumping CAS on one vertex word, lane 0 of one wavefront, 1M "
+                     "iterations, median of 11 runs; then 16,384 wavefronts on their own words and 1,024 on one "
+                     "shared word", kind="microbench"),
     "c3": dict(name="R-MAT scale 20 (a,b,c = .57,.19,.19), edge factor 48 (seed 42); dense-growth stream inside the "
                     "90th-percentile k-core, 2 inserts per delete (seed 42); E=32,768; plain engine",
                kind="rmat", scale=20, edge_factor=48.0),
@@ -192,6 +195,8 @@ def main():
         sys.exit(1)
 
     w = WORKLOADS[args.workload]
+    if w["kind"] == "microbench":
+        return run_trc(S, args, w)
     if w["kind"] == "served":
         return run_served(S, args, w)
     if w["kind"] == "batched":
@@ -566,6 +571,23 @@ def run_batched(S, args, w):
 
 
 # ---------------------------------------------------------------- served (C5)
+def run_trc(S, args, w):
+    """The T_rc microbenchmark (A/benchmark/microbench/t_rc_atomic.cu:18-62) and two wider forms."""
+    lat = S.microbench_trc(waves=1, iters=1_000_000, runs=11)
+    many = S.microbench_trc(waves=16384, iters=20_000, runs=5)
+    shared = S.microbench_trc(waves=1024, shared=True, iters=2_000, runs=5)
+    log(f"T_rc {lat['ns']:.1f} ns; 16384 words: {16384 / many['ns']:.1f} G CAS/s; "
+        f"1024 wavefronts on one word: {1024 / shared['ns']:.3f} G attempts/s")
+    print(json.dumps({
+        "metric": "T_rc: ns per dependent CAS on one vertex word", "value": round(lat["ns"], 2), "unit": "ns",
+        "n_gpus": 1, "steps": 11, "warmup": 1, "ms_per_step": round(lat["ns"] * 1e-3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": w["name"]},
+        "throughput": {"own_words_gcas_per_s": round(16384 / many["ns"], 2),
+                       "shared_word_gattempts_per_s": round(1024 / shared["ns"], 4)},
+        "roofline": None, "cpu_baseline": None}))
+
+
 def make_tenants(S, count, seed=42):
     """Tenant graphs of the C1-C4 shapes at tenant scale (seed 42 + index)."""
     rng = np.random.default_rng(seed)

@@ -146,6 +146,17 @@ int sgc_device_count(void);
  * name ("gfx950") when the runtime reports no marketing name. */
 int sgc_device_name(int device, char* name, uint32_t cap);
 
+/* T_rc (A/benchmark/microbench/t_rc_atomic.cu:18-62): `waves` wavefronts,
+ * lane 0 of each running `iters` dependent CAS operations that bump a
+ * vertex word's version and toggle its colour's low bit (a failed CAS takes
+ * the value it found).  shared = 0: each wavefront on its own word, a
+ * 128-byte line apart; 1: all on one word.  *ns = the median over `runs`
+ * (at most 101) of the kernel time per iteration of one wavefront (with one
+ * wavefront: the latency of a dependent CAS).  final_words (optional, 2):
+ * the first and the last wavefront's word after the last run. */
+int sgc_microbench_trc(int device, uint32_t waves, uint32_t shared, uint32_t iters, uint32_t runs, double* ns,
+                       uint32_t* final_words);
+
 /* = allocate_gpu_partition (A/src/streamgc.cu:144-444) + the priority fill
  * (:755-767).  Input: symmetric CSR with sorted rows (as the loader makes),
  * deg0 = row lengths (fixes the priority pi(v) = (deg0 << 32) | v). */

@@ -23,7 +23,7 @@ __all__ = ["StreamGCConfig", "default_config", "CSRGraph", "GPUPartition", "Devi
            "generate_init_stream", "generate_bench_stream_edges",
            "static_greedy_coloring", "check_coloring_valid", "edge_intervals", "make_updates", "SgcError",
            "SgcNoMemory", "device_count", "GraphPool", "GraphServer", "tag_graph", "write_coloring",
-           "write_coloring_meta", "read_coloring"]
+           "write_coloring_meta", "read_coloring", "microbench_trc"]
 
 from ._lib import SgcNoMemory  # noqa: E402
 
@@ -81,6 +81,19 @@ def default_config(**kw):
 
 def device_count():
     return lib().sgc_device_count()
+
+
+def microbench_trc(waves=1, shared=False, iters=1_000_000, runs=11, device=0):
+    """T_rc (A/benchmark/microbench/t_rc_atomic.cu:18-62): dependent version-bumping CAS on a vertex word.
+
+    waves wavefronts, lane 0 of each looping `iters` times on its own word
+    (shared=False, a cache line apart) or on one shared word.  Returns the
+    median kernel time per iteration in ns and the first and last
+    wavefront's word after the last run."""
+    ns = C.c_double()
+    fin = (C.c_uint32 * 2)()
+    check(lib().sgc_microbench_trc(device, waves, 1 if shared else 0, iters, runs, C.byref(ns), fin))
+    return {"ns": ns.value, "final": (fin[0], fin[1])}
 
 
 class CSRGraph:

@@ -142,6 +142,8 @@ SIGNATURES = {
     "sgc_version": [],
     "sgc_device_count": [],
     "sgc_device_name": [C.c_int, C.c_char_p, C.c_uint32],
+    "sgc_microbench_trc": [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                           C.POINTER(C.c_uint32)],
     "sgc_create": [C.POINTER(Config), C.c_uint32, _u32p, _u32p, _u32p, C.POINTER(_vp)],
     "sgc_destroy": [_vp],
     "sgc_submit": [_vp, _vp, C.c_uint64, C.c_int, C.POINTER(EpochStats), C.c_uint32, C.POINTER(C.c_uint32)],

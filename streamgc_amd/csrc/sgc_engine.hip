@@ -1889,6 +1889,47 @@ int sgc_device_name(int device, char* name, uint32_t cap) {
     return SGC_OK;
 }
 
+int sgc_microbench_trc(int device, uint32_t waves, uint32_t shared, uint32_t iters, uint32_t runs, double* ns,
+                       uint32_t* final_words) {
+    if (!ns || waves == 0 || waves > 65536 || iters == 0 || runs == 0 || runs > 101)
+        return global_fail(SGC_ERR_INVALID, "sgc_microbench_trc: bad argument");
+    hipError_t e = hipSetDevice(device);
+    const uint32_t stride = shared ? 0u : 32u;   // one 128-byte line per wavefront, or one word for all
+    const size_t nwords = shared ? 1 : (size_t)waves * stride;
+    uint32_t* words = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    std::vector<double> t;
+    if (e == hipSuccess) e = hipMalloc(&words, nwords * 4);
+    if (e == hipSuccess) e = hipMemset(words, 0, nwords * 4);
+    if (e == hipSuccess) e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = launch_trc(st, words, waves, stride, 1024);   // warm-up
+    if (e == hipSuccess) e = hipMemsetAsync(words, 0, nwords * 4, st);
+    for (uint32_t r = 0; r < runs && e == hipSuccess; r++) {
+        e = hipEventRecord(a, st);
+        if (e == hipSuccess) e = launch_trc(st, words, waves, stride, iters);
+        if (e == hipSuccess) e = hipEventRecord(b, st);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        t.push_back((double)ms * 1e6 / iters);
+    }
+    if (e == hipSuccess && final_words) {   // the first wavefront's word and, apart, the last one's
+        e = hipMemcpy(&final_words[0], words, 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(&final_words[1], words + (size_t)(waves - 1) * stride, 4, hipMemcpyDeviceToHost);
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (st) (void)hipStreamDestroy(st);
+    if (words) (void)hipFree(words);
+    if (e != hipSuccess) return global_fail(SGC_ERR_HIP, std::string("sgc_microbench_trc: ") + hipGetErrorString(e));
+    std::sort(t.begin(), t.end());
+    *ns = t[t.size() / 2];
+    return SGC_OK;
+}
+
 // ---------------------------------------------------------------- single graph
 int sgc_create(const sgc_config* cfg, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
                const uint32_t* deg0, sgc_engine** out) {

@@ -1811,4 +1811,26 @@ hipError_t launch_fill_rows(hipStream_t s, uint32_t n, const uint32_t* rp0, cons
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- T_rc microbenchmark
+// A/benchmark/microbench/t_rc_atomic.cu:18-28: one lane runs `iters`
+// dependent CAS operations on a vertex word, each bumping the version and
+// toggling the colour's low bit (a failed CAS takes the value it found).
+// Here every wavefront's lane 0 does so, on word (wave x stride): stride 32
+// gives each wavefront its own 128-byte line, stride 0 makes them all share
+// one word.
+__global__ __launch_bounds__(64) void k_trc(uint32_t* __restrict__ words, uint32_t stride, uint32_t iters) {
+    if (threadIdx.x != 0) return;
+    uint32_t* w = words + (size_t)blockIdx.x * stride;
+    uint32_t cur = ld(w);
+    for (uint32_t i = 0; i < iters; i++) {
+        const uint32_t nxt = bump(cur, (uint16_t)((cur & 0xFFFFu) ^ 1u));
+        const uint32_t got = cas32(w, cur, nxt);
+        cur = got == cur ? nxt : got;
+    }
+}
+hipError_t launch_trc(hipStream_t s, uint32_t* words, uint32_t waves, uint32_t stride, uint32_t iters) {
+    k_trc<<<waves, WAVE, 0, s>>>(words, stride, iters);
+    return hipGetLastError();
+}
+
 }  // namespace sgc

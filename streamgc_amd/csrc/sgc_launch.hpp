@@ -137,4 +137,8 @@ hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, ui
 hipError_t launch_fill_rows(hipStream_t s, uint32_t n, const uint32_t* rp0, const uint32_t* col0, const uint32_t* rp,
                             uint32_t* col, uint32_t* fill);
 
+// T_rc microbenchmark: `waves` wavefronts, lane 0 of each running `iters`
+// dependent version-bumping CAS operations on words[wave x stride].
+hipError_t launch_trc(hipStream_t s, uint32_t* words, uint32_t waves, uint32_t stride, uint32_t iters);
+
 }  // namespace sgc
