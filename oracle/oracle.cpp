@@ -26,6 +26,15 @@
 //                           invariants (proper after repair, version +1 per
 //                           recolor, only the lower-priority endpoint recolors)
 //                           and brute-force chromatic bounds on tiny graphs
+//   device quirks ......... hand-computed states, one per behaviour: colour
+//                           reduction on delete and which freed colour each
+//                           endpoint gets, propagation of a recolour to the
+//                           neighbours' bitmaps and histograms, stable
+//                           compaction, Tier-1's own-colour bit, the plateau
+//                           stop, drain in buffer order, hub bitmaps left to
+//                           the histogram.  tests/test_oracle_mutants.py plants
+//                           a slip in each of these (and others) in a scratch
+//                           copy of this file and requires a test to fail.
 //   tier-2 / validator .... brute force on tiny graphs; static greedy against a
 //                           textbook first-fit written independently in Python
 //
@@ -734,6 +743,8 @@ void orc_apply(void* h, const uint32_t* u, const uint32_t* v, const uint8_t* t, 
 void orc_epoch(void* h, Stats* st) { ((Engine*)h)->epoch(*st); }
 void orc_finish(void* h, Stats* st) { ((Engine*)h)->finish(*st); }
 void orc_tier1(void* h, Stats* st) { ((Engine*)h)->tier1(*st); }
+// One Tier-1 sweep on the current bitmaps (no rebuild first); returns its conflict count.
+uint32_t orc_tier1_sweep(void* h) { return ((Engine*)h)->tier1_sweep(); }
 void orc_tier2(void* h, Stats* st) { ((Engine*)h)->tier2(*st); }
 void orc_rebuild_bitmaps(void* h) { ((Engine*)h)->rebuild_bitmaps(); }
 void orc_compact(void* h) { ((Engine*)h)->compact(); }

@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+# SGC_ORACLE_LIB points the tests at another build of oracle.cpp: the mutation
+# tests use it to run the oracle's own tests against a copy with a planted slip.
+LIB_PATH = os.environ.get("SGC_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
@@ -83,6 +85,8 @@ def lib():
         L.orc_apply.argtypes = [C.c_void_p, _u32p, _u32p, _u8p, C.c_uint64]
         for f in ("orc_epoch", "orc_finish", "orc_tier1", "orc_tier2"):
             getattr(L, f).argtypes = [C.c_void_p, C.POINTER(Stats)]
+        L.orc_tier1_sweep.restype = C.c_uint32
+        L.orc_tier1_sweep.argtypes = [C.c_void_p]
         L.orc_rebuild_bitmaps.argtypes = [C.c_void_p]
         L.orc_compact.argtypes = [C.c_void_p]
         L.orc_num_slots.restype = C.c_uint64
@@ -240,6 +244,15 @@ class Engine:
 
     def tier2(self):
         return self._stat_call("orc_tier2")
+
+    def tier1_sweep(self):
+        return lib().orc_tier1_sweep(self.h)
+
+    def rebuild_bitmaps(self):
+        lib().orc_rebuild_bitmaps(self.h)
+
+    def compact(self):
+        lib().orc_compact(self.h)
 
     def words(self):
         w = np.zeros(self.g.n, np.uint32)

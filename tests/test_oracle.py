@@ -430,3 +430,218 @@ def test_ablation_live_degree_priority_picks_the_other_loser():
         after = e.words()
         assert (after[loser] & 0xFFFF) != 1 and (after[loser] >> 16) == 1
         assert after[1 - loser] == w[1 - loser]
+
+
+# ---------------------------------------------------------------- device quirks, hand-computed states
+# Each case sets a small state by hand (words, bitmaps, histograms, rows), runs
+# one oracle step and compares every affected array with values worked out by
+# hand from the reference lines cited.  tests/test_oracle_mutants.py plants a
+# slip in each of these oracle functions and requires one of these to fail.
+def _w(color, version):
+    return (version << 16) | color
+
+
+def test_delete_reduces_with_the_neighbours_colour_and_propagates():
+    # A/src/kernels/local_update.cu:126-141, recolor.cu:50-82,145-163, color_select.cuh:120-157,188-209.
+    # Histogram rows for every vertex (threshold 0).  Path 1 - 0 - 2; colours 0:3 (v5), 1:1 (v2), 2:2 (v7).
+    # DEL(0,1): freed_u = colour(1) = 1, freed_v = colour(0) = 3 (both read before anything changes).
+    #   hist[0] bin 1 -> 0, hist[1] bin 3 -> 0.
+    #   reduce(0, 1): 1 < 3 and the first zero bin of hist[0] is colour 1 < 3 -> recolour 0 to 1 (v6);
+    #     propagate 3 -> 1 along row 0 (now only vertex 2): bitmap[2] |= bit(1), hist[2] bin 3 -1, bin 1 +1.
+    #   reduce(1, 3): 3 >= colour(1) = 1 -> nothing.
+    g, e = _engine(3, [(0, 1), (0, 2)], histogram_threshold=0)
+    e.set_words(np.array([_w(3, 5), _w(1, 2), _w(2, 7)], np.uint32))
+    h = e.hist()
+    h[:] = 0
+    h[0, 0] = 1          # vertex 0 sees colour 1 (vertex 1)
+    h[0, 1] = 1          # and colour 2 (vertex 2)
+    h[1, 2] = 1          # vertex 1 sees colour 3
+    h[2, 2] = 1          # vertex 2 sees colour 3
+    e.set_hist(h)
+    e.apply(np.array([0]), np.array([1]), np.array([1]))
+    assert e.words().tolist() == [_w(1, 6), _w(1, 2), _w(2, 7)]
+    want = np.zeros((3, 1024), np.uint32)
+    want[0, 1] = 1
+    want[2, 0] = 1
+    assert np.array_equal(e.hist(), want)
+    assert e.bitmaps().tolist() == [0, 0, 0b1]
+
+
+def test_delete_leaves_bitmap_bits_and_reduces_after_a_rebuild():
+    # Bitmap rows (threshold 64).  Star 0 - {1, 2, 3}; colours 0:4 (v1), 1:1, 2:3, 3:2.
+    # A delete never clears a bitmap bit (color_select.cuh:188-209), so right after DEL(0,1)
+    # bitmap[0] = {1,2,3} still holds colour 1 and the first clear bit is 4 = colour(0): no reduction.
+    # The rebuild (epoch_maintenance.cu:113-128) makes bitmap[0] = {2,3} (rows 0: {2,3}, 1: {},
+    # 2: {4}, 3: {4}).  DEL(0,2): freed_u = 3 < 4 and the first clear bit is colour 1 -> 0 becomes 1 (v2);
+    # propagate 4 -> 1 to the one neighbour left, 3: bitmap[3] = {4} | {1} (the old bit stays set).
+    # freed_v = 4 >= colour(2) = 3: vertex 2 keeps its colour.
+    g, e = _engine(4, [(0, 1), (0, 2), (0, 3)])
+    words = np.array([_w(4, 1), _w(1, 1), _w(3, 1), _w(2, 1)], np.uint32)
+    e.set_words(words)
+    e.set_bitmaps(np.array([0b0111, 0b1000, 0b1000, 0b1000], np.uint64))
+    e.apply(np.array([0]), np.array([1]), np.array([1]))
+    assert e.words().tolist() == words.tolist()
+    assert e.bitmaps().tolist() == [0b0111, 0b1000, 0b1000, 0b1000]
+    e.rebuild_bitmaps()
+    assert e.bitmaps().tolist() == [0b0110, 0, 0b1000, 0b1000]
+    e.apply(np.array([0]), np.array([2]), np.array([1]))
+    assert e.words().tolist() == [_w(1, 2), _w(1, 1), _w(3, 1), _w(2, 1)]
+    assert e.bitmaps().tolist() == [0b0110, 0, 0b1000, 0b1001]
+
+
+def test_delete_of_a_conflicting_edge_does_not_reduce():
+    # recolor.cu:155: reduction needs freed < current.  Edge 0-1 with both endpoints colour 3 and empty
+    # bitmaps (first clear bit: colour 1): freed == current on both sides, so neither endpoint moves.
+    g, e = _engine(2, [(0, 1)])
+    words = np.array([_w(3, 4), _w(3, 9)], np.uint32)
+    e.set_words(words)
+    e.apply(np.array([0]), np.array([1]), np.array([1]))
+    assert e.words().tolist() == words.tolist()
+
+
+def test_add_conflict_recolour_propagates_to_the_losers_row():
+    # local_update.cu:110-122 then recolor.cu:87-111 and :50-82.  Vertex 1 (deg0 1) loses to
+    # vertex 0 (deg0 2) on colour 2.  Vertex 1's row: {0, 3} after the insert; vertex 3 is a hub
+    # (threshold 1, deg0 2) with histogram bins {2: 1}.  bitmap[1] = {1, 2} -> vertex 1 takes colour 3 (v5).
+    # Propagation 2 -> 3: bitmap[0] |= bit(3) (keeps bit 2), hist[0] bin 2 -1 and bin 3 +1;
+    # vertex 3: bitmap |= bit(3), hist bin 2 -1 -> 0 and bin 3 +1.
+    # Hubs (deg0 > 1): vertex 0 -> histogram row 0, vertex 2 -> row 1, vertex 3 -> row 2.
+    g, e = _engine(5, [(0, 2), (0, 4), (1, 3), (3, 2)], histogram_threshold=1)
+    e.set_words(np.array([_w(2, 1), _w(2, 4), _w(1, 1), _w(5, 1), _w(7, 1)], np.uint32))
+    e.set_bitmaps(np.array([0, 0b0011, 0, 0, 0], np.uint64))
+    h = e.hist()
+    h[:] = 0
+    h[0, 1] = 1          # vertex 0's histogram counts colour 2
+    h[2, 1] = 1          # vertex 3's histogram counts colour 2 (vertex 1)
+    e.set_hist(h)
+    # bitmap/histogram bookkeeping of the insert itself (color_select.cuh:160-183): vertex 0 gets
+    # +colour(1) = 2 in hist bin 2 (now 2) and bit 2; vertex 1 gets bit 2 (already set).
+    e.apply(np.array([0]), np.array([1]), np.array([0]))
+    w = e.words()
+    assert w.tolist() == [_w(2, 1), _w(3, 5), _w(1, 1), _w(5, 1), _w(7, 1)]
+    want = np.zeros((3, 1024), np.uint32)
+    want[0, 1] = 1       # 1 + 1 (insert) - 1 (propagated 2 -> 3)
+    want[0, 2] = 1
+    want[2, 2] = 1       # vertex 3: 2 -> 3
+    assert np.array_equal(e.hist(), want)
+    assert e.bitmaps().tolist() == [0b0110, 0b0011, 0, 0b0100, 0]
+
+
+def test_compaction_is_a_stable_left_pack():
+    # epoch_maintenance.cu:20-43: survivors keep their order, the tail becomes INVALID.
+    # Row 0 = [1 2 3 4 5] in a 20-slot row; deleting 2 and 4 leaves [1 _ 3 _ 5].
+    g, e = _engine(6, [(0, k) for k in range(1, 6)])
+    e.apply(np.array([0, 0]), np.array([2, 4]), np.array([1, 1]))
+    e.compact()
+    rp, col = e.csr()
+    inv = 0xFFFFFFFF
+    assert rp[1] - rp[0] == 20
+    assert col[rp[0]:rp[1]].tolist() == [1, 3, 5] + [inv] * 17
+    assert col[rp[2]:rp[3]].tolist() == [inv] * 8 and col[rp[1]:rp[2]].tolist() == [0] + [inv] * 7
+    # A later insert takes the first hole (slot 3); a re-insert of 3 finds it at slot 1 first.
+    e.apply(np.array([0, 0]), np.array([4, 3]), np.array([0, 0]))
+    rp, col = e.csr()
+    assert col[rp[0]:rp[0] + 5].tolist() == [1, 3, 5, 4, inv]
+
+
+def test_tier1_marks_its_own_colour_before_selecting():
+    # conflict_repair.cu:100-112: the loser ORs its own colour into its bitmap, then selects.
+    # Vertex 0 (pi below vertex 1's: equal degree, lower id) shares colour 2 with vertex 1;
+    # its bitmap {1} lacks colour 2, so without the own bit it would pick 2 again.
+    g, e = _engine(2, [(0, 1)])
+    e.set_words(np.array([_w(2, 3), _w(2, 9)], np.uint32))
+    e.set_bitmaps(np.array([0b001, 0], np.uint64))
+    assert e.tier1_sweep() == 1
+    assert e.words().tolist() == [_w(3, 4), _w(2, 9)]
+    assert e.bitmaps().tolist() == [0b011, 0]
+
+
+def test_tier1_stops_on_the_fourth_equal_conflict_count():
+    # streamgc.cu:1378-1389: a count equal to the previous one three times in a row stops the loop
+    # (counts c, c, c, c -> 4 sweeps).  Vertex 0 (histogram row, drifted: every bin 1 except its own
+    # colour 5) loses to vertex 1 on colour 5 in every sweep and re-selects colour 5, one conflict
+    # per sweep; histograms are not rebuilt (epoch_maintenance.cu:119-122), so the drift persists.
+    g, e = _engine(2, [(0, 1)], histogram_threshold=0)
+    e.set_words(np.array([_w(5, 0), _w(5, 0)], np.uint32))
+    h = e.hist()
+    h[:] = 1
+    h[0, 4] = 0
+    e.set_hist(h)
+    st = e.tier1()
+    assert st["t1_iterations"] == 4 and st["t1_conflicts_last"] == 1
+    assert e.words().tolist() == [_w(5, 4), _w(5, 0)]
+    # counts 2, 1, 1, 1, 1: a changed count resets the plateau -> 5 sweeps.  Vertices 2-3 hold a
+    # conflict that the first sweep fixes (vertex 2's exact histogram: colour 7 only -> colour 1).
+    g, e = _engine(4, [(0, 1), (2, 3)], histogram_threshold=0)
+    e.set_words(np.array([_w(5, 0), _w(5, 0), _w(7, 0), _w(7, 0)], np.uint32))
+    h = e.hist()
+    h[:] = 0
+    h[0, :] = 1
+    h[0, 4] = 0
+    h[2, 6] = 1
+    e.set_hist(h)
+    st = e.tier1()
+    assert st["t1_iterations"] == 5 and st["t1_conflicts_last"] == 1
+    assert e.words().tolist() == [_w(5, 5), _w(5, 0), _w(1, 1), _w(7, 0)]
+    # with the plateau out of reach the loop ends at max_repair_iters
+    g, e = _engine(2, [(0, 1)], histogram_threshold=0, plateau_iters=100, max_repair_iters=20)
+    e.set_words(np.array([_w(5, 0), _w(5, 0)], np.uint32))
+    h = e.hist()
+    h[:] = 1
+    h[0, 4] = 0
+    e.set_hist(h)
+    st = e.tier1()
+    assert st["t1_iterations"] == 20 and e.words().tolist() == [_w(5, 20), _w(5, 0)]
+
+
+def test_drain_retries_overflow_in_buffer_order():
+    # local_update.cu:27-38 buffers (0,3), (0,4), (0,5) in that order behind the full row [1 2];
+    # DEL(0,1) frees one slot; the reference-order epoch compacts row 0 to [2 _] and drains
+    # (epoch_maintenance.cu:66-87): the first entry takes the slot, the other two fail.  The
+    # reference drains with a thread per entry; one wave in entry order is the deterministic replay.
+    g, e = _engine(6, [(0, 1), (0, 2)], slack_pct=0, min_row_capacity=2, overflow_capacity=16)
+    e.apply(np.array([0, 0, 0, 0]), np.array([3, 4, 5, 1]), np.array([0, 0, 0, 1]))
+    assert e.overflow_count() == 3
+    st = e.epoch()
+    assert (st["overflow_count"], st["overflow_drained"], st["overflow_failed"]) == (3, 3, 2)
+    rp, col = e.csr()
+    assert col[rp[0]:rp[1]].tolist() == [2, 3]
+    assert e.overflow_count() == 0
+
+
+def test_rebuild_leaves_hub_bitmaps_and_histograms_alone():
+    # epoch_maintenance.cu:119-123: histogram vertices keep their bitmap; the reference never
+    # recounts histograms.  Star 0 - {1, 2, 3}, threshold 2: vertex 0 is the only hub.
+    g, e = _engine(4, [(0, 1), (0, 2), (0, 3)], histogram_threshold=2)
+    e.set_words(np.array([_w(4, 1), _w(1, 1), _w(2, 1), _w(3, 1)], np.uint32))
+    e.set_bitmaps(np.array([0xDEAD, 0, 0, 0], np.uint64))
+    h = e.hist()
+    h[:] = 0
+    h[0, 9] = 3          # drifted counts stay as they are
+    e.set_hist(h)
+    e.rebuild_bitmaps()
+    assert e.bitmaps().tolist() == [0xDEAD, 0b1000, 0b1000, 0b1000]
+    assert e.hist()[0, 9] == 3 and e.hist().sum() == 3
+    # the exact-histogram mode (no reference counterpart) recounts the hub's bins, bitmap still untouched
+    e.set_flags(O.FLAG_EXACT_HISTOGRAM)
+    e.rebuild_bitmaps()
+    want = np.zeros((1, 1024), np.uint32)
+    want[0, 0:3] = 1
+    assert np.array_equal(e.hist(), want)
+    assert e.bitmaps().tolist() == [0xDEAD, 0b1000, 0b1000, 0b1000]
+
+
+def test_tier1_colours_uncoloured_vertices_with_edges_and_counts_them():
+    # conflict_repair.cu:35-43: an uncoloured vertex with a live slot takes a colour, version +1,
+    # and counts as a conflict so the loop runs again; an isolated uncoloured vertex stays as it is.
+    g, e = _engine(3, [(0, 1)])
+    e.set_words(np.array([_w(0, 7), _w(1, 0), _w(0, 3)], np.uint32))
+    e.set_bitmaps(np.array([0b1, 0, 0], np.uint64))
+    assert e.tier1_sweep() == 1
+    assert e.words().tolist() == [_w(2, 8), _w(1, 0), _w(0, 3)]
+    # through the loop: sweep 1 counts it, sweep 2 finds nothing -> 2 iterations
+    g, e = _engine(3, [(0, 1)])
+    e.set_words(np.array([_w(0, 7), _w(1, 0), _w(0, 3)], np.uint32))
+    st = e.tier1()
+    assert st["t1_iterations"] == 2 and st["t1_conflicts_last"] == 0
+    assert e.words().tolist() == [_w(2, 8), _w(1, 0), _w(0, 3)]
