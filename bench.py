@@ -1,22 +1,33 @@
 #!/usr/bin/env python3
 """StreamGC headline benchmark on one MI355X.
 
-Workload (default "c2"): Chung-Lu graph of com-LiveJournal shape (3,997,962
+Default workload "c5", the headline metric (served vs plain, p50/p99
+update-to-snapshot latency): 64 tenant graphs of the C1-C4 shapes plus a
+com-Orkut-shape Chung-Lu graph (3,072,441 vertices, 117M edges), all
+resident in one pool.  Requests of U[1, 8192] updates to uniform tenants
+arrive as a Poisson process at offered rates of 1-50M updates/s, replayed
+open-loop by the library, with queries of U[1, 1024] vertices every 10
+requests.  The same native request loop serves them in two modes: served
+(continuous batching, shared epoch steps) and plain (the per-launch engine:
+one launch per request, one epoch step per tenant).  The value is the highest
+served rate whose p99 update-to-snapshot latency stays within one tenant
+epoch; served_vs_plain divides it by plain's achieved rate at the same load.
+
+Workload "c2": Chung-Lu graph of com-LiveJournal shape (3,997,962
 vertices, 34,681,189 edges, degree exponent 2.3, max degree ~15K, seed 42,
 random vertex permutation), churn stream at 50% insert (seed 42), epoch
 E = 32,768, histogram threshold 64, plain per-update GPU engine.  The init
 stream and its repair run first, untimed, in the reference's epoch order
 (launches of --init-batch updates: fewer concurrent first colourings give a
 colouring as tight as the sequential replay's); the measured phase repairs and
-publishes a proper snapshot every epoch.
-
-A step is one batch of E updates: the per-update kernel over the batch, then
-the epoch (compact, drain, Tier-1, Tier-2 check, publish).  Inputs are
-resident in HBM before timing.  K steps run host-driven (their per-step
-latency and per-kernel HIP-event timings feed latency_ms and the roofline),
-then K more run as captured hipGraph rounds, which give the value.  Every
-published snapshot is validated afterwards against the stream's live edge
-set (independent validator).
+publishes a proper snapshot every epoch.  A step is one batch of E updates:
+the per-update kernel over the batch, then the epoch (compact, drain, Tier-1,
+Tier-2 check, publish).  Inputs are resident in HBM before timing.  K steps
+run host-driven (their per-step latency and per-kernel HIP-event timings feed
+latency_ms and the roofline), then K more run as captured hipGraph rounds,
+which give the value.  Every published snapshot is validated afterwards
+against the stream's live edge set (independent validator).  Workloads c1,
+c3 (plain), c4 (batched) and trc (the CAS microbenchmark) are also here.
 
 Prints one JSON line (rank 0, one GPU).
 """
@@ -154,7 +165,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-batches", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
@@ -167,8 +178,12 @@ def main():
                     help="tuning: graphs of at most this many vertices run Tier-1 in vertex order")
     ap.add_argument("--profile-summary", default=None, metavar="DIR",
                     help="summarise the rocprofv3 output under DIR (timed window = the last --steps steps) and exit")
-    ap.add_argument("--requests", type=int, default=2000, help="served workload: requests per phase")
-    ap.add_argument("--load", type=float, default=0.5, help="served workload: offered load / served capacity")
+    ap.add_argument("--requests", type=int, default=2000, help="served workload: requests of each capacity phase")
+    ap.add_argument("--point-seconds", type=float, default=1.0,
+                    help="served workload: arrivals per sweep point span about this long (within the bounds below)")
+    ap.add_argument("--min-requests", type=int, default=250, help="served workload: fewest requests per sweep point")
+    ap.add_argument("--max-requests", type=int, default=3000, help="served workload: most requests per sweep point")
+    ap.add_argument("--max-rate", type=float, default=50.0, help="served workload: highest offered rate (M/s)")
     ap.add_argument("--max-delay-us", type=int, default=2000, help="served workload: publish deadline")
     ap.add_argument("--stream-type", default=None,
                     choices=["churn", "uniform", "holdout", "dense", "dense_growth"],
@@ -611,31 +626,104 @@ def pct(a, q):
     return round(float(np.percentile(a, q)), 3) if len(a) else None
 
 
-def run_served(S, args, w):
-    """Served vs plain on the mixed-tenant workload.
+SWEEP_RATES = (1e6, 2e6, 5e6, 10e6, 20e6, 30e6, 40e6, 50e6)
+MEAN_BATCH = 4096.5          # U[1, 8192] updates per request
 
-    plain: the per-launch engine -- every request is its own launch
-    (sgc_pool_apply of that request), and a tenant's epoch runs on its own as
-    soon as it is due.  served: the request loop packs every request that has
-    arrived into one launch and runs one shared epoch step for the due
-    tenants.  Both start from the same resident state; plain takes the first
-    half of every tenant's churn, served the second half (same distribution).
-    Throughput is measured at saturation (every request available at once);
-    latency at an offered Poisson load below the served capacity.
+
+def oracle_rate(jobs, threads, runs=5, warmup=1):
+    """The oracle (sequential updates, OpenMP sweeps on `threads` threads) on
+    `jobs` = [(csr, (words, col, bitmaps, hist), records, E)], each from its
+    state; timed from the first update to the last published snapshot over
+    all jobs; median updates/s of `runs` runs after `warmup` untimed ones
+    (the engines are rebuilt from the state before every run, untimed)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    rates = []
+    for r in range(warmup + runs):
+        engines = []
+        for g, (words, col, bitmaps, hist), recs, E in jobs:
+            cfg = O.default_config(epoch_size=E, batch_size=E, flags=O.FLAG_REPAIR_EVERY_EPOCH | O.FLAG_EXACT_HISTOGRAM)
+            eng = O.Engine(O.Csr(g.n, g.row_ptr, g.col_idx, g.degree), cfg, threads=threads)
+            eng.set_col(col)
+            eng.set_words(words)
+            eng.set_bitmaps(bitmaps)
+            if hist.size:
+                eng.set_hist(hist)
+            engines.append(eng)
+        n = 0
+        t0 = time.perf_counter()
+        for eng, (_, _, recs, _) in zip(engines, jobs):
+            eng.submit(recs["u"], recs["v"], recs["type"])
+            n += len(recs)
+        dt = time.perf_counter() - t0
+        if r >= warmup:
+            rates.append(n / dt)
+        del engines
+    return float(np.median(rates)), rates
+
+
+def cpu_baseline_both(jobs, sample):
+    """cpu_baseline entry: the box's CPU share (at most 16 threads) and 1 thread."""
+    share = min(16, os.cpu_count() or 1)
+    rate, runs = oracle_rate(jobs, share)
+    rate1, runs1 = oracle_rate(jobs, 1)
+    log(f"cpu baseline: {rate:.0f} updates/s on {share} threads, {rate1:.0f} on 1 thread")
+    return dict(value=round(rate, 1), unit="updates/s", cores=share, kind="port",
+                sample=sample + "; median of 5 runs after 1 warm-up, each from the same state",
+                runs=[round(x, 1) for x in runs], one_thread=dict(value=round(rate1, 1), cores=1,
+                                                                  runs=[round(x, 1) for x in runs1]))
+
+
+def roofline_of(ktimes, d, n, workload):
+    """Per-kernel algorithmic GB/s from HIP-event times and the device counters of
+    the same window; the roofline entry for the kernel with the most time."""
+    live_frac = d["live_slots"] / d["slots_compact"] if d["slots_compact"] else 0.25
+    per_kernel = {}
+    for name, (ms, launches) in ktimes.items():
+        b = kernel_bytes(name, launches, n, d, live_frac)
+        avg_ms = ms / max(launches, 1)
+        per_kernel[name] = dict(total_ms=round(ms, 3), launches=int(launches), avg_us=round(avg_ms * 1e3, 2),
+                                bytes_per_launch=int(b),
+                                gbs=round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 and b else None)
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["total_ms"])
+    pk = per_kernel[dom]
+    traffic, tsrc = measured_traffic(workload, dom)
+    roofline = dict(bound="hbm", kernel=dom, achieved=pk["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(pk["gbs"] / HBM_PEAK_GBS, 4) if pk["gbs"] else None, traffic=traffic,
+                    traffic_source=tsrc, bytes_per_launch=pk["bytes_per_launch"], avg_launch_us=pk["avg_us"])
+    return roofline, per_kernel
+
+
+def run_served(S, args, w):
+    """Served vs plain on the mixed-tenant workload (64 tenants + the Orkut shape).
+
+    Both modes run the same native request loop over the same resident pool
+    (sgc_server): served packs every arrived request into one launch and runs
+    one shared epoch step for the due tenants (continuous batching); plain is
+    the per-launch engine -- the earliest request in a launch of its own, one
+    epoch step per due tenant -- under the same epoch (E_g = 4,096) and
+    deadline (--max-delay-us) rules.  Requests of U[1, 8192] updates to uniform
+    tenants arrive as a Poisson process, replayed open-loop by the library at
+    each offered rate of the sweep, with a query of U[1, 1024] vertices every
+    10 requests; latencies run from each request's scheduled arrival to the
+    publish of the snapshot that holds it.  Each point takes the next slice of
+    every tenant's churn (50% insert).  The value is the highest served rate
+    whose p99 update-to-snapshot latency is within one tenant epoch (E_g over
+    the per-tenant arrival rate) at that point; served_vs_plain divides it by
+    the plain engine's achieved rate at the same offered load.
     """
     E = 4096
-    R = args.requests
     t = time.perf_counter()
     tenants = make_tenants(S, 64)
     orkut = S.gen_chung_lu(3072441, 117185083, 2.3, 33313.0, seed=42, permute=True)
     tenants.append(("orkut", orkut))
+    T = len(tenants)
     log(f"tenants: 64 + Orkut-shape ({orkut.n} vertices, {orkut.num_edges // 2} edges) "
         f"({time.perf_counter() - t:.1f}s)")
     # the untimed init streams are applied without intermediate epochs, then
     # repaired once; the timed phases cut every tenant's epochs at E
     cfg = S.StreamGCConfig(epoch_size=1 << 30, batch_size=1 << 16, exact_histogram=True, repair_every_epoch=True)
-    pool = S.GraphPool(max_graphs=len(tenants), batch_capacity=1 << 16, repair_every_epoch=True,
-                       exact_histogram=True)
+    pool = S.GraphPool(max_graphs=T, batch_capacity=1 << 16, repair_every_epoch=True, exact_histogram=True)
     gids = [pool.add(g, cfg) for _, g in tenants]
     used, budget, ng = pool.usage()
     t = time.perf_counter()
@@ -644,111 +732,180 @@ def run_served(S, args, w):
     pool.finish()
     log(f"init of {ng} tenants ({used / 2**30:.2f} GiB resident): {time.perf_counter() - t:.1f}s")
 
-    # requests: uniform tenant, U[1, 8192] updates, each tenant's churn in order
+    # every phase's requests drawn up front, then each tenant's churn long enough for all of them
     rng = np.random.default_rng(7)
-    picks = rng.integers(0, len(tenants), 3 * R)
-    sizes = rng.integers(1, 8193, 3 * R)
-    need = np.bincount(picks, weights=sizes, minlength=len(tenants)).astype(np.int64)
-    # 50% inserts (restoring deleted edges) keep every tenant near its loaded
-    # size however many requests it receives
+    R = args.requests
+    rates = [r for r in SWEEP_RATES if r <= args.max_rate * 1e6]
+    phases = [("plain_sat", R, None), ("served_sat", R, None)]
+    for r in rates:
+        nreq = int(np.clip(r * args.point_seconds / MEAN_BATCH, args.min_requests, args.max_requests))
+        phases += [("served", nreq, r), ("plain", nreq, r)]
+    draws = []
+    for name, nreq, rate in phases:
+        picks = rng.integers(0, T, nreq)
+        sizes = rng.integers(1, 8193, nreq)
+        gaps = rng.exponential(MEAN_BATCH / rate, nreq) if rate else None
+        draws.append((picks, sizes, gaps))
+    need = np.zeros(T, np.int64)
+    for picks, sizes, _ in draws:
+        need += np.bincount(picks, weights=sizes, minlength=T).astype(np.int64)
+    t = time.perf_counter()
     churn = [S.generate_bench_stream_edges(g, int(need[k]) + 1, 50, 42 + k) for k, (_, g) in enumerate(tenants)]
-    head = [0] * len(tenants)
-    reqs = []
-    for k, n in zip(picks, sizes):
-        reqs.append((int(k), S.tag_graph(churn[k][head[k]:head[k] + n], gids[k])))
-        head[k] += int(n)
-    plain_reqs, served_reqs, lat_reqs = reqs[:R], reqs[R:2 * R], reqs[2 * R:]
+    log(f"churn streams: {int(need.sum())} updates ({time.perf_counter() - t:.1f}s)")
+    head = np.zeros(T, np.int64)
 
-    # plain: one launch per request, per-tenant epochs when due
-    since = [0] * len(tenants)
-    pool.kernel_timing(True)
-    tot0 = pool.totals()
-    t0 = time.perf_counter()
-    for k, recs in plain_reqs:
-        pool.apply(recs)
-        since[k] += len(recs)
-        if since[k] >= E:
-            pool.epoch([gids[k]])
-            since[k] = 0
-    pool.epoch([gids[k] for k in range(len(tenants)) if since[k]])
-    plain_s = time.perf_counter() - t0
-    plain_n = sum(len(r) for _, r in plain_reqs)
-    plain_rate = plain_n / plain_s
-    plain_k = pool.kernel_times()
-    tot1 = pool.totals()
-    plain_tot = {k: tot1[k] - tot0[k] for k in ("t1_sweeps", "t2_runs", "overflow_events")}
-    pool.kernel_timing(False)
-    pool.kernel_timing(True)
-    log(f"plain: {len(plain_reqs)} requests, {plain_n} updates in {plain_s:.2f}s = {plain_rate / 1e6:.2f} M updates/s")
+    def build(picks, sizes, gaps):
+        """The phase's requests (tenant index, records) and, with gaps, its replay trace."""
+        reqs, parts = [], []
+        off = 0
+        items = np.zeros(len(picks) + (len(picks) + 9) // 10, dtype=S.TRACE_ITEM)
+        qids = []
+        qoff = 0
+        at = np.concatenate([[0.0], np.cumsum(gaps[1:])]) if gaps is not None else np.zeros(len(picks))
+        k_item = 0
+        for j, (k, n) in enumerate(zip(picks.tolist(), sizes.tolist())):
+            recs = churn[k][head[k]:head[k] + n]
+            head[k] += n
+            reqs.append((k, recs))
+            parts.append(recs)
+            items[k_item] = (int(at[j] * 1e9), off, gids[k], n, 0, 0)
+            off += n
+            k_item += 1
+            if j % 10 == 9:
+                qk = int(rng.integers(0, T))
+                ids = rng.integers(0, tenants[qk][1].n, int(rng.integers(1, 1025))).astype(np.uint32)
+                qids.append(ids)
+                items[k_item] = (int(at[j] * 1e9), qoff, gids[qk], len(ids), 1, 0)
+                qoff += len(ids)
+                k_item += 1
+        upd = np.concatenate(parts) if parts else np.zeros(0, S.EDGE_UPDATE)
+        q = np.concatenate(qids) if qids else np.zeros(1, np.uint32)
+        return reqs, items[:k_item], upd, q
 
-    # served at saturation
-    srv = S.GraphServer(pool, max_batch=1 << 16, max_delay_us=args.max_delay_us)
-    for k, gid in enumerate(gids):
-        srv.attach(gid, E)
-    t0 = time.perf_counter()
-    for k, recs in served_reqs:
-        srv.submit(gids[k], recs)
-    srv.flush()
-    served_s = time.perf_counter() - t0
-    served_n = sum(len(r) for _, r in served_reqs)
-    served_rate = served_n / served_s
-    st = srv.stats()
-    served_k = pool.kernel_times()
-    tot2 = pool.totals()
-    served_tot = {k: tot2[k] - tot1[k] for k in ("t1_sweeps", "t2_runs", "overflow_events")}
-    pool.kernel_timing(False)
-    log(f"served: {served_n} updates in {served_s:.2f}s = {served_rate / 1e6:.2f} M updates/s "
-        f"({st['batches']} launches, {st['epoch_steps']} epoch steps for {st['graph_epochs']} graph epochs)")
+    def new_server(mode):
+        srv = S.GraphServer(pool, max_batch=1 << 16, max_delay_us=args.max_delay_us, mode=mode)
+        for gid in gids:
+            srv.attach(gid, E)
+        return srv
 
-    # latency at an offered Poisson load of args.load x the served capacity
-    srv.reset_stats()
-    rate = args.load * served_rate
-    gaps = rng.exponential(1.0, len(lat_reqs)) * (np.mean([len(r) for _, r in lat_reqs]) / rate)
-    t0 = time.perf_counter()
-    due = t0
-    qids = [rng.integers(0, tenants[k][1].n, int(rng.integers(1, 1025))).astype(np.uint32) for k, _ in lat_reqs]
-    for j, (k, recs) in enumerate(lat_reqs):
-        due += gaps[j]
-        now = time.perf_counter()
-        if due > now:
-            time.sleep(due - now)
-        srv.submit(gids[k], recs)
-        if j % 10 == 9:
-            srv.query(gids[k], qids[j])
-    srv.flush()
-    lat_s = time.perf_counter() - t0
-    lat = srv.latencies("update") / 1e3
-    qlat = srv.latencies("query") / 1e3
-    lat_n = sum(len(r) for _, r in lat_reqs)
-    epoch_period_ms = E / (rate / len(tenants)) * 1e3
-    log(f"at {rate / 1e6:.1f} M updates/s offered: p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms; "
-        f"query p50 {pct(qlat, 50)} ms p99 {pct(qlat, 99)} ms; one tenant epoch = {epoch_period_ms:.1f} ms")
-    srv.close()
+    def saturation(mode, reqs):
+        """Every request available at once (closed loop): the mode's capacity."""
+        srv = new_server(mode)
+        tot0 = pool.totals()
+        pool.kernel_timing(True)
+        t0 = time.perf_counter()
+        for k, recs in reqs:
+            srv.submit(gids[k], recs)
+        srv.flush()
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        kt = pool.kernel_times()
+        pool.kernel_timing(False)
+        srv.close()
+        tot1 = pool.totals()
+        n = sum(len(r) for _, r in reqs)
+        d = {k: tot1[k] - tot0[k] for k in tot1}
+        return dict(value=round(n / dt, 1), seconds=round(dt, 3), updates=n,
+                    launches=st["batches"], epoch_steps=st["epoch_steps"], graph_epochs=st["graph_epochs"]), kt, d
+
+    # capacities (closed loop)
+    reqs_p, _, _, _ = build(*draws[0])
+    sat_plain, kt_p, d_p = saturation(S.GraphServer.PLAIN, reqs_p)
+    log(f"plain capacity: {sat_plain['value'] / 1e6:.2f} M updates/s ({sat_plain['launches']} launches, "
+        f"{sat_plain['epoch_steps']} epoch steps)")
+    reqs_s, _, _, _ = build(*draws[1])
+    sat_served, kt_s, d_s = saturation(S.GraphServer.BATCHED, reqs_s)
+    log(f"served capacity: {sat_served['value'] / 1e6:.2f} M updates/s ({sat_served['launches']} launches, "
+        f"{sat_served['epoch_steps']} epoch steps for {sat_served['graph_epochs']} graph epochs)")
+    roofline, per_kernel = roofline_of(kt_s, d_s, sum(g.n for _, g in tenants), "c5")
+
+    # the offered-load sweep (open loop)
+    points = []
+    for pi in range(2, len(phases), 2):
+        rate = phases[pi][2]
+        point = {"offered_updates_per_s": rate, "requests": int(phases[pi][1]),
+                 "tenant_epoch_ms": round(E / (rate / T) * 1e3, 3)}
+        for ph in (pi, pi + 1):
+            mode_name = phases[ph][0]
+            _, items, upd, q = build(*draws[ph])
+            srv = new_server(S.GraphServer.PLAIN if mode_name == "plain" else S.GraphServer.BATCHED)
+            rs = srv.replay(items, upd, q)
+            lat = srv.latencies("update") / 1e3
+            qlat = srv.latencies("query") / 1e3
+            st = srv.stats()
+            srv.close()
+            point[mode_name] = {
+                "achieved_updates_per_s": round(rs["updates"] / (rs["span_us"] * 1e-6), 1),
+                "update_to_snapshot_p50_ms": pct(lat, 50), "update_to_snapshot_p99_ms": pct(lat, 99),
+                "query_p50_ms": pct(qlat, 50), "query_p99_ms": pct(qlat, 99),
+                "launches": int(st["batches"]), "epoch_steps": int(st["epoch_steps"]),
+                "submit_late_max_us": round(rs["max_late_us"], 1), "queries": int(rs["queries"])}
+            m = point[mode_name]
+            log(f"{rate / 1e6:4.0f}M/s offered, {mode_name:6s}: {m['achieved_updates_per_s'] / 1e6:6.2f} M/s, "
+                f"p50 {m['update_to_snapshot_p50_ms']} ms p99 {m['update_to_snapshot_p99_ms']} ms "
+                f"(epoch {point['tenant_epoch_ms']} ms); query p99 {m['query_p99_ms']} ms; "
+                f"{m['launches']} launches, {m['epoch_steps']} epoch steps")
+        points.append(point)
+
+    def compliant(mode):
+        return [p for p in points if p[mode]["update_to_snapshot_p99_ms"] is not None and
+                p[mode]["update_to_snapshot_p99_ms"] <= p["tenant_epoch_ms"]]
+
+    ok_s, ok_p = compliant("served"), compliant("plain")
+    best = max(ok_s, key=lambda p: p["served"]["achieved_updates_per_s"]) if ok_s else None
+    value = best["served"]["achieved_updates_per_s"] if best else 0.0
+    ratio = round(value / best["plain"]["achieved_updates_per_s"], 2) if best else None
+    plain_best = max((p["plain"]["achieved_updates_per_s"] for p in ok_p), default=0.0)
+    log(f"served sustains {value / 1e6:.2f} M updates/s at p99 <= one tenant epoch "
+        f"(offered {best['offered_updates_per_s'] / 1e6 if best else 0:.0f}M/s); plain at that load "
+        f"{best['plain']['achieved_updates_per_s'] / 1e6 if best else 0:.2f} M/s -> {ratio}x; "
+        f"plain's own best within the bound {plain_best / 1e6:.2f} M/s")
+
+    # every tenant's published snapshot proper, colour counts per shape
+    t = time.perf_counter()
     proper = all(pool.validate(gid, snapshot=True)[:2] == (0, 0) for gid in gids)
     max_colors = {kind: 0 for kind, _ in tenants}
     for gid, (kind, g) in zip(gids, tenants):
         max_colors[kind] = max(max_colors[kind], pool.validate(gid)[2])
+    log(f"validated {len(gids)} published snapshots: proper={proper} ({time.perf_counter() - t:.1f}s)")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # a bounded sample: 8 small tenants (two of each shape), 2 epochs of each one's next churn
+        jobs = []
+        for k in range(8):
+            g = tenants[k][1]
+            st = (pool.words(gids[k]), pool.csr(gids[k])[1], pool.bitmaps(gids[k]), pool.hist(gids[k]))
+            more = S.generate_bench_stream_edges(g, int(head[k]) + 2 * E, 50, 42 + k)[int(head[k]):]
+            jobs.append((g, st, more, E))
+        cpu = cpu_baseline_both(jobs, f"oracle engines of 8 tenants (two of each small shape), {2 * E} churn "
+                                      f"updates each with an epoch (compact, drain, Tier-1, Tier-2, publish) every "
+                                      f"{E}, from their state after the sweep")
+    lat_best = best["served"] if best else {}
     out = {
-        "metric": METRIC, "value": round(served_rate, 1), "unit": "updates/s", "n_gpus": 1, "steps": R,
-        "warmup": 0, "ms_per_step": round(served_s * 1e3 / R, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": w["name"], "tenants": len(tenants), "resident_gib": round(used / 2**30, 2),
-                   "requests_per_phase": R, "epoch_size": E, "max_delay_us": args.max_delay_us},
-        "plain": {"value": round(plain_rate, 1), "seconds": round(plain_s, 3), "updates": plain_n,
-                  "totals": plain_tot, "kernels_ms": {k: round(v[0], 2) for k, v in plain_k.items()}},
-        "served_vs_plain": round(served_rate / plain_rate, 2),
-        "served": {"seconds": round(served_s, 3), "updates": served_n, **{k: st[k] for k in
-                   ("batches", "epoch_steps", "graph_epochs")}, "totals": served_tot,
-                   "kernels_ms": {k: round(v[0], 2) for k, v in served_k.items()}},
-        "latency_ms": {"offered_updates_per_s": round(rate, 1), "achieved_updates_per_s": round(lat_n / lat_s, 1),
-                       "update_to_snapshot_p50": pct(lat, 50), "update_to_snapshot_p99": pct(lat, 99),
-                       "query_p50": pct(qlat, 50), "query_p99": pct(qlat, 99),
-                       "tenant_epoch_period_ms": round(epoch_period_ms, 2), "requests": int(len(lat)),
-                       "queries": int(len(qlat))},
+        "metric": METRIC, "value": round(value, 1), "unit": "updates/s", "n_gpus": 1,
+        "steps": int(best["requests"]) if best else 0, "warmup": 0,
+        "ms_per_step": round(MEAN_BATCH / value * 1e3, 4) if value else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": w["name"], "tenants": T, "resident_gib": round(used / 2**30, 2), "epoch_size": E,
+                   "max_delay_us": args.max_delay_us, "offered_rates": [r for r in rates],
+                   "step": "one request of U[1,8192] updates (mean 4096.5), applied and published"},
+        "served_vs_plain": ratio,
+        "offered_at_value": best["offered_updates_per_s"] if best else None,
+        "latency_ms": {"update_to_snapshot_p50": lat_best.get("update_to_snapshot_p50_ms"),
+                       "update_to_snapshot_p99": lat_best.get("update_to_snapshot_p99_ms"),
+                       "query_p50": lat_best.get("query_p50_ms"), "query_p99": lat_best.get("query_p99_ms"),
+                       "tenant_epoch_ms": best["tenant_epoch_ms"] if best else None},
+        "plain_best_within_bound": round(plain_best, 1),
+        "sweep": points,
+        "capacity": {"served": sat_served, "plain": sat_plain,
+                     "ratio": round(sat_served["value"] / sat_plain["value"], 2)},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "kernels_served": per_kernel,
         "parity": {"all_snapshots_proper": proper, "max_color_by_shape": max_colors},
-        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
+
 
 if __name__ == "__main__":
     main()

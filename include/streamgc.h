@@ -378,12 +378,18 @@ int sgc_batch_release(sgc_engine* e);
  * completes when the snapshot holding its last update is published; its
  * update-to-snapshot latency is kept raw.  Admissions wait in arrival order
  * until the pool's HBM budget has room.  While a server runs, its pool is
- * driven only by the server. */
+ * driven only by the server.  SGC_SERVE_PLAIN runs the same loop as the plain
+ * per-launch engine would serve the tenants (the baseline of the served
+ * pattern). */
 typedef struct sgc_server sgc_server;
+#define SGC_SERVE_BATCHED 0u  /* continuous batching: every arrived request in one launch, shared epoch steps */
+#define SGC_SERVE_PLAIN   1u  /* the plain per-launch engine: one launch per request (oldest arrival first),
+                                 one epoch step per due graph; same epoch and deadline rules */
 typedef struct sgc_server_config {
     uint32_t max_batch;     /* updates per launch over all tenants (default 32768) */
     uint32_t max_delay_us;  /* epoch for a graph whose oldest unpublished update waited this long (0: full epochs only) */
-    uint32_t reserved[2];
+    uint32_t mode;          /* SGC_SERVE_BATCHED or SGC_SERVE_PLAIN */
+    uint32_t reserved;
 } sgc_server_config;
 typedef struct sgc_server_stats {
     uint64_t updates_applied;
@@ -420,6 +426,31 @@ int sgc_server_get_stats(sgc_server* s, sgc_server_stats* out);
 /* Raw latencies in microseconds: which 0 = update-to-snapshot per request, 1 = query. */
 int sgc_server_latencies(sgc_server* s, int which, double* out, uint64_t cap, uint64_t* count);
 int sgc_server_reset_stats(sgc_server* s);
+/* Open-loop trace replay (no reference counterpart; the reference's feeder,
+ * A/src/host/feeder_thread.cpp:28-138, is never started).  Each item arrives
+ * at_ns after the call starts: update items are submitted from the calling
+ * thread, queries from a second thread, each at its own time, and latencies
+ * run from that scheduled arrival (not from when the submission happened).
+ * Items must be in non-decreasing at_ns per kind.  Returns after everything is
+ * published (a flush), or SGC_ERR_STATE after timeout_ms. */
+typedef struct sgc_trace_item {
+    uint64_t at_ns;         /* arrival, ns after the replay starts */
+    uint64_t offset;        /* first record in `updates` (updates) or first id in `query_ids` (query) */
+    uint32_t gid;
+    uint32_t count;
+    uint32_t kind;          /* 0: updates, 1: query */
+    uint32_t reserved;
+} sgc_trace_item;
+typedef struct sgc_replay_stats {
+    double span_us;         /* replay start to the last publish */
+    double max_late_us;     /* largest delay of a submission behind its schedule */
+    double mean_late_us;
+    uint64_t requests;
+    uint64_t updates;
+    uint64_t queries;
+} sgc_replay_stats;
+int sgc_server_replay(sgc_server* s, const sgc_trace_item* items, uint64_t count, const sgc_edge_update* updates,
+                      const uint32_t* query_ids, uint32_t timeout_ms, sgc_replay_stats* out);
 
 /* ---- the reference's result files (A/src/streamgc.cu:595-642,1469-1573) ----
  * sgc_write_coloring writes <path> (raw u16 per vertex) and <path>.txt (the

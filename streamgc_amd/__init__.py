@@ -15,9 +15,9 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import EDGE_UPDATE, SgcError, check, lib
+from ._lib import EDGE_UPDATE, TRACE_ITEM, SgcError, check, lib
 
-__all__ = ["StreamGCConfig", "default_config", "CSRGraph", "GPUPartition", "DeviceStream", "EDGE_UPDATE",
+__all__ = ["StreamGCConfig", "default_config", "CSRGraph", "GPUPartition", "DeviceStream", "EDGE_UPDATE", "TRACE_ITEM",
            "load_graph_csr", "csr_from_edges", "csr_from_edge_arrays", "gen_chung_lu", "gen_rmat",
            "gen_delaunay", "core_numbers", "dense_growth_stream", "dense_stream", "uniform_stream", "holdout_stream",
            "generate_init_stream", "generate_bench_stream_edges",
@@ -762,11 +762,17 @@ class GraphServer:
     shared epoch steps for the graphs that are due.  While it runs, the pool
     is driven only by the server."""
 
-    def __init__(self, pool, max_batch=32768, max_delay_us=0):
+    BATCHED = 0
+    PLAIN = 1
+
+    def __init__(self, pool, max_batch=32768, max_delay_us=0, mode=0):
+        """mode GraphServer.PLAIN serves the same requests as the plain per-launch
+        engine would (one launch per request, one epoch step per due graph)."""
         cfg = _lib.ServerConfig()
         lib().sgc_default_server_config(C.byref(cfg))
         cfg.max_batch = max_batch
         cfg.max_delay_us = max_delay_us
+        cfg.mode = mode
         h = C.c_void_p()
         check(lib().sgc_server_create(pool.h, C.byref(cfg), C.byref(h)))
         self.h = h.value
@@ -862,6 +868,18 @@ class GraphServer:
 
     def reset_stats(self):
         self._ck(lib().sgc_server_reset_stats(self.h))
+
+    def replay(self, items, updates, query_ids=None, timeout_ms=600000):
+        """Open-loop replay of a trace (TRACE_ITEM records, at_ns after the start):
+        updates and queries submitted at their scheduled times, latencies from
+        those times.  Returns the replay statistics once everything is published."""
+        items = np.ascontiguousarray(items, dtype=TRACE_ITEM)
+        updates = np.ascontiguousarray(updates, dtype=EDGE_UPDATE)
+        qids = np.ascontiguousarray(query_ids if query_ids is not None else np.zeros(1, np.uint32), np.uint32)
+        st = _lib.ReplayStats()
+        self._ck(lib().sgc_server_replay(self.h, items.ctypes.data, len(items), updates.ctypes.data,
+                                         qids.ctypes.data, timeout_ms, C.byref(st)))
+        return st.as_dict()
 
 
 def write_coloring(path, colors):

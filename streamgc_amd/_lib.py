@@ -28,6 +28,9 @@ ABLATIONS = {"force_bitmap": SGC_ABL_FORCE_BITMAP, "force_histogram": SGC_ABL_FO
 # numpy view of the 16-byte update record (A/include/types.h:16-21)
 EDGE_UPDATE = np.dtype([("u", "<u4"), ("v", "<u4"), ("type", "u1"), ("pad", "u1", 3), ("graph", "<u4")])
 assert EDGE_UPDATE.itemsize == 16
+# One arrival of a replayed trace (sgc_trace_item): kind 0 = updates, 1 = query.
+TRACE_ITEM = np.dtype([("at_ns", "<u8"), ("offset", "<u8"), ("gid", "<u4"), ("count", "<u4"), ("kind", "<u4"),
+                       ("reserved", "<u4")])
 
 
 class Config(C.Structure):
@@ -68,7 +71,21 @@ class Tuning(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("max_batch", C.c_uint32), ("max_delay_us", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("max_batch", C.c_uint32), ("max_delay_us", C.c_uint32), ("mode", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class TraceItem(C.Structure):
+    _fields_ = [("at_ns", C.c_uint64), ("offset", C.c_uint64), ("gid", C.c_uint32), ("count", C.c_uint32),
+                ("kind", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ReplayStats(C.Structure):
+    _fields_ = [("span_us", C.c_double), ("max_late_us", C.c_double), ("mean_late_us", C.c_double)] + \
+               [(k, C.c_uint64) for k in ("requests", "updates", "queries")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ServerStats(C.Structure):
@@ -230,6 +247,7 @@ SIGNATURES = {
     "sgc_server_get_stats": [_vp, C.POINTER(ServerStats)],
     "sgc_server_latencies": [_vp, C.c_int, _vp, C.c_uint64, C.POINTER(C.c_uint64)],
     "sgc_server_reset_stats": [_vp],
+    "sgc_server_replay": [_vp, _vp, C.c_uint64, _vp, _vp, C.c_uint32, C.POINTER(ReplayStats)],
     "sgc_write_coloring": [C.c_char_p, _u16p, C.c_uint32],
     "sgc_write_meta": [C.c_char_p, C.POINTER(RunMeta)],
     "sgc_csr_load_mtx": [C.c_char_p, C.POINTER(_vp)],

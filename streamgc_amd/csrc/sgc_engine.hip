@@ -34,6 +34,7 @@
 #include "sgc_device.hpp"
 #include "sgc_internal.hpp"
 #include "sgc_launch.hpp"
+#include "sgc_round_control.hpp"
 #include "streamgc.h"
 
 using namespace sgc;
@@ -1546,8 +1547,16 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
             if (e == hipSuccess) e = hipMemcpyAsync(B.h_ctl, B.d_ctl, RC_WORDS * 4, hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             if (e != hipSuccess) break;
-            const bool stopped = B.h_ctl[RC_STOP] != 0;
-            const uint32_t done = stopped ? B.h_ctl[RC_STOP_ROUND] : end;
+            // the replays either all ran or stopped at one round in [r, end);
+            // any other reading ends the run (each accepted one moves r on)
+            RoundReading rd;
+            if (!read_rounds(r, end, B.rounds, B.h_ctl[RC_ROUND], B.h_ctl[RC_STOP], B.h_ctl[RC_STOP_ROUND], &rd)) {
+                rc = fail(SGC_ERR_STATE, "batch run: inconsistent round control after replays of rounds " +
+                                             std::to_string(r) + ".." + std::to_string(end));
+                break;
+            }
+            const bool stopped = rd.stopped;
+            const uint32_t done = rd.done;
             if (!tuned)
                 for (uint32_t k = r; k < done; k++) {
                     float ms = 0;
@@ -1556,7 +1565,7 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
                     cand_upd[k % C] += B.tab[2 * k + 1];
                 }
             if (stopped) {
-                const uint32_t s = B.h_ctl[RC_STOP_ROUND];
+                const uint32_t s = rd.host;
                 S.host_rounds++;
                 rc = batch_host_round(B, s);
                 if (rc) break;
@@ -1575,7 +1584,7 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
                     }
                     if (rc) break;
                 }
-                r = s + 1;
+                r = rd.next;
                 B.h_ctl[RC_ROUND] = r;
                 B.h_ctl[RC_STOP] = 0;
                 B.h_ctl[RC_T2_MISSING] = 0;

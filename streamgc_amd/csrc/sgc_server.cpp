@@ -9,7 +9,13 @@
 // epoch step for every graph that is due: its epoch is full, or its oldest
 // unpublished update has waited max_delay_us.  A request completes when the
 // snapshot holding its last update is published; its update-to-snapshot
-// latency is publish time minus submit time, kept raw per request.
+// latency is publish time minus arrival time (the submission, or the
+// scheduled arrival of a replayed trace), kept raw per request.
+//
+// The plain mode (SGC_SERVE_PLAIN) is the baseline: the loop serves the same
+// requests as the plain per-launch engine would -- the earliest arrival in a
+// launch of its own, one epoch step per due graph -- under the same epoch and
+// deadline rules.
 //
 // Graphs are admitted against the pool's HBM budget by the loop: an admission
 // that does not fit waits until removals free memory.  Queries read the
@@ -105,15 +111,22 @@ struct sgc_server {
 
     void run();
     int turn();
+    int intake(bool& flush, bool& work);
+    void take_front(uint32_t g, uint32_t take, std::vector<sgc_edge_update>& batch);
+    int launch(const std::vector<sgc_edge_update>& batch, const std::vector<uint32_t>& taken);
+    int apply_batched(bool& work);
+    int apply_plain(bool& work);
+    int publish_due(bool flush, bool& work);
+    int submit_at(uint32_t gid, const sgc_edge_update* upd, uint32_t count, Clock::time_point arrival,
+                  uint64_t* ticket);
+    int query_at(uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out, Clock::time_point arrival);
 };
 
-// One loop turn: admissions and removals, one packed batch, one shared epoch
-// step.  Returns 1 if it did work, 0 if idle, negative on error.
-int sgc_server::turn() {
+// Admissions, removals and the routing of arrived requests to their tenants.
+int sgc_server::intake(bool& flush, bool& work) {
     std::vector<std::unique_ptr<Request>> fresh;
     std::vector<std::unique_ptr<Admission>> adm;
     std::vector<uint32_t> rem;
-    bool flush = false;
     {
         std::lock_guard<std::mutex> lk(mu);
         while (!inbox.empty()) {
@@ -129,8 +142,7 @@ int sgc_server::turn() {
         rem.swap(removals);
         flush = flush_req;
     }
-    const auto t0 = Clock::now();
-    bool work = !fresh.empty();
+    work = !fresh.empty();
     // route arrivals to their tenants
     for (auto& r : fresh) {
         Tenant& T = tenants[r->gid];
@@ -195,8 +207,36 @@ int sgc_server::turn() {
         for (auto it = keep.rbegin(); it != keep.rend(); ++it) admissions.push_front(std::move(*it));
     }
     if (!adm.empty()) cv_done.notify_all();
+    return SGC_OK;
+}
 
-    // one packed batch: round-robin over tenants, each up to its epoch boundary
+// Takes up to `take` more updates of tenant g's front request into `batch`.
+void sgc_server::take_front(uint32_t g, uint32_t take, std::vector<sgc_edge_update>& batch) {
+    Tenant& T = tenants[g];
+    Request& R = *T.queue.front();
+    batch.insert(batch.end(), R.recs.begin() + R.applied, R.recs.begin() + R.applied + take);
+    R.applied += take;
+    if (R.applied == R.recs.size()) {
+        T.applied.push_back(std::move(T.queue.front()));
+        T.queue.pop_front();
+    }
+}
+
+// One launch over `batch`, then the tenants' update counts move on.
+int sgc_server::launch(const std::vector<sgc_edge_update>& batch, const std::vector<uint32_t>& taken) {
+    int r = sgc_pool_apply(pool, batch.data(), batch.size(), 0);
+    if (r) return r;
+    for (uint32_t g = 0; g < tenants.size(); g++) tenants[g].since += taken[g];
+    std::lock_guard<std::mutex> lk(mu);
+    updates_applied += batch.size();
+    batches++;
+    queued_updates -= batch.size();
+    return SGC_OK;
+}
+
+// Continuous batching: one packed batch, round-robin over the tenants, each
+// filled at most up to its epoch boundary.
+int sgc_server::apply_batched(bool& work) {
     std::vector<sgc_edge_update> batch;
     std::vector<uint32_t> taken(tenants.size(), 0);
     const uint32_t cap = cfg.max_batch;
@@ -208,30 +248,55 @@ int sgc_server::turn() {
             if (!T.live || T.queue.empty()) continue;
             const uint64_t room = T.epoch_size - std::min<uint64_t>(T.since + taken[g], T.epoch_size);
             if (room == 0) continue;
-            Request& R = *T.queue.front();
+            const Request& R = *T.queue.front();
             const uint32_t left = (uint32_t)R.recs.size() - R.applied;
             const uint32_t take = (uint32_t)std::min<uint64_t>({(uint64_t)left, room, (uint64_t)(cap - batch.size())});
-            batch.insert(batch.end(), R.recs.begin() + R.applied, R.recs.begin() + R.applied + take);
-            R.applied += take;
+            take_front(g, take, batch);
             taken[g] += take;
             progress = true;
-            if (R.applied == R.recs.size()) {
-                T.applied.push_back(std::move(T.queue.front()));
-                T.queue.pop_front();
-            }
         }
     }
-    if (!batch.empty()) {
-        int r = sgc_pool_apply(pool, batch.data(), batch.size(), 0);
-        if (r) return r;
-        for (uint32_t g = 0; g < tenants.size(); g++) tenants[g].since += taken[g];
-        work = true;
-        std::lock_guard<std::mutex> lk(mu);
-        updates_applied += batch.size();
-        batches++;
-        queued_updates -= batch.size();
+    if (batch.empty()) return SGC_OK;
+    work = true;
+    return launch(batch, taken);
+}
+
+// The plain per-launch engine: the request that arrived first (of the tenants
+// with room left in their epoch) in one launch of its own, up to its tenant's
+// epoch boundary.
+int sgc_server::apply_plain(bool& work) {
+    int64_t best = -1;
+    for (uint32_t g = 0; g < tenants.size(); g++) {
+        const Tenant& T = tenants[g];
+        if (!T.live || T.queue.empty() || T.since >= T.epoch_size) continue;
+        const Request& R = *T.queue.front();
+        if (best < 0) {
+            best = g;
+            continue;
+        }
+        const Request& B = *tenants[best].queue.front();
+        if (R.arrival < B.arrival || (R.arrival == B.arrival && R.ticket < B.ticket)) best = g;
     }
-    // due graphs: full epochs, stale updates, or a flush
+    if (best < 0) return SGC_OK;
+    const uint32_t g = (uint32_t)best;
+    Tenant& T = tenants[g];
+    const Request& R = *T.queue.front();
+    const uint32_t take = (uint32_t)std::min<uint64_t>({(uint64_t)(R.recs.size() - R.applied),
+                                                         T.epoch_size - T.since, (uint64_t)cfg.max_batch});
+    std::vector<sgc_edge_update> batch;
+    std::vector<uint32_t> taken(tenants.size(), 0);
+    take_front(g, take, batch);
+    taken[g] = take;
+    work = true;
+    return launch(batch, taken);
+}
+
+// The epoch steps of the due graphs (epoch full, oldest unpublished update
+// older than max_delay_us, or a flush): one shared step (batched) or one step
+// per graph (plain).  sgc_pool_epoch returns after the pool's stream has
+// synchronised, so the snapshot kernel of the publish has completed when the
+// completion time is taken.
+int sgc_server::publish_due(bool flush, bool& work) {
     const auto now = Clock::now();
     std::vector<uint32_t> due;
     for (uint32_t g = 0; g < tenants.size(); g++) {
@@ -242,13 +307,17 @@ int sgc_server::turn() {
         const bool flushing = (flush || T.removing) && T.queue.empty();
         if (full || stale || flushing) due.push_back(g);
     }
-    if (!due.empty()) {
-        int r = sgc_pool_epoch(pool, due.data(), (uint32_t)due.size(), nullptr);
+    if (due.empty()) return SGC_OK;
+    work = true;
+    const bool plain = cfg.mode == SGC_SERVE_PLAIN;
+    for (size_t k = 0; k < due.size(); k += plain ? 1 : due.size()) {
+        const uint32_t cnt = plain ? 1u : (uint32_t)due.size();
+        int r = sgc_pool_epoch(pool, due.data() + k, cnt, nullptr);
         if (r) return r;
         const auto tp = Clock::now();
         std::lock_guard<std::mutex> lk(mu);
-        for (uint32_t g : due) {
-            Tenant& T = tenants[g];
+        for (uint32_t j = 0; j < cnt; j++) {
+            Tenant& T = tenants[due[k + j]];
             T.since = 0;
             for (auto& R : T.applied) {
                 latency_us.push_back(us_between(R->arrival, tp));
@@ -261,15 +330,27 @@ int sgc_server::turn() {
             if (T.has_unpublished) T.oldest = T.queue.front()->arrival;
         }
         epoch_steps++;
-        graph_epochs += due.size();
+        graph_epochs += cnt;
         last_publish = tp;
-        work = true;
     }
+    cv_done.notify_all();
+    return SGC_OK;
+}
+
+// One loop turn: admissions and removals, one launch (packed, or one request
+// in plain mode), the epoch steps of the due graphs.  Returns 1 if it did
+// work, 0 if idle, negative on error.
+int sgc_server::turn() {
+    bool flush = false, work = false;
+    const auto t0 = Clock::now();
+    int r = intake(flush, work);
+    if (r == SGC_OK) r = cfg.mode == SGC_SERVE_PLAIN ? apply_plain(work) : apply_batched(work);
+    if (r == SGC_OK) r = publish_due(flush, work);
+    if (r) return r;
     if (work) {
         std::lock_guard<std::mutex> lk(mu);
         busy_us += us_between(t0, Clock::now());
     }
-    if (!due.empty()) cv_done.notify_all();
     return work ? 1 : 0;
 }
 
@@ -309,6 +390,46 @@ void sgc_server::run() {
     }
 }
 
+int sgc_server::submit_at(uint32_t gid, const sgc_edge_update* upd, uint32_t count, Clock::time_point arrival,
+                          uint64_t* ticket) {
+    auto R = std::make_unique<Request>();
+    R->gid = gid;
+    R->arrival = arrival;
+    R->recs.assign(upd, upd + count);
+    for (auto& r : R->recs) memcpy(r.pad + 3, &gid, 4);
+    {
+        std::shared_lock<std::shared_mutex> rl(smu);
+        if (gid >= tenants.size() || !tenants[gid].live || tenants[gid].removing) return SGC_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (stop || loop_error) return SGC_ERR_STATE;
+    R->ticket = next_ticket++;
+    *ticket = R->ticket;
+    if (!any_submit) {
+        first_submit = R->arrival;
+        any_submit = true;
+    }
+    queued_updates += count;
+    inbox.push_back(std::move(R));
+    cv_work.notify_all();
+    return SGC_OK;
+}
+
+// A query's latency runs from its arrival to the colours being out.
+int sgc_server::query_at(uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out,
+                         Clock::time_point arrival) {
+    int r;
+    {
+        std::shared_lock<std::shared_mutex> rl(smu);
+        if (gid >= tenants.size() || !tenants[gid].live) return SGC_ERR_INVALID;
+        r = sgc_pool_query(pool, gid, ids, count, out);
+    }
+    const double us = us_between(arrival, Clock::now());
+    std::lock_guard<std::mutex> lk(mu);
+    query_us.push_back(us);
+    return r;
+}
+
 extern "C" {
 
 void sgc_default_server_config(sgc_server_config* c) {
@@ -320,6 +441,7 @@ void sgc_default_server_config(sgc_server_config* c) {
 
 int sgc_server_create(sgc_pool* pool, const sgc_server_config* cfg, sgc_server** out) {
     if (!pool || !cfg || !out || cfg->max_batch == 0) return SGC_ERR_INVALID;
+    if (cfg->mode != SGC_SERVE_BATCHED && cfg->mode != SGC_SERVE_PLAIN) return SGC_ERR_INVALID;
     sgc_server* s = new sgc_server;
     s->pool = pool;
     s->cfg = *cfg;
@@ -403,27 +525,7 @@ int sgc_server_remove(sgc_server* s, uint32_t gid) {
 
 int sgc_server_submit(sgc_server* s, uint32_t gid, const sgc_edge_update* upd, uint32_t count, uint64_t* ticket) {
     if (!s || (!upd && count) || !ticket || count == 0) return SGC_ERR_INVALID;
-    auto R = std::make_unique<Request>();
-    R->gid = gid;
-    R->arrival = Clock::now();
-    R->recs.assign(upd, upd + count);
-    for (auto& r : R->recs) memcpy(r.pad + 3, &gid, 4);
-    {
-        std::shared_lock<std::shared_mutex> rl(s->smu);
-        if (gid >= s->tenants.size() || !s->tenants[gid].live || s->tenants[gid].removing) return SGC_ERR_INVALID;
-    }
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop || s->loop_error) return SGC_ERR_STATE;
-    R->ticket = s->next_ticket++;
-    *ticket = R->ticket;
-    if (!s->any_submit) {
-        s->first_submit = R->arrival;
-        s->any_submit = true;
-    }
-    s->queued_updates += count;
-    s->inbox.push_back(std::move(R));
-    s->cv_work.notify_all();
-    return SGC_OK;
+    return s->submit_at(gid, upd, count, Clock::now(), ticket);
 }
 
 int sgc_server_wait(sgc_server* s, uint64_t ticket, uint32_t timeout_ms) {
@@ -450,17 +552,7 @@ int sgc_server_flush(sgc_server* s, uint32_t timeout_ms) {
 
 int sgc_server_query(sgc_server* s, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out) {
     if (!s) return SGC_ERR_INVALID;
-    const auto t0 = Clock::now();
-    int r;
-    {
-        std::shared_lock<std::shared_mutex> rl(s->smu);
-        if (gid >= s->tenants.size() || !s->tenants[gid].live) return SGC_ERR_INVALID;
-        r = sgc_pool_query(s->pool, gid, ids, count, out);
-    }
-    const double us = us_between(t0, Clock::now());
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->query_us.push_back(us);
-    return r;
+    return s->query_at(gid, ids, count, out, Clock::now());
 }
 
 int sgc_server_get_stats(sgc_server* s, sgc_server_stats* out) {
@@ -496,6 +588,80 @@ int sgc_server_reset_stats(sgc_server* s) {
     s->updates_applied = s->batches = s->epoch_steps = s->graph_epochs = s->requests_done = 0;
     s->any_submit = false;
     s->busy_us = 0;
+    return SGC_OK;
+}
+
+// Open-loop replay of an arrival trace: update items from this thread,
+// queries from a second one, each submitted at its scheduled time (the last
+// stretch of each wait spins, so submissions run at most a few microseconds
+// late unless the host falls behind), with its scheduled time as arrival.
+int sgc_server_replay(sgc_server* s, const sgc_trace_item* items, uint64_t count, const sgc_edge_update* updates,
+                      const uint32_t* query_ids, uint32_t timeout_ms, sgc_replay_stats* out) {
+    if (!s || (!items && count)) return SGC_ERR_INVALID;
+    uint32_t max_q = 0;
+    for (uint64_t i = 0; i < count; i++) {
+        const sgc_trace_item& it = items[i];
+        if (it.kind > 1 || it.count == 0) return SGC_ERR_INVALID;
+        if (it.kind == 0 && !updates) return SGC_ERR_INVALID;
+        if (it.kind == 1 && !query_ids) return SGC_ERR_INVALID;
+        if (it.kind == 1) max_q = std::max(max_q, it.count);
+    }
+    sgc_replay_stats st{};
+    const auto start = Clock::now() + std::chrono::microseconds(200);
+    auto wait_until = [](Clock::time_point t) {
+        const auto coarse = t - std::chrono::microseconds(100);
+        if (Clock::now() < coarse) std::this_thread::sleep_until(coarse);
+        while (Clock::now() < t) {
+        }
+    };
+    std::atomic<int> qerr{0};
+    std::atomic<uint64_t> nq{0};
+    std::thread qthread([&] {
+        std::vector<uint16_t> colours(std::max<uint32_t>(max_q, 1));
+        for (uint64_t i = 0; i < count && !qerr.load(); i++) {
+            const sgc_trace_item& it = items[i];
+            if (it.kind != 1) continue;
+            const auto at = start + std::chrono::nanoseconds(it.at_ns);
+            wait_until(at);
+            const int r = s->query_at(it.gid, query_ids + it.offset, it.count, colours.data(), at);
+            if (r) qerr = r;
+            nq++;
+        }
+    });
+    int err = SGC_OK;
+    double late_sum = 0;
+    for (uint64_t i = 0; i < count && err == SGC_OK; i++) {
+        const sgc_trace_item& it = items[i];
+        if (it.kind != 0) continue;
+        const auto at = start + std::chrono::nanoseconds(it.at_ns);
+        wait_until(at);
+        const double late = us_between(at, Clock::now());
+        st.max_late_us = std::max(st.max_late_us, late);
+        late_sum += late;
+        uint64_t ticket = 0;
+        err = s->submit_at(it.gid, updates + it.offset, it.count, at, &ticket);
+        st.requests++;
+        st.updates += it.count;
+    }
+    qthread.join();
+    if (err == SGC_OK) err = qerr.load();
+    // the last requests are published by the deadline rule as in the rest of
+    // the trace (a flush right away would publish them early); then a flush
+    // publishes whatever is left
+    if (err == SGC_OK && count) {
+        uint64_t last = 0;
+        for (uint64_t i = 0; i < count; i++) last = std::max(last, items[i].at_ns);
+        wait_until(start + std::chrono::nanoseconds(last) + std::chrono::microseconds(s->cfg.max_delay_us));
+    }
+    if (err == SGC_OK) err = sgc_server_flush(s, timeout_ms);
+    if (err) return err;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        st.span_us = us_between(start, s->last_publish);
+    }
+    st.mean_late_us = st.requests ? late_sum / (double)st.requests : 0;
+    st.queries = nq.load();
+    if (out) *out = st;
     return SGC_OK;
 }
 

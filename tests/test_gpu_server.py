@@ -39,14 +39,15 @@ def _requests(streams, rng):
     return out
 
 
-@pytest.mark.parametrize("max_batch", [64, 4096])
-def test_served_mix_equals_each_graph_alone(max_batch, graph_path):
+@pytest.mark.parametrize("max_batch,mode", [(64, S.GraphServer.BATCHED), (4096, S.GraphServer.BATCHED),
+                                            (4096, S.GraphServer.PLAIN)])
+def test_served_mix_equals_each_graph_alone(max_batch, mode, graph_path):
     cfg = S.StreamGCConfig(epoch_size=160, batch_size=160, deterministic=True, repair_every_epoch=True,
                            exact_histogram=True)
     graphs = {name: S.load_graph_csr(graph_path(name)) for name in TENANTS}
     pool = S.GraphPool(max_graphs=8, batch_capacity=8192, deterministic=True, repair_every_epoch=True,
                        exact_histogram=True)
-    srv = S.GraphServer(pool, max_batch=max_batch)
+    srv = S.GraphServer(pool, max_batch=max_batch, mode=mode)
     gid = {name: srv.add(g, cfg) for name, g in graphs.items()}
     streams = {name: _streams(g, k + 1) for k, (name, g) in enumerate(graphs.items())}
     tickets = [srv.submit(gid[t], recs) for t, recs in _requests(streams, np.random.default_rng(5))]
@@ -56,6 +57,8 @@ def test_served_mix_equals_each_graph_alone(max_batch, graph_path):
     st = srv.stats()
     assert st["requests_done"] == len(tickets) and st["queued_updates"] == 0
     assert st["updates_applied"] == sum(len(s) for s in streams.values())
+    if mode == S.GraphServer.PLAIN:              # one launch per request (more where one crosses an epoch)
+        assert st["batches"] >= len(tickets) and st["epoch_steps"] == st["graph_epochs"]
     for name, g in graphs.items():
         alone = S.GPUPartition(g, cfg)
         alone.process_stream(streams[name])
@@ -146,4 +149,53 @@ def test_served_max_delay_publishes_partial_epochs(graph_path):
     lat = srv.latencies("update")
     assert len(lat) == 12 and np.max(lat) < 2e6
     assert srv.stats()["graph_epochs"] >= 1
+    srv.close()
+
+
+@pytest.mark.parametrize("mode", [S.GraphServer.BATCHED, S.GraphServer.PLAIN])
+def test_replayed_trace_equals_each_graph_alone(mode, graph_path):
+    """An open-loop trace replayed by the library (updates at their scheduled
+    times, queries from a second thread) gives every tenant the same result as
+    alone, and one latency per request and per query."""
+    cfg = S.StreamGCConfig(epoch_size=160, batch_size=160, deterministic=True, repair_every_epoch=True,
+                           exact_histogram=True)
+    graphs = {name: S.load_graph_csr(graph_path(name)) for name in TENANTS}
+    pool = S.GraphPool(max_graphs=8, batch_capacity=8192, deterministic=True, repair_every_epoch=True,
+                       exact_histogram=True)
+    srv = S.GraphServer(pool, max_batch=4096, mode=mode)
+    gid = {name: srv.add(g, cfg) for name, g in graphs.items()}
+    streams = {name: _streams(g, k + 11) for k, (name, g) in enumerate(graphs.items())}
+    rng = np.random.default_rng(9)
+    reqs = _requests(streams, rng)
+    items = np.zeros(len(reqs) + len(reqs) // 5 + 1, dtype=S.TRACE_ITEM)
+    parts, qids = [], []
+    off = qoff = k = 0
+    at = 0
+    for j, (t, recs) in enumerate(reqs):
+        at += int(rng.exponential(150e3))          # ns
+        items[k] = (at, off, gid[t], len(recs), 0, 0)
+        parts.append(recs)
+        off += len(recs)
+        k += 1
+        if j % 5 == 4:
+            ids = rng.integers(0, graphs[t].n, int(rng.integers(1, 200))).astype(np.uint32)
+            items[k] = (at, qoff, gid[t], len(ids), 1, 0)
+            qids.append(ids)
+            qoff += len(ids)
+            k += 1
+    items = items[:k]
+    rs = srv.replay(items, np.concatenate(parts), np.concatenate(qids), timeout_ms=120000)
+    assert rs["requests"] == len(reqs) and rs["queries"] == len(qids)
+    assert rs["updates"] == sum(len(s) for s in streams.values())
+    assert rs["span_us"] * 1e3 >= at
+    lat = srv.latencies("update")
+    assert len(lat) == len(reqs) and np.all(lat > 0)
+    assert len(srv.latencies("query")) == len(qids)
+    for name, g in graphs.items():
+        alone = S.GPUPartition(g, cfg)
+        alone.process_stream(streams[name])
+        if len(streams[name]) % cfg.epoch_size:
+            alone.epoch()
+        assert np.array_equal(pool.words(gid[name]), alone.words()), name
+        assert np.array_equal(pool.snapshot(gid[name]), alone.snapshot()), name
     srv.close()
