@@ -429,31 +429,6 @@ def profile_summary(args):
 
 
 # ---------------------------------------------------------------- batched (C4)
-def make_batch_graphs(S, count, seed=42):
-    """(kind, graph, churn length, insert %) of the batched workload, graph i seeded 42 + i."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        if i < count // 2:
-            g = S.gen_delaunay(int(round(2 ** rng.uniform(10, 16))), seed=seed + i)
-            kind = "delaunay"
-        else:
-            g = S.gen_rmat(int(rng.integers(10, 17)), float(rng.uniform(8, 40)), seed=seed + i)
-            kind = "rmat"
-        out.append((kind, g, int(rng.integers(1000, 64001)), int(rng.integers(30, 71))))
-    return out
-
-
-def chunked_launches(streams, chunk):
-    """Launch j holds records [j*chunk, (j+1)*chunk) of every stream that has them."""
-    allr = np.concatenate(streams)
-    launch = np.concatenate([np.arange(len(s), dtype=np.int64) // chunk for s in streams])
-    order = np.argsort(launch, kind="stable")
-    bounds = np.concatenate([[0], np.cumsum(np.bincount(launch))])
-    allr = allr[order]
-    return [allr[bounds[j]:bounds[j + 1]] for j in range(len(bounds) - 1)]
-
-
 def run_batched(S, args, w):
     """The batched pattern on 256 graphs, timed three ways.
 
@@ -465,11 +440,12 @@ def run_batched(S, args, w):
     rounds replayed from captured hipGraphs with the launch-shape autotuner
     on its first rounds.  Every snapshot of the captured leg is validated.
     """
+    from streamgc_amd import workloads as W
     t = time.perf_counter()
-    spec = make_batch_graphs(S, args.graphs)
+    spec = W.batch_graphs(S, args.graphs)
     nv = sum(g.n for _, g, _, _ in spec)
     ne = sum(g.num_edges // 2 for _, g, _, _ in spec)
-    E = [min(L, 4096) for _, _, L, _ in spec]
+    E = W.batch_epoch_sizes(spec)
     churn = [S.generate_bench_stream_edges(g, L, pct, 42 + i) for i, (_, g, L, pct) in enumerate(spec)]
     n_upd = sum(len(x) for x in churn)
     log(f"{len(spec)} graphs, {nv} vertices, {ne} edges; churn {n_upd} updates ({time.perf_counter() - t:.1f}s)")
@@ -480,7 +456,7 @@ def run_batched(S, args, w):
                            exact_histogram=True)
         gids = [pool.add(g, S.StreamGCConfig(epoch_size=e, batch_size=e)) for (_, g, _, _), e in zip(spec, E)]
         init = [S.tag_graph(S.generate_init_stream(g), gid) for (_, g, _, _), gid in zip(spec, gids)]
-        for launch in chunked_launches(init, args.init_batch):
+        for launch in W.chunked_launches(init, args.init_batch):
             pool.apply(launch)
         pool.finish()
         log(f"pool: {sum(len(x) for x in init)} init inserts in launches of {args.init_batch} per graph, "
@@ -603,25 +579,6 @@ def run_trc(S, args, w):
         "roofline": None, "cpu_baseline": None}))
 
 
-def make_tenants(S, count, seed=42):
-    """Tenant graphs of the C1-C4 shapes at tenant scale (seed 42 + index)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        kind = ("delaunay", "rmat", "chung_lu", "rmat_dense")[i % 4]
-        if kind == "delaunay":
-            g = S.gen_delaunay(int(2 ** rng.uniform(10, 14)), seed=seed + i)
-        elif kind == "rmat":
-            g = S.gen_rmat(int(rng.integers(10, 15)), float(rng.uniform(8, 40)), seed=seed + i)
-        elif kind == "chung_lu":
-            n = int(2 ** rng.uniform(12, 15))
-            g = S.gen_chung_lu(n, n * 8, 2.3, 4.0 * n ** 0.5, seed=seed + i, permute=True)
-        else:
-            g = S.gen_rmat(12, 48.0, seed=seed + i)
-        out.append((kind, g))
-    return out
-
-
 def pct(a, q):
     return round(float(np.percentile(a, q)), 3) if len(a) else None
 
@@ -712,11 +669,11 @@ def run_served(S, args, w):
     the per-tenant arrival rate) at that point; served_vs_plain divides it by
     the plain engine's achieved rate at the same offered load.
     """
+    from streamgc_amd import workloads as W
     E = 4096
     t = time.perf_counter()
-    tenants = make_tenants(S, 64)
-    orkut = S.gen_chung_lu(3072441, 117185083, 2.3, 33313.0, seed=42, permute=True)
-    tenants.append(("orkut", orkut))
+    tenants = W.tenants(S, 64)
+    orkut = tenants[-1][1]
     T = len(tenants)
     log(f"tenants: 64 + Orkut-shape ({orkut.n} vertices, {orkut.num_edges // 2} edges) "
         f"({time.perf_counter() - t:.1f}s)")

@@ -297,7 +297,9 @@ typedef struct sgc_graph_info {
     int32_t published;           /* published snapshot buffer (-1: none yet) */
     double tier2_ms;             /* host Tier-2 time, all runs */
     uint32_t tier2_runs;
-    uint32_t reserved;
+    uint32_t lost;               /* half-edges lost for good (the reference drops them too): overflow entries past
+                                    the buffer's capacity, drains that found no room, full rows under the
+                                    no-overflow ablation.  An edge lost from both rows is invisible to repair. */
 } sgc_graph_info;
 int sgc_pool_graph_info(sgc_pool* p, uint32_t gid, sgc_graph_info* out);
 /* Per-graph state downloads (as sgc_download_csr / _bitmaps / _hist). */
@@ -426,6 +428,10 @@ int sgc_server_get_stats(sgc_server* s, sgc_server_stats* out);
 /* Raw latencies in microseconds: which 0 = update-to-snapshot per request, 1 = query. */
 int sgc_server_latencies(sgc_server* s, int which, double* out, uint64_t cap, uint64_t* count);
 int sgc_server_reset_stats(sgc_server* s);
+/* Per published snapshot of graph gid, in publish order: how many of the
+ * graph's updates submitted through the server it contains (the updates are
+ * applied in submission order, so each watermark is a prefix of them). */
+int sgc_server_watermarks(sgc_server* s, uint32_t gid, uint64_t* out, uint64_t cap, uint64_t* count);
 /* Open-loop trace replay (no reference counterpart; the reference's feeder,
  * A/src/host/feeder_thread.cpp:28-138, is never started).  Each item arrives
  * at_ns after the call starts: update items are submitted from the calling

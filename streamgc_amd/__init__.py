@@ -869,6 +869,14 @@ class GraphServer:
     def reset_stats(self):
         self._ck(lib().sgc_server_reset_stats(self.h))
 
+    def watermarks(self, gid):
+        """Per published snapshot of gid (publish order): updates submitted through the server it contains."""
+        n = C.c_uint64()
+        self._ck(lib().sgc_server_watermarks(self.h, gid, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.uint64)
+        self._ck(lib().sgc_server_watermarks(self.h, gid, out.ctypes.data, n.value, C.byref(n)))
+        return out[:n.value]
+
     def replay(self, items, updates, query_ids=None, timeout_ms=600000):
         """Open-loop replay of a trace (TRACE_ITEM records, at_ns after the start):
         updates and queries submitted at their scheduled times, latencies from

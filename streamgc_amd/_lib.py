@@ -132,10 +132,10 @@ class GraphInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("num_hd", C.c_uint32), ("slots", C.c_uint64), ("bytes", C.c_uint64),
                 ("epoch_size", C.c_uint32), ("epochs", C.c_uint32), ("overflow_events", C.c_uint32),
                 ("published", C.c_int32), ("tier2_ms", C.c_double), ("tier2_runs", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("lost", C.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PoolConfig(C.Structure):
@@ -247,6 +247,7 @@ SIGNATURES = {
     "sgc_server_get_stats": [_vp, C.POINTER(ServerStats)],
     "sgc_server_latencies": [_vp, C.c_int, _vp, C.c_uint64, C.POINTER(C.c_uint64)],
     "sgc_server_reset_stats": [_vp],
+    "sgc_server_watermarks": [_vp, C.c_uint32, _vp, C.c_uint64, C.POINTER(C.c_uint64)],
     "sgc_server_replay": [_vp, _vp, C.c_uint64, _vp, _vp, C.c_uint32, C.POINTER(ReplayStats)],
     "sgc_write_coloring": [C.c_char_p, _u16p, C.c_uint32],
     "sgc_write_meta": [C.c_char_p, C.POINTER(RunMeta)],

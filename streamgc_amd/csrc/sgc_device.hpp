@@ -47,6 +47,8 @@ enum : uint32_t {
     C_T2FIXED,          //   vertices the last pass recoloured
     C_T2TOTAL,          //   vertices recoloured by this repair
     C_REVOVF,           //   reverse-only neighbours that did not fit the scratch list
+    C_LOST,             // half-edges lost for good: overflow entries past capacity, failed drains, and full
+                        // rows under the no-overflow ablation; never reset
     C_COUNT = 16
 };
 
@@ -381,6 +383,8 @@ __device__ __forceinline__ uint32_t row_insert(const GraphDev& g, uint32_t x, ui
                     st(&g.ovf[2 * (size_t)pos], x);
                     st(&g.ovf[2 * (size_t)pos + 1], nb);
                 }
+            } else {
+                add32(&g.ctr[C_LOST], 1u);
             }
             // the two rows of this edge may now disagree: always maintain both
             mark_sticky(g, x);
@@ -560,6 +564,8 @@ __device__ __forceinline__ uint32_t finish_insert(const GraphDev& g, uint32_t x,
                 st(&g.ovf[2 * (size_t)at], x);
                 st(&g.ovf[2 * (size_t)at + 1], nb);
             }
+        } else {
+            add32(&g.ctr[C_LOST], 1u);
         }
         mark_sticky(g, x);
         mark_sticky(g, nb);

@@ -2453,8 +2453,8 @@ int sgc_pool_download_hist(sgc_pool* p, uint32_t gid, uint32_t* hist) {
 int sgc_pool_graph_info(sgc_pool* p, uint32_t gid, sgc_graph_info* out) {
     GraphState* G = graph_of(p, gid);
     if (!G || !out) return SGC_ERR_INVALID;
-    uint32_t asym = 0;
-    int r = download(p, &asym, p->d_ctr + (size_t)gid * C_COUNT + C_ASYM, 4);
+    uint32_t c[C_COUNT];
+    int r = download(p, c, p->d_ctr + (size_t)gid * C_COUNT, sizeof c);
     if (r) return r;
     memset(out, 0, sizeof *out);
     out->n = G->n;
@@ -2463,7 +2463,8 @@ int sgc_pool_graph_info(sgc_pool* p, uint32_t gid, sgc_graph_info* out) {
     out->bytes = G->bytes;
     out->epoch_size = G->cfg.epoch_size;
     out->epochs = G->epochs;
-    out->overflow_events = asym;
+    out->overflow_events = c[C_ASYM];
+    out->lost = c[C_LOST];
     out->published = G->published;
     out->tier2_ms = G->t2_ms;
     out->tier2_runs = G->t2_count;

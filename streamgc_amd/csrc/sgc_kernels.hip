@@ -717,6 +717,7 @@ __global__ void k_drain_prep(const GraphDev* __restrict__ graphs, uint32_t ng, c
     uint32_t raw = c[C_OVF];
     c[C_OVF_RAW] = raw;
     c[C_OVF_SNAP] = raw < graphs[gid].ovf_cap ? raw : graphs[gid].ovf_cap;
+    if (raw > graphs[gid].ovf_cap) c[C_LOST] += raw - graphs[gid].ovf_cap;   // never buffered
     c[C_OVF] = 0;
     c[C_DRAIN_FAILED] = 0;
 }
@@ -736,7 +737,10 @@ __global__ __launch_bounds__(256) void k_drain_elem(const GraphDev* __restrict__
         if (lane() == 0 && x < g.n) mark(g, x, D_BITS | D_STATUS);
         failed += f;
     }
-    if (failed && lane() == 0) add32(&g.ctr[C_DRAIN_FAILED], failed);
+    if (failed && lane() == 0) {
+        add32(&g.ctr[C_DRAIN_FAILED], failed);
+        add32(&g.ctr[C_LOST], failed);
+    }
 }
 
 // The same in buffer order, one wavefront per graph (deterministic mode).
@@ -754,7 +758,10 @@ __global__ __launch_bounds__(64) void k_drain_serial(const GraphDev* __restrict_
             if (lane() == 0 && x < g.n) mark(g, x, D_BITS | D_STATUS);
             failed += f;
         }
-        if (lane() == 0) gp(g.ctr)[C_DRAIN_FAILED] = failed;
+        if (lane() == 0) {
+            gp(g.ctr)[C_DRAIN_FAILED] = failed;
+            gp(g.ctr)[C_LOST] += failed;
+        }
     }
 }
 

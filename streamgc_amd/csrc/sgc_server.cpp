@@ -72,6 +72,8 @@ struct Tenant {
     std::vector<std::unique_ptr<Request>> applied;  // fully applied, waiting for a publish
     Clock::time_point oldest;                // arrival of the oldest unpublished update
     bool has_unpublished = false;
+    uint64_t applied_total = 0;              // updates applied through the server, in the tenant's order
+    std::vector<uint64_t> watermarks;        // applied_total at each publish
 };
 
 }  // namespace
@@ -226,7 +228,10 @@ void sgc_server::take_front(uint32_t g, uint32_t take, std::vector<sgc_edge_upda
 int sgc_server::launch(const std::vector<sgc_edge_update>& batch, const std::vector<uint32_t>& taken) {
     int r = sgc_pool_apply(pool, batch.data(), batch.size(), 0);
     if (r) return r;
-    for (uint32_t g = 0; g < tenants.size(); g++) tenants[g].since += taken[g];
+    for (uint32_t g = 0; g < tenants.size(); g++) {
+        tenants[g].since += taken[g];
+        tenants[g].applied_total += taken[g];
+    }
     std::lock_guard<std::mutex> lk(mu);
     updates_applied += batch.size();
     batches++;
@@ -319,6 +324,7 @@ int sgc_server::publish_due(bool flush, bool& work) {
         for (uint32_t j = 0; j < cnt; j++) {
             Tenant& T = tenants[due[k + j]];
             T.since = 0;
+            T.watermarks.push_back(T.applied_total);
             for (auto& R : T.applied) {
                 latency_us.push_back(us_between(R->arrival, tp));
                 done[R->ticket] = true;
@@ -577,6 +583,16 @@ int sgc_server_latencies(sgc_server* s, int which, double* out, uint64_t cap, ui
     const std::vector<double>& v = which == 0 ? s->latency_us : s->query_us;
     *count = v.size();
     if (out) std::copy(v.begin(), v.begin() + std::min<uint64_t>(cap, v.size()), out);
+    return SGC_OK;
+}
+
+int sgc_server_watermarks(sgc_server* s, uint32_t gid, uint64_t* out, uint64_t cap, uint64_t* count) {
+    if (!s || !count) return SGC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (gid >= s->tenants.size() || !s->tenants[gid].live) return SGC_ERR_INVALID;
+    const std::vector<uint64_t>& w = s->tenants[gid].watermarks;
+    *count = w.size();
+    if (out) std::copy(w.begin(), w.begin() + std::min<uint64_t>(cap, w.size()), out);
     return SGC_OK;
 }
 
