@@ -71,11 +71,17 @@ def test_c2_parallel_colouring_is_proper_and_close_to_the_replay():
     assert len(epochs) == 8 and all(e["t2_remaining"] == 0 for e in epochs)
     bad = gpu.validate_history(S.edge_intervals(g, bench), [E * (k + 1) for k in range(8)])
     assert bad.tolist() == [0] * 8
-    # sequential replay with exact histograms: ref["final_max_color"] (83); the
-    # reference's own semantics (drifting histograms) needs REF["c2/reference"] (180)
-    final = epochs[-1]["max_color"]
-    assert final <= ref["final_max_color"] * 1.06, (final, ref["final_max_color"])
-    assert final < REF["c2/reference"]["final_max_color"]
+    # Epoch k's colour count against the sequential replay's after the same
+    # epoch, within 2%.  The replay selects from exact histograms, as this
+    # engine does; the reference's own semantics (drifting histograms) uses
+    # far more colours on the same stream (REF["c2/reference"]).
+    got = [e["max_color"] for e in epochs]
+    exact = ref["epoch_max_color"][:8]
+    drift = REF["c2/reference"]["epoch_max_color"][:8]
+    print(f"C2 colours per epoch: GPU {got}, replay (exact histograms) {exact}, replay (reference semantics) {drift}")
+    for k, (c, r) in enumerate(zip(got, exact)):
+        assert c <= r * 1.02, (k, got, exact)
+        assert c < drift[k], (k, got, drift)
 
 
 def _parallel_run(g, stream, ref, tol):
@@ -96,9 +102,12 @@ def _parallel_run(g, stream, ref, tol):
     assert len(epochs) == epochs_n and all(e["t2_remaining"] == 0 for e in epochs)
     bad = gpu.validate_history(S.edge_intervals(g, stream), [E * (k + 1) for k in range(epochs_n)])
     assert bad.tolist() == [0] * epochs_n
-    final = epochs[-1]["max_color"]
-    assert final <= ref * tol, (final, ref)
-    return final
+    got = [e["max_color"] for e in epochs]
+    exact = ref["epoch_max_color"][:epochs_n]
+    print(f"colours per epoch: GPU {got}, replay (exact histograms) {exact}")
+    for k, (c, r) in enumerate(zip(got, exact)):
+        assert c <= r * tol, (k, got, exact)
+    return got[-1]
 
 
 def test_c3_dense_growth_is_proper_and_within_two_percent_of_the_replay():
@@ -106,6 +115,6 @@ def test_c3_dense_growth_is_proper_and_within_two_percent_of_the_replay():
     ref = REF["c3/exact"]
     assert (g.n, g.num_edges // 2) == (ref["vertices"], ref["edges"])
     stream = S.dense_growth_stream(g, ref["stream"], core_pct=90, ins_per_del=2, seed=42)
-    # the replay's last publish follows the 31st epoch: compare the 30 full epochs' last colour count
-    final = _parallel_run(g, stream, ref["final_max_color"], 1.02)
-    assert final < REF["c3/reference"]["final_max_color"]
+    # every epoch's colour count against the replay's after the same epoch, within 2%
+    final = _parallel_run(g, stream, ref, 1.02)
+    assert final < REF["c3/reference"]["epoch_max_color"][len(stream) // E - 1]
