@@ -283,6 +283,17 @@ int sgc_pool_finish(sgc_pool* p, const uint32_t* gids, uint32_t count, sgc_epoch
 int sgc_pool_submit(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, int on_device,
                     sgc_epoch_stats* stats_out, uint32_t max_stats, uint32_t* nstats);
 int sgc_pool_query(sgc_pool* p, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors);
+/* sgc_pool_query without any device allocation, for callers that cannot
+ * allocate (tight serving loops, code that forbids hipMalloc).  No counterpart
+ * in the reference.  Same arguments, results and errors as sgc_pool_query;
+ * the ids and colours are staged in a static device array of
+ * SGC_QUERY_NOALLOC_MAX entries (384 KiB, held by each GPU the library runs
+ * on until the process ends) instead of the pool's growable buffers.  More
+ * than SGC_QUERY_NOALLOC_MAX ids: SGC_ERR_RANGE.  Every call on one GPU
+ * shares that array; the library serialises the calls of one process with a
+ * lock, and the array keeps the last call's contents. */
+#define SGC_QUERY_NOALLOC_MAX 65536u
+int sgc_pool_query_noalloc(sgc_pool* p, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors);
 int sgc_pool_download_words(sgc_pool* p, uint32_t gid, uint32_t* words);
 int sgc_pool_upload_words(sgc_pool* p, uint32_t gid, const uint32_t* words);  /* every sweep then visits every vertex */
 /* One resident graph (no reference counterpart). */

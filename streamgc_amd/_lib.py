@@ -207,6 +207,7 @@ SIGNATURES = {
     "sgc_pool_finish": [_vp, _u32p, C.c_uint32, C.POINTER(EpochStats)],
     "sgc_pool_submit": [_vp, _vp, C.c_uint64, C.c_int, C.POINTER(EpochStats), C.c_uint32, C.POINTER(C.c_uint32)],
     "sgc_pool_query": [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p],
+    "sgc_pool_query_noalloc": [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p],
     "sgc_pool_download_words": [_vp, C.c_uint32, _u32p],
     "sgc_pool_upload_words": [_vp, C.c_uint32, _u32p],
     "sgc_pool_graph_info": [_vp, C.c_uint32, C.POINTER(GraphInfo)],

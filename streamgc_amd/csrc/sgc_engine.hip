@@ -239,6 +239,7 @@ struct sgc_pool {
     uint32_t* d_qids = nullptr;
     uint16_t* d_qout = nullptr;
     uint32_t qcap = 0;
+    int device = 0;                       // the HIP device current when the pool was created
     std::mutex qmu;
     std::mutex run_mu;                    // held by a batched run; queries wait for it (taken before qmu)
     uint64_t t1_sweeps = 0, t2_runs = 0;
@@ -1649,6 +1650,7 @@ __global__ void k_gather16(const uint16_t* __restrict__ src, uint32_t n, const u
 int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t flags, uint64_t budget) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return P->fail(SGC_ERR_HIP, "no HIP device");
+    if (hipGetDevice(&P->device) != hipSuccess) return P->fail(SGC_ERR_HIP, "no current HIP device");
     P->max_graphs = max_graphs;
     P->batch_cap = batch_cap;
     P->flags = flags;
@@ -1810,6 +1812,26 @@ GraphState* graph_of(sgc_pool* P, uint32_t gid) {
     return &P->gs[gid];
 }
 
+// The published colours of `count` ids through device buffers d_ids (count
+// u32) and d_out (count u16), on the pool's query stream after the publish.
+// Caller holds run_mu and qmu.
+int query_run(sgc_pool* P, GraphState* G, const uint32_t* ids, uint32_t count, uint16_t* out, uint32_t* d_ids,
+              uint16_t* d_out) {
+    const int half = G->published;
+    hipError_t r = hipStreamWaitEvent(P->qstream, G->pub_ev[half], 0);
+    if (r == hipSuccess) r = hipMemcpyAsync(d_ids, ids, (size_t)count * 4, hipMemcpyHostToDevice, P->qstream);
+    if (r == hipSuccess) {
+        const uint32_t grid = std::min<uint32_t>((count + 255) / 256, 1024);
+        k_gather16<<<grid, 256, 0, P->qstream>>>(G->snap + (size_t)half * G->n, G->n, d_ids, count, d_out);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipEventRecord(G->read_ev[half], P->qstream);
+    if (r == hipSuccess) r = hipMemcpyAsync(out, d_out, (size_t)count * 2, hipMemcpyDeviceToHost, P->qstream);
+    if (r == hipSuccess) r = hipStreamSynchronize(P->qstream);
+    if (r != hipSuccess) return P->hip_fail(r, "query");
+    return SGC_OK;
+}
+
 int query_impl(sgc_pool* P, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out) {
     GraphState* G = graph_of(P, gid);
     if (!G || (!ids && count) || (!out && count)) return SGC_ERR_INVALID;
@@ -1831,19 +1853,67 @@ int query_impl(sgc_pool* P, uint32_t gid, const uint32_t* ids, uint32_t count, u
         if (r != hipSuccess) return P->hip_fail(r, "query buffers");
         P->qcap = count;
     }
-    const int half = G->published;
-    hipError_t r = hipStreamWaitEvent(P->qstream, G->pub_ev[half], 0);
-    if (r == hipSuccess) r = hipMemcpyAsync(P->d_qids, ids, (size_t)count * 4, hipMemcpyHostToDevice, P->qstream);
-    if (r == hipSuccess) {
-        const uint32_t grid = std::min<uint32_t>((count + 255) / 256, 1024);
-        k_gather16<<<grid, 256, 0, P->qstream>>>(G->snap + (size_t)half * G->n, G->n, P->d_qids, count, P->d_qout);
-        r = hipGetLastError();
+    return query_run(P, G, ids, count, out, P->d_qids, P->d_qout);
+}
+
+// ---------------------------------------------------------------- allocation-free query
+// Staging of sgc_pool_query_noalloc: SGC_QUERY_NOALLOC_MAX ids (u32) followed
+// by as many colours (u16), in a file-scope device array that every GPU the
+// library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel or a copy: it passes the device address that
+// hipGetSymbolAddress returns for the device, looked up once per device.
+}  // namespace
+
+static constexpr uint32_t QSCRATCH_WORDS = SGC_QUERY_NOALLOC_MAX + SGC_QUERY_NOALLOC_MAX / 2;
+static __device__ __align__(256) uint32_t scratch[QSCRATCH_WORDS];
+
+namespace {
+
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;                  // guards g_scratch_ptr and serialises the calls that use it
+void* g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+// The scratch array's address on `device` (looked up on first use, with that
+// device current; the caller's current device is restored).  Caller holds
+// g_scratch_mu.
+hipError_t scratch_on(int device, uint32_t** out) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        hipError_t r = hipGetDevice(&cur);
+        if (r == hipSuccess && cur != device) r = hipSetDevice(device);
+        void* ptr = nullptr;
+        if (r == hipSuccess) r = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) {
+            hipError_t r2 = hipSetDevice(cur);
+            if (r == hipSuccess) r = r2;
+        }
+        if (r != hipSuccess) return r;
+        if (!ptr) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = ptr;
     }
-    if (r == hipSuccess) r = hipEventRecord(G->read_ev[half], P->qstream);
-    if (r == hipSuccess) r = hipMemcpyAsync(out, P->d_qout, (size_t)count * 2, hipMemcpyDeviceToHost, P->qstream);
-    if (r == hipSuccess) r = hipStreamSynchronize(P->qstream);
-    if (r != hipSuccess) return P->hip_fail(r, "query");
-    return SGC_OK;
+    *out = (uint32_t*)g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+int query_noalloc_impl(sgc_pool* P, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out) {
+    GraphState* G = graph_of(P, gid);
+    if (!G || (!ids && count) || (!out && count)) return SGC_ERR_INVALID;
+    if (count > SGC_QUERY_NOALLOC_MAX)
+        return P->fail(SGC_ERR_RANGE, "query_noalloc: " + std::to_string(count) + " ids exceed the " +
+                                          std::to_string(SGC_QUERY_NOALLOC_MAX) + " of its static staging");
+    if (count == 0) return SGC_OK;
+    std::lock_guard<std::mutex> run(P->run_mu);
+    std::lock_guard<std::mutex> lk(P->qmu);
+    if (G->published < 0) {
+        memset(out, 0, (size_t)count * 2);
+        return SGC_OK;
+    }
+    std::lock_guard<std::mutex> sk(g_scratch_mu);
+    uint32_t* base = nullptr;
+    hipError_t r = scratch_on(P->device, &base);
+    if (r != hipSuccess) return P->hip_fail(r, "query_noalloc: scratch address");
+    return query_run(P, G, ids, count, out, base, (uint16_t*)(base + SGC_QUERY_NOALLOC_MAX));
 }
 
 }  // namespace
@@ -2418,6 +2488,10 @@ int sgc_pool_submit(sgc_pool* p, const sgc_edge_update* updates, uint64_t count,
 int sgc_pool_query(sgc_pool* p, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors) {
     if (!p) return SGC_ERR_INVALID;
     return query_impl(p, gid, ids, count, out_colors);
+}
+int sgc_pool_query_noalloc(sgc_pool* p, uint32_t gid, const uint32_t* ids, uint32_t count, uint16_t* out_colors) {
+    if (!p) return SGC_ERR_INVALID;
+    return query_noalloc_impl(p, gid, ids, count, out_colors);
 }
 int sgc_pool_download_words(sgc_pool* p, uint32_t gid, uint32_t* words) {
     GraphState* G = graph_of(p, gid);
