@@ -96,7 +96,7 @@ def make_stream(S, w, g, size, kind=None, held=None):
     return S.generate_bench_stream_edges(g, size, 50, 42)
 
 
-SWEEP_ROW_BYTES = {"compact": 8, "rebuild": 16, "tier1": 12, "t2_check": 12}
+SWEEP_ROW_BYTES = {"compact": 8, "rebuild": 16, "tier1": 12, "t2_check": 12, "tier1_serial": 12}
 
 
 def kernel_bytes(name, launches, n, tot, live_frac):
@@ -126,7 +126,8 @@ def kernel_bytes(name, launches, n, tot, live_frac):
 
 
 ROCPROF_NAMES = {"apply": "k_apply_elem", "group": "k_group_link", "rebuild": "k_rebuild", "tier1": "k_tier1_elem",
-                 "compact": "k_compact", "build_list": "k_build_list", "snapshot": "k_snapshot"}
+                 "tier1_serial": "k_tier1_serial",
+                 "compact": "k_compact", "build_list": "k_list_tiles", "snapshot": "k_snapshot"}
 
 
 def measured_traffic(workload, kernel):
@@ -138,6 +139,12 @@ def measured_traffic(workload, kernel):
     t = json.load(open(path))
     b = t["bytes_per_launch"].get(ROCPROF_NAMES.get(kernel, kernel))
     return (int(b), t["source"]) if b else (None, None)
+
+
+def tuning_knobs(args):
+    """The sgc_tuning knobs given on the command line."""
+    return {k: getattr(args, k) for k in ("long_row", "sweep_blocks", "serial_repair", "serial_conflicts",
+                                          "fresh_repair") if getattr(args, k) is not None}
 
 
 def cpu_baseline(g, state, bench, E, batches, threads):
@@ -176,6 +183,8 @@ def main():
     ap.add_argument("--fresh-repair", type=int, default=None, help="tuning: 0/1/2 exact-colour Tier-1 selection")
     ap.add_argument("--serial-repair", type=int, default=None,
                     help="tuning: graphs of at most this many vertices run Tier-1 in vertex order")
+    ap.add_argument("--serial-conflicts", type=int, default=None,
+                    help="tuning: ... once two parallel sweeps have left at most this many conflicts")
     ap.add_argument("--profile-summary", default=None, metavar="DIR",
                     help="summarise the rocprofv3 output under DIR (timed window = the last --steps steps) and exit")
     ap.add_argument("--requests", type=int, default=2000, help="served workload: requests of each capacity phase")
@@ -231,9 +240,7 @@ def main():
 
     eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True,
                                              full_sweeps=args.full_sweeps))
-    knobs = {k: v for k, v in (("long_row", args.long_row), ("sweep_blocks", args.sweep_blocks),
-                               ("serial_repair", args.serial_repair), ("fresh_repair", args.fresh_repair))
-             if v is not None}
+    knobs = tuning_knobs(args)
     if knobs:
         eng.set_tuning(**knobs)
     t = time.perf_counter()
@@ -515,29 +522,45 @@ def run_batched(S, args, w):
     value = n_upd / (st["ms"] / 1e3)
     legs["captured_rounds"] = value
     stats["captured_rounds"] = finish_leg(pool, gids, c0)
+    after_upd, after_ms = n_upd - st["tune_updates"], st["ms"] - st["tune_ms"]
+    after = after_upd / (after_ms / 1e3) if after_upd and after_ms > 0 else None
     log(f"captured rounds: {rounds} rounds ({st['host_rounds']} finished by the host), {st['ms']:.1f} ms = "
-        f"{value / 1e6:.2f} M updates/s; autotuner ns/update {st['candidate_ns_per_update']} -> {st['chosen']}")
+        f"{value / 1e6:.2f} M updates/s; autotuner ns/update {st['candidate_ns_per_update']} on "
+        f"{st['candidate_samples']} rounds each -> {st['chosen']}; tuning took rounds 0..{st['tune_rounds']} "
+        f"({st['tune_updates']} updates, {st['tune_ms']:.1f} ms), the rest ran at "
+        f"{(after or 0) / 1e6:.2f} M updates/s")
     pool.batch_release()
 
     # Every graph's live colouring must be proper on its rows (the reference's
-    # check).  A graph whose rows never overflowed holds exactly the stream's
-    # live edges, so each of its published snapshots is also checked against
-    # the edge set the stream implies at that point; rows that overflowed may
-    # have dropped edges (as the reference's do), so theirs are not.
-    parity = {"graphs_overflowed": 0, "snapshots_validated": 0, "snapshots_improper": 0}
+    # check), and every published snapshot is checked against the edge set the
+    # stream implies at that point.  A graph that never lost a half-edge (to a
+    # full overflow buffer or a failed drain; the reference drops those too)
+    # holds exactly the stream's live edges and must show no violation; a graph
+    # that lost some can only disagree on the edges it no longer holds, so its
+    # violations are bounded by what it lost, and both are reported.
+    parity = {"graphs_lossless": 0, "graphs_lossy": 0, "half_edges_lost": 0, "snapshots_validated": 0,
+              "snapshots_improper_lossless": 0, "violating_edge_snapshots_lossy": 0, "lossy_within_bound": True}
     colours = {"delaunay": [], "rmat": []}
     if not args.no_validate:
         t = time.perf_counter()
         for (kind, g, L, _), gid, x, k, e in zip(spec, gids, churn, epochs, E):
             colours[kind].append(pool.validate(gid)[2])
-            if pool.info(gid)["overflow_events"]:
-                parity["graphs_overflowed"] += 1
-                continue
             bad = pool.validate_history(gid, S.edge_intervals(g, x), [min(e * (j + 1), L) for j in range(k)])
             parity["snapshots_validated"] += k
-            parity["snapshots_improper"] += int(np.count_nonzero(bad))
-        log(f"{parity['graphs_overflowed']} graphs overflowed; validated {parity['snapshots_validated']} snapshots "
-            f"of the others against the stream: {parity['snapshots_improper']} improper ({time.perf_counter() - t:.1f}s)")
+            lost = pool.info(gid)["lost"]
+            if lost == 0:
+                parity["graphs_lossless"] += 1
+                parity["snapshots_improper_lossless"] += int(np.count_nonzero(bad))
+            else:
+                parity["graphs_lossy"] += 1
+                parity["half_edges_lost"] += int(lost)
+                parity["violating_edge_snapshots_lossy"] += int(bad.sum())
+                parity["lossy_within_bound"] = parity["lossy_within_bound"] and int(bad.max()) <= lost
+        log(f"validated {parity['snapshots_validated']} snapshots against the stream: "
+            f"{parity['graphs_lossless']} graphs lost nothing ({parity['snapshots_improper_lossless']} improper "
+            f"snapshots), {parity['graphs_lossy']} lost {parity['half_edges_lost']} half-edges to full rows "
+            f"({parity['violating_edge_snapshots_lossy']} violating edge-snapshots on them, within the loss: "
+            f"{parity['lossy_within_bound']}) ({time.perf_counter() - t:.1f}s)")
     pool.close()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "updates/s", "n_gpus": 1, "steps": rounds,
@@ -554,7 +577,10 @@ def run_batched(S, args, w):
                                                          stats["one_graph_at_a_time"]["max_color_sum"], 4),
         "host_rounds_kernels_ms": {k: round(v[0], 2) for k, v in host_k.items()},
         "autotune": {"candidates": st["candidate"], "ns_per_update": st["candidate_ns_per_update"],
-                     "chosen": st["chosen"], "rounds_each": args.autotune_rounds},
+                     "samples": st["candidate_samples"], "chosen": st["chosen"],
+                     "rounds_each": args.autotune_rounds, "tuning_rounds": st["tune_rounds"],
+                     "tuning_updates": st["tune_updates"], "tuning_ms": round(st["tune_ms"], 2),
+                     "after_tuning_updates_per_s": round(after, 1) if after else None},
         "parity": {**parity, "max_color_by_shape": {k: (max(v) if v else None) for k, v in colours.items()}},
         "roofline": None, "cpu_baseline": None,
     }
@@ -681,6 +707,9 @@ def run_served(S, args, w):
     # repaired once; the timed phases cut every tenant's epochs at E
     cfg = S.StreamGCConfig(epoch_size=1 << 30, batch_size=1 << 16, exact_histogram=True, repair_every_epoch=True)
     pool = S.GraphPool(max_graphs=T, batch_capacity=1 << 16, repair_every_epoch=True, exact_histogram=True)
+    knobs = tuning_knobs(args)
+    if knobs:
+        pool.set_tuning(**knobs)
     gids = [pool.add(g, cfg) for _, g in tenants]
     used, budget, ng = pool.usage()
     t = time.perf_counter()
@@ -762,8 +791,10 @@ def run_served(S, args, w):
         tot1 = pool.totals()
         n = sum(len(r) for _, r in reqs)
         d = {k: tot1[k] - tot0[k] for k in tot1}
+        host = {k: round(st[k] / 1e3, 2) for k in ("busy_us", "intake_us", "pack_us", "apply_us", "epoch_us")}
         return dict(value=round(n / dt, 1), seconds=round(dt, 3), updates=n,
-                    launches=st["batches"], epoch_steps=st["epoch_steps"], graph_epochs=st["graph_epochs"]), kt, d
+                    launches=st["batches"], epoch_steps=st["epoch_steps"], graph_epochs=st["graph_epochs"],
+                    loop_ms={k[:-3]: v for k, v in host.items()}), kt, d
 
     # capacities (closed loop)
     reqs_p, _, _, _ = build(*draws[0])
@@ -773,7 +804,8 @@ def run_served(S, args, w):
     reqs_s, _, _, _ = build(*draws[1])
     sat_served, kt_s, d_s = saturation(S.GraphServer.BATCHED, reqs_s)
     log(f"served capacity: {sat_served['value'] / 1e6:.2f} M updates/s ({sat_served['launches']} launches, "
-        f"{sat_served['epoch_steps']} epoch steps for {sat_served['graph_epochs']} graph epochs)")
+        f"{sat_served['epoch_steps']} epoch steps for {sat_served['graph_epochs']} graph epochs); loop ms "
+        f"{sat_served['loop_ms']}; kernels ms " + ", ".join(f"{k} {v[0]:.1f}" for k, v in kt_s.items()))
     roofline, per_kernel = roofline_of(kt_s, d_s, sum(g.n for _, g in tenants), "c5")
 
     # the offered-load sweep (open loop)

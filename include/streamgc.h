@@ -113,8 +113,9 @@ typedef struct sgc_totals {
     uint64_t overflow_events;
     uint64_t live_slots;         /* live row slots summed over the rows compacted */
     uint64_t grouped;            /* parallel-batch updates applied in stream order by the wave owning their edge */
-    uint64_t sweep_rows[4];      /* rows visited by: compaction, bitmap/histogram rebuild, Tier-1, Tier-2 check */
-    uint64_t sweep_slots[4];     /* row slots read by those sweeps */
+    uint64_t sweep_rows[5];      /* rows visited by: compaction, bitmap/histogram rebuild, Tier-1 (parallel
+                                    sweeps), Tier-2 check, Tier-1 (sweeps in vertex order) */
+    uint64_t sweep_slots[5];     /* row slots read by those sweeps */
 } sgc_totals;
 
 typedef struct sgc_engine sgc_engine;
@@ -354,7 +355,8 @@ typedef struct sgc_batch_config {
     uint32_t t1_sweeps;        /* Tier-1 sweeps captured per round (at most max_repair_iters); a graph still
                                   repairing after them makes the host finish the round.  0: start with 2 and
                                   double them after each round the host finished for that reason */
-    uint32_t autotune_rounds;  /* > 0: rounds given to each launch-shape candidate (hipGraph mode) */
+    uint32_t autotune_rounds;  /* > 0: captured rounds each launch-shape candidate is timed on (hipGraph mode);
+                                  rounds the host finished do not count, and a recapture restarts the timing */
     uint32_t t2_passes;        /* device Tier-2 passes captured per round (0: 4); a graph needing more, or a
                                   Tier-2 on the host (SGC_HOST_TIER2), makes the host finish the round */
 } sgc_batch_config;
@@ -366,8 +368,12 @@ typedef struct sgc_batch_stats {
     uint32_t chosen;           /* the candidate kept */
     uint64_t updates;
     double ms;                 /* wall time of the run, first launch to the last snapshot published */
-    double candidate_ns_per_update[SGC_BATCH_CANDIDATES];
+    double candidate_ns_per_update[SGC_BATCH_CANDIDATES];   /* over the candidate's samples (0: none) */
     sgc_tuning candidate[SGC_BATCH_CANDIDATES];
+    uint32_t candidate_samples[SGC_BATCH_CANDIDATES];         /* captured rounds each candidate was timed on */
+    uint32_t tune_rounds;      /* rounds [0, tune_rounds) ran while tuning (included in ms) */
+    uint64_t tune_updates;     /* their updates */
+    double tune_ms;            /* wall time from the start of the run to the end of tuning */
 } sgc_batch_stats;
 void sgc_default_batch_config(sgc_batch_config* c);
 int sgc_pool_batch_prepare(sgc_pool* p, const sgc_edge_update* updates, uint64_t count, const sgc_batch_config* cfg,
@@ -413,7 +419,11 @@ typedef struct sgc_server_stats {
     uint64_t queries;
     uint64_t queued_updates;  /* submitted, not yet applied */
     double span_us;           /* first submit to last publish */
-    double busy_us;           /* loop time in turns that did work */
+    double busy_us;           /* loop time in turns that did work, split into: */
+    double intake_us;         /*   admissions, removals, routing arrived requests */
+    double pack_us;           /*   packing launches on the host */
+    double apply_us;          /*   sgc_pool_apply (upload, grouping and per-update kernels) */
+    double epoch_us;          /*   sgc_pool_epoch (epoch steps up to the published snapshot) */
 } sgc_server_stats;
 void sgc_default_server_config(sgc_server_config* c);
 int sgc_server_create(sgc_pool* pool, const sgc_server_config* cfg, sgc_server** out);

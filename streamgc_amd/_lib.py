@@ -53,9 +53,9 @@ class Totals(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "updates", "scan_slots", "hd_endpoints", "recolors", "recolor_cap_slots", "recolor_live",
         "epochs", "t1_sweeps", "t2_runs", "overflow_events", "live_slots", "grouped")] + \
-        [("sweep_rows", C.c_uint64 * 4), ("sweep_slots", C.c_uint64 * 4)]
+        [("sweep_rows", C.c_uint64 * 5), ("sweep_slots", C.c_uint64 * 5)]
 
-    SWEEPS = ("compact", "rebuild", "tier1", "t2_check")
+    SWEEPS = ("compact", "rebuild", "tier1", "t2_check", "tier1_serial")
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("sweep_")}
@@ -91,7 +91,7 @@ class ReplayStats(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("updates_applied", "batches", "epoch_steps", "graph_epochs",
                                           "requests_done", "queries", "queued_updates")] + \
-               [("span_us", C.c_double), ("busy_us", C.c_double)]
+               [(k, C.c_double) for k in ("span_us", "busy_us", "intake_us", "pack_us", "apply_us", "epoch_us")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -118,14 +118,18 @@ class BatchStats(C.Structure):
     _fields_ = [("rounds", C.c_uint32), ("host_rounds", C.c_uint32), ("candidates", C.c_uint32),
                 ("chosen", C.c_uint32), ("updates", C.c_uint64), ("ms", C.c_double),
                 ("candidate_ns_per_update", C.c_double * BATCH_CANDIDATES),
-                ("candidate", Tuning * BATCH_CANDIDATES)]
+                ("candidate", Tuning * BATCH_CANDIDATES),
+                ("candidate_samples", C.c_uint32 * BATCH_CANDIDATES),
+                ("tune_rounds", C.c_uint32), ("tune_updates", C.c_uint64), ("tune_ms", C.c_double)]
 
     def as_dict(self):
         k = max(self.candidates, 1)
         return {"rounds": self.rounds, "host_rounds": self.host_rounds, "candidates": self.candidates,
                 "chosen": self.chosen, "updates": self.updates, "ms": self.ms,
                 "candidate_ns_per_update": [self.candidate_ns_per_update[i] for i in range(k)],
-                "candidate": [{f: getattr(self.candidate[i], f) for f, _ in Tuning._fields_} for i in range(k)]}
+                "candidate": [{f: getattr(self.candidate[i], f) for f, _ in Tuning._fields_} for i in range(k)],
+                "candidate_samples": [self.candidate_samples[i] for i in range(k)],
+                "tune_rounds": self.tune_rounds, "tune_updates": self.tune_updates, "tune_ms": self.tune_ms}
 
 
 class GraphInfo(C.Structure):

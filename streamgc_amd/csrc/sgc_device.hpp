@@ -73,9 +73,10 @@ enum : uint32_t {
 // Launch-wide u64 totals (algorithmic counters).
 enum : uint32_t {
     T_UPDATES = 0, T_SCAN, T_HD, T_RECOLOR, T_RC_CAP, T_RC_LIVE, T_OVF_EVENTS, T_LIVE_SLOTS, T_GROUPED,
-    T_SWEEP_ROWS,                  // 4 counters: rows visited by compact, rebuild, tier-1, tier-2 check
-    T_SWEEP_SLOTS = T_SWEEP_ROWS + 4,  // 4 counters: row slots those sweeps read
-    T_COUNT = T_SWEEP_SLOTS + 4
+    T_SWEEP_ROWS,                  // 5 counters: rows visited by compact, rebuild, tier-1 (parallel),
+                                   // tier-2 check, tier-1 (vertex order)
+    T_SWEEP_SLOTS = T_SWEEP_ROWS + 5,  // 5 counters: row slots those sweeps read
+    T_COUNT = T_SWEEP_SLOTS + 5
 };
 // The totals are striped: T_STRIPES copies of the counters, T_STRIDE u64
 // apart (two cache lines), a block adding into copy blockIdx % T_STRIPES;
@@ -318,8 +319,8 @@ __device__ __forceinline__ void on_neighbor_recolor(const GraphDev& g, uint32_t 
 // A/src/kernels/local_update.cu:12-40); a later delete then removes one copy
 // per row and the two rows disagree.  Such endpoints are maintained like an
 // overflowed edge's (sticky, and counted in C_ASYM).
-__device__ __forceinline__ void flag_duplicate(const GraphDev& g, uint32_t x, uint32_t nb, uint32_t from,
-                                               uint32_t end) {
+// Whether slots [from, end) hold nb (wave-uniform).
+__device__ __forceinline__ bool row_holds(const GraphDev& g, uint32_t nb, uint32_t from, uint32_t end) {
     constexpr uint32_t CH = 2 * ROW_UNROLL;   // 64-slot chunks per round trip
     bool dup = false;
     for (uint32_t base = from; base < end && !dup; base += CH * WAVE) {
@@ -334,11 +335,17 @@ __device__ __forceinline__ void flag_duplicate(const GraphDev& g, uint32_t x, ui
         for (uint32_t q = 0; q < CH; q++) hit |= val[q] == nb;
         dup = ballot(hit) != 0;
     }
-    if (dup && lane() == 0) {
-        add32(&g.ctr[C_ASYM], 1u);
-        mark_sticky(g, x);
-        mark_sticky(g, nb);
-    }
+    return dup;
+}
+__device__ __forceinline__ void mark_duplicate(const GraphDev& g, uint32_t x, uint32_t nb) {
+    if (lane() != 0) return;
+    add32(&g.ctr[C_ASYM], 1u);
+    mark_sticky(g, x);
+    mark_sticky(g, nb);
+}
+__device__ __forceinline__ void flag_duplicate(const GraphDev& g, uint32_t x, uint32_t nb, uint32_t from,
+                                               uint32_t end) {
+    if (row_holds(g, nb, from, end)) mark_duplicate(g, x, nb);
 }
 
 __device__ __forceinline__ uint32_t row_insert(const GraphDev& g, uint32_t x, uint32_t nb, bool use_overflow, bool* failed,
@@ -629,9 +636,15 @@ __device__ __forceinline__ bool recolor(const GraphDev& g, uint32_t x, WaveAcc& 
 }
 
 // A/src/kernels/recolor.cu:145-163; cur = x's colour (read by the caller
-// after the removal).
-__device__ __forceinline__ void try_reduce(const GraphDev& g, uint32_t x, uint16_t freed, uint16_t cur, WaveAcc& acc) {
+// after the removal).  early = x's bitmap as read before the removal, hd =
+// hd_map[x].  Bitmap bits are only ever added between rebuilds, so when that
+// earlier read already leaves no free colour below cur (fewer than 64 bits
+// set), a read now would not either, and the reduction ends without it.
+__device__ __forceinline__ void try_reduce(const GraphDev& g, uint32_t x, uint16_t freed, uint16_t cur, uint64_t early,
+                                           uint32_t hd, WaveAcc& acc) {
     if (cur == 0 || freed >= cur) return;
+    const bool by_hist = hd != INVALID && !(g.flags & ABL_FORCE_BITMAP);
+    if (!by_hist && ~early && (uint32_t)__builtin_ctzll(~early) + 1u >= cur) return;
     if (find_color(g, x) < cur) recolor(g, x, acc);
 }
 
@@ -752,7 +765,24 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
         // each row lost one copy of the edge and gained one, so the rows hold
         // as many copies as before (and if they disagreed, both endpoints are
         // sticky already).
-        const bool skip_dup = (stu && stv) || (hinted && !scan_u && !scan_v);
+        bool skip_dup = (stu && stv) || (hinted && !scan_u && !scan_v);
+        if (!skip_dup && two && pu && pv) {
+            // The edge is in both rows now.  Unless both endpoints are sticky,
+            // its two rows held it equally often before this insert (a row
+            // that gains a second copy, or keeps one the other row lost, makes
+            // both endpoints sticky).  So this insert made a second copy iff
+            // one row had the edge already while the other took a hole, or
+            // both took a hole and the row with fewer slots past its hole
+            // holds a copy there (the other row then holds one too): one short
+            // row is read instead of both, which spares a hub row's tail.
+            bool dup = hadu != hadv;
+            if (!hadu && !hadv) {
+                const uint32_t tail_u = fu > pu ? fu - pu : 0u, tail_v = fv > pv ? fv - pv : 0u;
+                dup = tail_u <= tail_v ? row_holds(g, v, su + pu, su + fu) : row_holds(g, u, sv + pv, sv + fv);
+            }
+            if (dup) mark_duplicate(g, u, v);
+            skip_dup = true;
+        }
         uint32_t p = finish_insert(g, u, v, su, eu, pu, fu, skip_dup, acc);
         p += finish_insert(g, v, u, sv, ev, pv, fv, skip_dup, acc);
         if (lane() == 0) {
@@ -800,6 +830,7 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
     } else {
         const uint32_t ru = ld(&g.word[u]), rv = ld(&g.word[v]);
         const uint32_t lu = ld(&g.fill[u]), lv = ld(&g.fill[v]);
+        const uint64_t bu = ld(&g.bitmap[u]), bv = ld(&g.bitmap[v]);   // for the reductions below
         const uint16_t cu = color_of(bcast0(ru)), cv = color_of(bcast0(rv));   // freed at v, freed at u
         const uint32_t tu = su + bcast0(lu), tv = sv + bcast0(lv);
         scan_pair<false>(g, true, u, v, su, tu, two, v, u, sv, tv, pu, pv, fu, fv, hadu, hadv, xu, xv);
@@ -818,8 +849,8 @@ __device__ __forceinline__ void apply_update(const GraphDev& g, uint32_t u, uint
         }
         on_removed_neighbor(g, u, hu, cv);
         on_removed_neighbor(g, v, hv, cu);
-        try_reduce(g, u, cv, cu, acc);
-        try_reduce(g, v, cu, cv, acc);
+        try_reduce(g, u, cv, cu, bu, hu, acc);
+        try_reduce(g, v, cu, cv, bv, hv, acc);
     }
     acc.updates++;
     acc.hd += (hu != INVALID) + (hv != INVALID);

@@ -136,38 +136,61 @@ struct GraphState {
     }
 };
 
-// A pinned host mirror of a small device array, uploaded in stream order; the
-// mirror is rewritten only after its last upload has completed.
+// A pinned host mirror of a small device array, uploaded in stream order.
+// The mirror rotates over STAGED_RING pinned copies: writable() moves to the
+// copy uploaded STAGED_RING uploads ago once that copy's upload has completed
+// (nearly always long done), so the host rarely waits for the GPU to catch
+// up before it writes the next parameters.  keep = false when the caller
+// rewrites everything it uploads (the copy's old contents are then left as
+// they are); keep = true carries the current contents over.
+constexpr int STAGED_RING = 4;
 template <class T>
 struct Staged {
     T* h = nullptr;
     T* d = nullptr;
     size_t count = 0;
-    hipEvent_t done = nullptr;
+    T* ring[STAGED_RING] = {};
+    hipEvent_t done[STAGED_RING] = {};
+    int cur = 0;
     hipError_t alloc(size_t c) {
         count = c;
-        hipError_t e = hipHostMalloc((void**)&h, c * sizeof(T));
-        if (e == hipSuccess) e = hipMalloc((void**)&d, c * sizeof(T));
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
-        if (e == hipSuccess) memset(h, 0, c * sizeof(T));
+        hipError_t e = hipMalloc((void**)&d, c * sizeof(T));
+        for (int k = 0; k < STAGED_RING && e == hipSuccess; k++) {
+            e = hipHostMalloc((void**)&ring[k], c * sizeof(T));
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&done[k], hipEventDisableTiming);
+            if (e == hipSuccess) memset(ring[k], 0, c * sizeof(T));
+        }
+        cur = 0;
+        h = ring[0];
         return e;
     }
-    hipError_t writable() { return hipEventSynchronize(done); }
+    hipError_t writable(bool keep = true) {
+        const int next = (cur + 1) % STAGED_RING;
+        hipError_t e = hipEventSynchronize(done[next]);
+        if (e != hipSuccess) return e;
+        if (keep) memcpy(ring[next], ring[cur], count * sizeof(T));
+        cur = next;
+        h = ring[cur];
+        return hipSuccess;
+    }
     hipError_t upload(hipStream_t s, size_t first = 0, size_t n = (size_t)-1) {
         if (n == (size_t)-1) n = count - first;
         if (n == 0) return hipSuccess;
         hipError_t e = hipMemcpyAsync(d + first, h + first, n * sizeof(T), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipEventRecord(done, s);
+        if (e == hipSuccess) e = hipEventRecord(done[cur], s);
         return e;
     }
     void release() {
-        if (done) (void)hipEventSynchronize(done);
-        if (h) (void)hipHostFree(h);
+        for (int k = 0; k < STAGED_RING; k++) {
+            if (done[k]) (void)hipEventSynchronize(done[k]);
+            if (ring[k]) (void)hipHostFree(ring[k]);
+            if (done[k]) (void)hipEventDestroy(done[k]);
+            ring[k] = nullptr;
+            done[k] = nullptr;
+        }
         if (d) (void)hipFree(d);
-        if (done) (void)hipEventDestroy(done);
         h = nullptr;
         d = nullptr;
-        done = nullptr;
     }
 };
 
@@ -216,8 +239,16 @@ struct sgc_pool {
     hipStream_t stream = nullptr, qstream = nullptr;
     std::vector<GraphState> gs;
     Staged<GraphDev> graphs;              // descriptors
-    Staged<uint32_t> vbase, gbase, active;   // all graphs: vertex prefix, edge-key offsets, active flags
-    Staged<uint32_t> sbase, sgid;            // the active graphs: vertex prefix and ids
+    Staged<uint32_t> vbase, gbase, vtile;    // all graphs: vertex prefix, edge-key offsets, 4096-vertex tile prefix
+    // The graphs taking part in the next launches, uploaded at once: per-graph
+    // flags (max_graphs), then the active graphs' vertex prefix
+    // (max_graphs + 1), ids (max_graphs) and tile prefix (max_graphs + 1).
+    Staged<uint32_t> act;
+    std::vector<uint8_t> act_last;           // the flags on the device (empty: unknown)
+    uint32_t act_tiles = 0, all_tiles = 0;
+    size_t a_sbase() const { return max_graphs; }
+    size_t a_sgid() const { return 2 * (size_t)max_graphs + 1; }
+    size_t a_stile() const { return 3 * (size_t)max_graphs + 1; }
     Staged<uint32_t> perm, seg;           // deterministic per-graph replay
     uint32_t* d_ctr = nullptr;            // max_graphs * C_COUNT
     uint32_t* h_ctr = nullptr;            // pinned copy
@@ -274,17 +305,21 @@ struct sgc_pool {
         L.stream = stream;
         L.graphs = graphs.d;
         L.ng = ng ? ng : 1;
-        L.active = use_active ? active.d : nullptr;
+        L.active = use_active ? act.d : nullptr;
         if (use_active && act_n != ng) {
-            L.sbase = sbase.d;
-            L.sgid = sgid.d;
+            L.sbase = act.d + a_sbase();
+            L.sgid = act.d + a_sgid();
             L.ns = act_n;
             L.V = act_V;
+            L.stile = act.d + a_stile();
+            L.tiles = act_tiles;
         } else {
             L.sbase = ng > 1 ? vbase.d : nullptr;
             L.sgid = nullptr;
             L.ns = ng ? ng : 1;
             L.V = V;
+            L.stile = vtile.d;
+            L.tiles = all_tiles;
         }
         L.gbase = gbase.d;
         L.totals = d_totals;
@@ -341,26 +376,34 @@ struct sgc_pool {
     // vertex space (uploaded in stream order).  A LaunchCtx made with
     // ctx(true) describes the set current when it was made.
     hipError_t set_active(const std::vector<uint32_t>& gids) {
-        hipError_t e = active.writable();
-        if (e == hipSuccess) e = sbase.writable();
-        if (e == hipSuccess) e = sgid.writable();
+        std::vector<uint8_t> fl(ng, 0);
+        for (uint32_t g : gids) fl[g] = 1;
+        if (fl == act_last) return hipSuccess;   // the same set is on the device already
+        hipError_t e = act.writable(false);      // rewritten below
         if (e != hipSuccess) return e;
-        memset(active.h, 0, ng * 4);
-        for (uint32_t g : gids) active.h[g] = 1;
+        uint32_t* flags_h = act.h;
+        uint32_t* sbase_h = act.h + a_sbase();
+        uint32_t* sgid_h = act.h + a_sgid();
+        uint32_t* stile_h = act.h + a_stile();
         uint64_t v = 0;
+        uint32_t tiles = 0;
         act_n = 0;
         for (uint32_t g = 0; g < ng; g++) {
-            if (!active.h[g]) continue;
-            sbase.h[act_n] = (uint32_t)v;
-            sgid.h[act_n] = g;
+            flags_h[g] = fl[g];
+            if (!fl[g]) continue;
+            sbase_h[act_n] = (uint32_t)v;
+            stile_h[act_n] = tiles;
+            sgid_h[act_n] = g;
             act_n++;
             v += gs[g].n;
+            tiles += (gs[g].n + 4095) / 4096;
         }
-        sbase.h[act_n] = (uint32_t)v;
+        sbase_h[act_n] = (uint32_t)v;
+        stile_h[act_n] = tiles;
         act_V = (uint32_t)v;
-        e = active.upload(stream, 0, ng);
-        if (e == hipSuccess && act_n) e = sbase.upload(stream, 0, act_n + 1);
-        if (e == hipSuccess && act_n) e = sgid.upload(stream, 0, act_n);
+        act_tiles = tiles;
+        e = act.upload(stream);
+        act_last = e == hipSuccess ? fl : std::vector<uint8_t>();
         return e;
     }
     hipError_t set_active_iterating(const std::vector<uint32_t>& gids) {
@@ -371,30 +414,41 @@ struct sgc_pool {
     }
 
     hipError_t refresh_layout() {
-        hipError_t e = vbase.writable();
-        if (e == hipSuccess) e = gbase.writable();
+        hipError_t e = vbase.writable(false);   // rewritten below
+        if (e == hipSuccess) e = gbase.writable(false);
+        if (e == hipSuccess) e = vtile.writable(false);
         if (e != hipSuccess) return e;
         ng = 0;
         for (uint32_t g = 0; g < max_graphs; g++)
             if (gs[g].used) ng = g + 1;
         uint64_t v = 0;
+        uint32_t tiles = 0;
         for (uint32_t g = 0; g < ng; g++) {
             vbase.h[g] = (uint32_t)v;
             gbase.h[g] = (uint32_t)v;
-            if (gs[g].used) v += gs[g].n;
+            vtile.h[g] = tiles;
+            if (gs[g].used) {
+                v += gs[g].n;
+                tiles += (gs[g].n + 4095) / 4096;
+            }
         }
         vbase.h[ng] = (uint32_t)v;
+        vtile.h[ng] = tiles;
         V = (uint32_t)v;
+        all_tiles = tiles;
         act_n = ng;   // every graph until the next set_active
         act_V = V;
+        act_tiles = tiles;
+        act_last.clear();
         e = vbase.upload(stream, 0, ng + 1);
         if (e == hipSuccess) e = gbase.upload(stream, 0, ng + 1);
+        if (e == hipSuccess) e = vtile.upload(stream, 0, ng + 1);
         if (e == hipSuccess && V > list_cap) {
             e = hipStreamSynchronize(stream);
             if (d_list) (void)hipFree(d_list);
             d_list = nullptr;
             list_cap = 0;
-            if (e == hipSuccess) e = hipMalloc((void**)&d_list, list_words(V) * 4);
+            if (e == hipSuccess) e = hipMalloc((void**)&d_list, list_words(V, max_graphs) * 4);
             if (e == hipSuccess) list_cap = V;
         }
         return e;
@@ -666,8 +720,8 @@ int sgc_pool::apply(const sgc_edge_update* upd, uint32_t count, bool on_device, 
     }
     LaunchCtx L = ctx(false);
     if (deterministic()) {
-        e = perm.writable();
-        if (e == hipSuccess) e = seg.writable();
+        e = perm.writable(false);   // rewritten below, as far as uploaded
+        if (e == hipSuccess) e = seg.writable(false);
         if (e != hipSuccess) return hip_fail(e, "segments");
         uint32_t nseg = 0;
         if (!multi) {
@@ -1054,7 +1108,7 @@ int sgc_pool::epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<
     for (uint32_t g : gids) ovf_cap = std::max(ovf_cap, gs[g].cfg.overflow_capacity);
     if (e == hipSuccess) e = timed("drain", [&] { return launch_drain(L, deterministic(), ovf_cap); });
     if (e != hipSuccess) return hip_fail(e, "compact/drain");
-    if (repair || out)
+    if (out)   // drain statistics (the repair below reads the counters it needs itself)
         if ((r = read_drain(gids))) return r;
     if (repair) {
         if (finish && rebuild_each_sweep()) {
@@ -1090,14 +1144,13 @@ void sgc_pool::release() {
     for (auto& G : gs) G.release();
     graphs.release();
     vbase.release();
-    sbase.release();
-    sgid.release();
+    act.release();
+    vtile.release();
     gbase.release();
-    active.release();
     perm.release();
     seg.release();
     for (void* p : {(void*)d_ctr, (void*)d_totals, (void*)d_batch, (void*)d_tmp16, (void*)d_qids, (void*)d_qout,
-                    (void*)eg.keys, (void*)eg.head, (void*)eg.next, (void*)eg.heads, (void*)eg.nheads,
+                    (void*)eg.keys, (void*)eg.head, (void*)eg.next, (void*)eg.heads, (void*)eg.nheads, (void*)eg.pack,
                     (void*)d_list, (void*)d_list_count,
                     (void*)d_t2_items, d_rev, (void*)d_rev_count})
         if (p) (void)hipFree(p);
@@ -1528,22 +1581,36 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
         B.h_ctl[RC_ROUNDS] = B.rounds;
         e = batch_push(B);
         if (e != hipSuccess) return hip_fail(e, "batch run");
+        // Autotuning.  Every candidate is timed (HIP events around its replay)
+        // on cfg.autotune_rounds rounds that ran captured to the end: a round
+        // the host finished gives no sample, and a recapture (the captured
+        // body changed) discards the samples taken so far.  Each tuning
+        // replay gives one round to every candidate still short of samples,
+        // fewest samples first.  The fastest per update is kept for the rest
+        // of the run; the tuning rounds are reported apart (tune_*).
         const uint32_t C = (uint32_t)B.execs.size();
-        const uint32_t tune_rounds = C > 1 ? std::min(B.rounds, B.cfg.autotune_rounds * C) : 0;
-        std::vector<hipEvent_t> ev(2 * (size_t)tune_rounds, nullptr);
+        const uint32_t need = C > 1 ? B.cfg.autotune_rounds : 0;
+        std::vector<hipEvent_t> ev(2 * (size_t)C, nullptr);
         for (auto& x : ev)
             if (e == hipSuccess) e = hipEventCreate(&x);
         std::vector<double> cand_ms(C, 0.0);
         std::vector<uint64_t> cand_upd(C, 0);
+        std::vector<uint32_t> cand_n(C, 0), who;
         uint32_t best = 0, r = 0;
-        bool tuned = tune_rounds == 0;
+        bool tuned = need == 0;
         int rc = SGC_OK;
         while (e == hipSuccess && rc == SGC_OK && r < B.rounds) {
-            const uint32_t end = tuned ? B.rounds : tune_rounds;
+            who.clear();
+            if (!tuned) {
+                for (uint32_t k = 0; k < C; k++)
+                    if (cand_n[k] < need) who.push_back(k);
+                std::stable_sort(who.begin(), who.end(), [&](uint32_t a, uint32_t b) { return cand_n[a] < cand_n[b]; });
+            }
+            const uint32_t end = tuned ? B.rounds : std::min<uint32_t>(B.rounds, r + (uint32_t)who.size());
             for (uint32_t k = r; k < end && e == hipSuccess; k++) {
-                if (!tuned) e = hipEventRecord(ev[2 * k], stream);
-                if (e == hipSuccess) e = hipGraphLaunch(B.execs[tuned ? best : k % C], stream);
-                if (e == hipSuccess && !tuned) e = hipEventRecord(ev[2 * k + 1], stream);
+                if (!tuned) e = hipEventRecord(ev[2 * (k - r)], stream);
+                if (e == hipSuccess) e = hipGraphLaunch(B.execs[tuned ? best : who[k - r]], stream);
+                if (e == hipSuccess && !tuned) e = hipEventRecord(ev[2 * (k - r) + 1], stream);
             }
             if (e == hipSuccess) e = hipMemcpyAsync(B.h_ctl, B.d_ctl, RC_WORDS * 4, hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -1559,11 +1626,13 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
             const bool stopped = rd.stopped;
             const uint32_t done = rd.done;
             if (!tuned)
-                for (uint32_t k = r; k < done; k++) {
+                for (uint32_t k = r; k < done; k++) {   // rounds that ran captured to the end
                     float ms = 0;
-                    (void)hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
-                    cand_ms[k % C] += ms;
-                    cand_upd[k % C] += B.tab[2 * k + 1];
+                    (void)hipEventElapsedTime(&ms, ev[2 * (k - r)], ev[2 * (k - r) + 1]);
+                    const uint32_t q = who[k - r];
+                    cand_ms[q] += ms;
+                    cand_upd[q] += B.tab[2 * k + 1];
+                    cand_n[q]++;
                 }
             if (stopped) {
                 const uint32_t s = rd.host;
@@ -1584,6 +1653,11 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
                         }
                     }
                     if (rc) break;
+                    if (!tuned) {   // samples of the old bodies no longer apply
+                        std::fill(cand_ms.begin(), cand_ms.end(), 0.0);
+                        std::fill(cand_upd.begin(), cand_upd.end(), 0);
+                        std::fill(cand_n.begin(), cand_n.end(), 0);
+                    }
                 }
                 r = rd.next;
                 B.h_ctl[RC_ROUND] = r;
@@ -1594,18 +1668,25 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
             } else {
                 r = end;
             }
-            if (!tuned && r >= tune_rounds) {
-                tuned = true;
-                double best_v = 0;
-                for (uint32_t k = 0; k < C; k++) {
-                    if (!cand_upd[k]) continue;
-                    const double v = cand_ms[k] / (double)cand_upd[k];
-                    if (best_v == 0 || v < best_v) {
-                        best_v = v;
-                        best = k;
+            if (!tuned) {
+                bool all = true;
+                for (uint32_t k = 0; k < C; k++) all = all && cand_n[k] >= need;
+                if (all || r >= B.rounds) {
+                    tuned = true;
+                    double best_v = 0;
+                    for (uint32_t k = 0; k < C; k++) {
+                        if (!cand_n[k]) continue;
+                        const double v = cand_ms[k] / (double)std::max<uint64_t>(cand_upd[k], 1);
+                        if (best_v == 0 || v < best_v) {
+                            best_v = v;
+                            best = k;
+                        }
                     }
+                    tune = B.cands[best];
+                    S.tune_rounds = r;
+                    for (uint32_t k = 0; k < r; k++) S.tune_updates += B.tab[2 * k + 1];
+                    S.tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                 }
-                tune = B.cands[best];
             }
         }
         for (auto& x : ev)
@@ -1627,6 +1708,7 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
         for (uint32_t k = 0; k < C && k < SGC_BATCH_CANDIDATES; k++) {
             S.candidate[k] = B.cands[k];
             S.candidate_ns_per_update[k] = cand_upd[k] ? cand_ms[k] * 1e6 / (double)cand_upd[k] : 0.0;
+            S.candidate_samples[k] = cand_n[k];
         }
     }
     S.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1660,10 +1742,9 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&P->qstream, hipStreamNonBlocking);
     if (e == hipSuccess) e = P->graphs.alloc(max_graphs);
     if (e == hipSuccess) e = P->vbase.alloc(max_graphs + 1);
-    if (e == hipSuccess) e = P->sbase.alloc(max_graphs + 1);
-    if (e == hipSuccess) e = P->sgid.alloc(max_graphs);
+    if (e == hipSuccess) e = P->act.alloc(4 * (size_t)max_graphs + 2);
+    if (e == hipSuccess) e = P->vtile.alloc(max_graphs + 1);
     if (e == hipSuccess) e = P->gbase.alloc(max_graphs + 1);
-    if (e == hipSuccess) e = P->active.alloc(max_graphs);
     if (e == hipSuccess) e = P->perm.alloc(batch_cap);
     if (e == hipSuccess) e = P->seg.alloc(2 * (size_t)max_graphs);
     if (e == hipSuccess) e = hipMalloc(&P->d_ctr, (size_t)max_graphs * C_COUNT * 4);
@@ -1684,6 +1765,7 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     if (e == hipSuccess) e = hipMalloc(&P->eg.next, (size_t)batch_cap * 4);
     if (e == hipSuccess) e = hipMalloc(&P->eg.heads, (size_t)batch_cap * 4);
     if (e == hipSuccess) e = hipMalloc(&P->eg.nheads, 8);
+    if (e == hipSuccess) e = hipMalloc(&P->eg.pack, (size_t)batch_cap * 4 * sizeof(uint4));
     if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
     if (e != hipSuccess) return P->hip_fail(e, "pool create");
     return SGC_OK;
@@ -2195,7 +2277,7 @@ static int totals_impl(sgc_pool* P, sgc_totals* out) {
     out->overflow_events = P->h_totals[T_OVF_EVENTS];
     out->live_slots = P->h_totals[T_LIVE_SLOTS];
     out->grouped = P->h_totals[T_GROUPED];
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 5; k++) {
         out->sweep_rows[k] = P->h_totals[T_SWEEP_ROWS + k];
         out->sweep_slots[k] = P->h_totals[T_SWEEP_SLOTS + k];
     }

@@ -107,79 +107,12 @@ __device__ __forceinline__ uint32_t block_scan32(uint32_t x, uint32_t* total) {
     return before + inc - x;
 }
 
-// Worklist of the vertices marked in array `kind` (or sticky); their marks
-// are cleared.  One lane per vertex; each wavefront covers LIST_TILE chunks of
-// 64 vertices, and each block appends its hits with one atomic per list end.
-constexpr uint32_t LIST_TILE = 16;
-__global__ __launch_bounds__(256) void k_build_list(const GraphDev* __restrict__ graphs, Space sp, uint32_t kind,
-                                                    uint32_t long_row, uint32_t* __restrict__ list, uint32_t cap,
-                                                    uint32_t* __restrict__ count, const uint32_t* __restrict__ active,
-                                                    const uint32_t* __restrict__ gate) {
-    __shared__ uint32_t wave_small[BLOCK / WAVE], wave_big[BLOCK / WAVE], block_at[2];
-    if (gate && *gate == 0) return;
-    const uint64_t lt_mask = (1ull << lane()) - 1ull;
-    const uint32_t wv = threadIdx.x / WAVE;
-    const uint32_t span = BLOCK * LIST_TILE;
-    for (uint32_t bbase = blockIdx.x * span; bbase < sp.total; bbase += gridDim.x * span) {
-        const uint32_t base = bbase + wv * WAVE * LIST_TILE;
-        uint64_t small_m[LIST_TILE], big_m[LIST_TILE];
-        uint32_t nsmall = 0, nbig = 0;
-#pragma unroll
-        for (uint32_t t = 0; t < LIST_TILE; t++) {
-            const uint32_t i = base + t * WAVE + lane();
-            bool hit = false, big = false;
-            uint32_t gid = 0, v = 0;
-            if (i < sp.total) sp.locate(i, gid, v);
-            if (i < sp.total && (!active || active[gid])) {
-                const GraphDev& g = graphs[gid];
-                SGC_G uint8_t* d = gp(g.dirty) + v;
-                const size_t st = g.dstride;
-                const bool marked = d[kind * st] != 0;
-                hit = marked || d[M_STICKY * st] != 0;
-                if (marked) d[kind * st] = 0;
-                if (hit) big = gp(g.fill)[v] > long_row;
-            }
-            small_m[t] = ballot(hit && !big);
-            big_m[t] = ballot(big);
-            nsmall += (uint32_t)__builtin_popcountll(small_m[t]);
-            nbig += (uint32_t)__builtin_popcountll(big_m[t]);
-        }
-        if (lane() == 0) {
-            wave_small[wv] = nsmall;
-            wave_big[wv] = nbig;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t ts = 0, tb = 0;
-            for (uint32_t k = 0; k < BLOCK / WAVE; k++) {
-                ts += wave_small[k];
-                tb += wave_big[k];
-            }
-            block_at[0] = ts ? add32(&count[0], ts) : 0;
-            block_at[1] = tb ? add32(&count[1], tb) : 0;
-        }
-        __syncthreads();
-        uint32_t at = block_at[0], bt = block_at[1];
-        for (uint32_t k = 0; k < wv; k++) {
-            at += wave_small[k];
-            bt += wave_big[k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t t = 0; t < LIST_TILE; t++) {
-            const uint32_t i = base + t * WAVE + lane();
-            if ((small_m[t] >> lane()) & 1ull) list[at + (uint32_t)__builtin_popcountll(small_m[t] & lt_mask)] = i;
-            if ((big_m[t] >> lane()) & 1ull) list[cap - 1 - (bt + (uint32_t)__builtin_popcountll(big_m[t] & lt_mask))] = i;
-            at += (uint32_t)__builtin_popcountll(small_m[t]);
-            bt += (uint32_t)__builtin_popcountll(big_m[t]);
-        }
-    }
-}
-
-// The same worklist for a launch over one graph, in two passes with no
-// shared counter.  Each thread reads the marks of 4 vertices as one word
-// (mark arrays are padded to a multiple of 4) with no per-vertex branches.
-// Pass 1: each block takes a tile of LIST_TILE_V vertices, lists its hits in
+// The worklist of a sweep (the vertices marked `kind`, or sticky; their marks
+// are cleared), in two passes with no shared counter.  The vertices are cut
+// into tiles of LIST_TILE_V, each inside one graph (with several graphs, each
+// graph's tiles follow the previous graph's).  Each thread reads the marks of
+// 4 vertices as one word (mark arrays are padded to a multiple of 4) with no
+// per-vertex branches.  Pass 1: each block takes a tile, lists its hits in
 // the tile's own region of a scratch array (short rows up from the start,
 // long rows down from the end), clears their marks and stores its two
 // counts.  Pass 2: each block adds up the counts of the tiles before it and
@@ -223,19 +156,41 @@ __device__ __forceinline__ void list_tile(const GraphDev& g, uint32_t kind, uint
     }
 }
 
-__global__ __launch_bounds__(256) void k_list_tiles(const GraphDev* __restrict__ graphs, Space sp, uint32_t kind,
+__global__ __launch_bounds__(256) void k_list_tiles(const GraphDev* __restrict__ graphs, Space sp,
+                                                    const uint32_t* __restrict__ stile, uint32_t kind,
                                                     uint32_t long_row, uint32_t* __restrict__ tmp,
                                                     uint32_t* __restrict__ tcount,
                                                     const uint32_t* __restrict__ active,
                                                     const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0) return;
-    const uint32_t gid = sp.gid ? sp.gid[0] : 0u;
-    const uint32_t off = sp.base ? sp.base[0] : 0u;   // virtual index of the graph's vertex 0
-    if (active && active[gid] == 0) return;
+    const uint32_t t = blockIdx.x;
+    // Several graphs: their tiles laid end to end (stile = prefix of each
+    // part's tile count); the part holding tile t, found by the whole block.
+    uint32_t part = 0, first = 0;
+    if (stile && sp.ns > 1) {
+        uint32_t lo = 0, hi = sp.ns;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (stile[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        part = lo;
+        first = stile[lo];
+    }
+    const uint32_t gid = sp.gid ? sp.gid[part] : part;
+    const uint32_t off = sp.base ? sp.base[part] : 0u;   // virtual index of the graph's vertex 0
+    if (active && active[gid] == 0) {
+        if (stile && sp.ns > 1 && threadIdx.x == 0) {   // nothing listed from this tile
+            gp(tcount)[2 * t] = 0;
+            gp(tcount)[2 * t + 1] = 0;
+        }
+        return;
+    }
     const GraphDev g = graphs[gid];
-    const uint32_t words = (g.n + 3) / 4, t = blockIdx.x;
+    const uint32_t words = (g.n + 3) / 4;
+    const uint32_t lt_index = t - first;   // the tile within its graph
     ListTile lt;
-    list_tile(g, kind, long_row, t, words, lt);
+    list_tile(g, kind, long_row, lt_index, words, lt);
     uint32_t cs = 0, cb = 0;
 #pragma unroll
     for (uint32_t k = 0; k < LIST_WORDS; k++) {
@@ -248,7 +203,7 @@ __global__ __launch_bounds__(256) void k_list_tiles(const GraphDev* __restrict__
     SGC_G uint32_t* region = gp(tmp) + (size_t)t * LIST_TILE_V;
 #pragma unroll
     for (uint32_t k = 0; k < LIST_WORDS; k++) {
-        const uint32_t v0 = off + 4 * (t * (BLOCK * LIST_WORDS) + k * BLOCK + threadIdx.x);
+        const uint32_t v0 = off + 4 * (lt_index * (BLOCK * LIST_WORDS) + k * BLOCK + threadIdx.x);
         for (uint32_t q = 0; q < 4; q++) {
             if ((lt.small[k] >> q) & 1u) region[ps++] = v0 + q;
             if ((lt.big[k] >> q) & 1u) region[LIST_TILE_V - 1 - pb++] = v0 + q;
@@ -260,15 +215,17 @@ __global__ __launch_bounds__(256) void k_list_tiles(const GraphDev* __restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void k_list_gather(const GraphDev* __restrict__ graphs, Space sp,
+__global__ __launch_bounds__(256) void k_list_gather(const GraphDev* __restrict__ graphs, Space sp, int multi,
                                                      const uint32_t* __restrict__ tmp,
                                                      const uint32_t* __restrict__ tcount,
                                                      uint32_t* __restrict__ list, uint32_t cap,
                                                      uint32_t* __restrict__ count,
                                                      const uint32_t* __restrict__ active,
                                                      const uint32_t* __restrict__ gate) {
+    // one graph not taking part: nothing listed (several graphs: the tiles of
+    // those not taking part counted nothing)
     const uint32_t gid = sp.gid ? sp.gid[0] : 0u;
-    if ((gate && *gate == 0) || (active && active[gid] == 0)) {   // nothing listed
+    if ((gate && *gate == 0) || (!multi && active && active[gid] == 0)) {
         if (blockIdx.x == 0 && threadIdx.x < 2) count[threadIdx.x] = 0;
         return;
     }
@@ -408,6 +365,60 @@ __global__ __launch_bounds__(256) void k_group_link(const GraphDev* __restrict__
     }
 }
 
+// One thread per listed edge: the records of its updates, in stream order,
+// into the edge's own GROUP_PACK x 16-byte line of `pack`, so that the
+// per-update kernel has a whole group after one load instead of walking
+// head, list links and records one dependent load after another.  Unused
+// entries are PACK_NONE records; an edge with more than GROUP_PACK updates is
+// left to the walk (its first entry says so and carries the list entry).
+constexpr uint32_t GROUP_PACK = 4;
+constexpr uint32_t PACK_NONE = 0xFFFFFFFFu;   // x of an entry with no record (a listed record has x < n)
+constexpr uint32_t PACK_END = 0, PACK_WALK = 1;   // y of such an entry
+
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+__global__ __launch_bounds__(256) void k_group_pack(const uint4* __restrict__ upd, const uint32_t* __restrict__ ghead,
+                                                    const uint32_t* __restrict__ gnext,
+                                                    const uint32_t* __restrict__ heads, uint32_t cap,
+                                                    const uint32_t* __restrict__ nheads, uint4* __restrict__ pack,
+                                                    const uint32_t* __restrict__ dyn) {
+    if (dyn) upd += dyn[0];
+    const uint32_t nf = nheads[0], n = nf + nheads[1];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t e = j < nf ? heads[j] : heads[cap - 1 - (j - nf)];
+        uint32_t p0 = NONE, p1 = NONE, p2 = NONE, p3 = NONE;
+        bool more = false;
+        if (e & GROUP_SINGLE) {
+            p0 = e & ~GROUP_SINGLE;
+        } else {
+            p0 = gp(ghead)[e];
+            if (p0 != NONE) p1 = gp(gnext)[p0];
+            if (p1 != NONE) p2 = gp(gnext)[p1];
+            if (p2 != NONE) p3 = gp(gnext)[p2];
+            more = p3 != NONE && gp(gnext)[p3] != NONE;
+        }
+        uint4* out = pack + (size_t)j * GROUP_PACK;
+        if (more) {
+            out[0] = make_uint4(PACK_NONE, PACK_WALK, 0u, e);
+            continue;
+        }
+        cswap(p0, p1);   // sorting network for 4 (NONE sorts last)
+        cswap(p2, p3);
+        cswap(p0, p2);
+        cswap(p1, p3);
+        cswap(p1, p2);
+        const uint4 none = make_uint4(PACK_NONE, PACK_END, 0u, 0u);
+        out[0] = p0 != NONE ? upd[p0] : none;
+        out[1] = p1 != NONE ? upd[p1] : none;
+        out[2] = p2 != NONE ? upd[p2] : none;
+        out[3] = p3 != NONE ? upd[p3] : none;
+    }
+}
+
 // ---------------------------------------------------------------- updates
 __device__ __forceinline__ void apply_rec(const GraphDev* __restrict__ graphs, const uint4& r, int multi, WaveAcc& acc,
                                           RowHint* hint = nullptr) {
@@ -420,21 +431,13 @@ __device__ __forceinline__ void apply_record(const GraphDev* __restrict__ graphs
     apply_rec(graphs, upd[i], multi, acc, hint);
 }
 
-// Edge lists of up to 4 positions are applied from registers; NONE (all
-// ones) sorts last.
-__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
-    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    a = lo;
-    b = hi;
-}
-
-
 __global__ __launch_bounds__(64) void k_apply_elem(const GraphDev* __restrict__ graphs,
                                                     const uint4* __restrict__ upd, uint32_t count, int multi,
                                                     const uint32_t* __restrict__ ghead,
                                                     const uint32_t* __restrict__ gnext,
                                                     const uint32_t* __restrict__ heads, uint32_t cap,
-                                                    const uint32_t* __restrict__ nheads, uint64_t* totals,
+                                                    const uint32_t* __restrict__ nheads,
+                                                    const uint4* __restrict__ pack, uint64_t* totals,
                                                     const uint32_t* __restrict__ dyn) {
     WaveAcc acc;
     uint64_t grouped = 0;
@@ -448,48 +451,50 @@ __global__ __launch_bounds__(64) void k_apply_elem(const GraphDev* __restrict__ 
         return;
     }
     // One wavefront per edge of the batch: its updates in stream order.  The
-    // edge's list (head, then next links) holds its positions; take
-    // repeatedly the smallest position above the last one applied (groups
-    // are short).
+    // records come from the edge's packed 64-byte line (lanes 0..3 load one
+    // each), or, for a group longer than the line, from its list (head, then
+    // next links): repeatedly the record at the smallest position above the
+    // last one applied.
     const uint32_t nf = nheads[0], n = nf + nheads[1];
     for (uint32_t j = wave_id(); j < n; j += wave_count()) {
-        const uint32_t e = j < nf ? heads[j] : heads[cap - 1 - (j - nf)];   // long scans first
-        if (e & GROUP_SINGLE) {
-            apply_record(graphs, upd, e & ~GROUP_SINGLE, multi, acc);
-            continue;
+        uint4 mine = make_uint4(PACK_NONE, PACK_WALK, 0u, 0u);
+        if (pack && lane() < GROUP_PACK) mine = pack[(size_t)j * GROUP_PACK + lane()];
+        const bool walk = __builtin_amdgcn_readlane(mine.x, 0) == PACK_NONE &&
+                          __builtin_amdgcn_readlane(mine.y, 0) == PACK_WALK;
+        uint32_t h = NONE;
+        bool single = false;
+        if (walk) {
+            const uint32_t e = j < nf ? heads[j] : heads[cap - 1 - (j - nf)];   // long scans first
+            single = (e & GROUP_SINGLE) != 0;
+            h = single ? e & ~GROUP_SINGLE : gp(ghead)[e];
         }
-        const uint32_t h = gp(ghead)[e];
-        uint32_t last = 0, applied = 0;
-        RowHint hint;   // a delete's slots for the same edge's re-insert
+        RowHint hint;   // a delete's slots for the same edge's re-insert (a last delete's marks wait in it)
         hint.valid = false;
-        // Short lists (nearly all: a delete and its re-insert) are read once
-        // into registers and put in stream order by a fixed network.
-        uint32_t p0 = h, p1 = NONE, p2 = NONE, p3 = NONE, k = NONE;
-        if (p0 != NONE) p1 = gp(gnext)[p0];
-        if (p1 != NONE) p2 = gp(gnext)[p1];
-        if (p2 != NONE) p3 = gp(gnext)[p2];
-        if (p3 != NONE) k = gp(gnext)[p3];
-        const bool held = k == NONE;   // the whole list is in p0..p3
-        cswap(p0, p1);                 // sorting network for 4 (NONE sorts last)
-        cswap(p2, p3);
-        cswap(p0, p2);
-        cswap(p1, p3);
-        cswap(p1, p2);
+        uint32_t applied = 0, last = 0, last_g = 0;
         for (;;) {
-            uint32_t best = NONE;
-            if (held) {
-                best = applied == 0 ? p0 : applied == 1 ? p1 : applied == 2 ? p2 : applied == 3 ? p3 : NONE;
-            } else {   // a longer list: the smallest position above the last one applied
+            uint4 r;
+            if (!walk) {
+                if (applied == GROUP_PACK) break;
+                r = make_uint4(__builtin_amdgcn_readlane(mine.x, applied), __builtin_amdgcn_readlane(mine.y, applied),
+                               __builtin_amdgcn_readlane(mine.z, applied), __builtin_amdgcn_readlane(mine.w, applied));
+                if (r.x == PACK_NONE) break;
+            } else if (single) {
+                if (applied) break;
+                r = upd[h];
+            } else {
+                uint32_t best = NONE;
                 for (uint32_t k2 = h; k2 != NONE; k2 = gp(gnext)[k2])
                     if ((applied == 0 || k2 > last) && (best == NONE || k2 < best)) best = k2;
+                if (best == NONE) break;
+                r = upd[best];
+                last = best;
             }
-            if (best == NONE) break;
-            apply_record(graphs, upd, best, multi, acc, &hint);
-            last = best;
+            apply_rec(graphs, r, multi, acc, &hint);
+            last_g = r.w;
             applied++;
         }
-        if (hint.valid) hint_release(graphs[multi ? upd[last].w : 0u], &hint);   // marks of a last delete
-        grouped += applied - 1;
+        if (hint.valid) hint_release(graphs[multi ? last_g : 0u], &hint);   // marks of a last delete
+        grouped += applied ? applied - 1 : 0;
     }
     acc.flush(totals);
     if (grouped && totals && lane() == 0) tot_add(totals, T_GROUPED, grouped);
@@ -1199,13 +1204,16 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
                                                      const uint32_t* __restrict__ active, int fresh,
                                                      uint64_t* totals) {
     __shared__ uint32_t presence[PRESENCE_WORDS];
-    uint64_t rows = 0;
+    uint64_t rows = 0, slots = 0;
     for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
         if (active && !active[gid]) continue;
         const GraphDev g = graphs[gid];
         uint32_t cnt = 0;
         if (!g.dirty) {
-            for (uint32_t v = 0; v < g.n; v++) cnt += tier1_vertex(g, v, presence, fresh);
+            for (uint32_t v = 0; v < g.n; v++) {
+                slots += gp(g.fill)[v];
+                cnt += tier1_vertex(g, v, presence, fresh);
+            }
             rows += g.n;
         } else {
             // 1024 vertices per step: each lane reads the marks of 16 as two
@@ -1237,6 +1245,7 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
                     const uint32_t q = (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(m16, k));
                     const uint32_t u = base + 16 * k + q;
                     if (lane() == 0) gp(g.dirty)[cst + u] = 0;
+                    slots += gp(g.fill)[u];
                     cnt += tier1_vertex(g, u, presence, fresh);
                     rows++;
                     from = 16 * k + q + 1;
@@ -1245,7 +1254,10 @@ __global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict_
         }
         if (lane() == 0 && cnt) add32(&g.ctr[C_CONFLICTS], cnt);
     }
-    if (lane() == 0 && totals && rows) tot_add(totals, T_SWEEP_ROWS + 2, rows);
+    if (lane() == 0 && totals && rows) {
+        tot_add(totals, T_SWEEP_ROWS + 4, rows);
+        tot_add(totals, T_SWEEP_SLOTS + 4, slots);
+    }
 }
 
 // Vertices the host Tier-2 recoloured: their own status and their
@@ -1667,7 +1679,8 @@ hipError_t launch_apply(const LaunchCtx& L, const void* upd, uint32_t count, int
     // one-wavefront blocks: a wavefront's slot is refilled as soon as it ends
     uint32_t grid = grid_for_waves(count, 1, 4 * L.max_blocks);
     k_apply_elem<<<grid, WAVE, 0, L.stream>>>(L.graphs, (const uint4*)upd, count, multi, head, next, heads,
-                                               eg ? eg->next_cap : 0u, nheads, L.totals, L.dyn);
+                                               eg ? eg->next_cap : 0u, nheads, heads ? eg->pack : nullptr,
+                                               L.totals, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int multi, const EdgeGroups* eg) {
@@ -1679,6 +1692,9 @@ hipError_t launch_group(const LaunchCtx& L, const void* upd, uint32_t count, int
     uint32_t mg = grid_for_waves(count, 4 * WAVE, 1024);
     k_group_link<<<mg, 256, 0, L.stream>>>(L.graphs, L.gbase, (const uint4*)upd, count, multi, eg->keys, eg->head,
                                            eg->next, mask, eg->heads, eg->next_cap, eg->nheads, L.dyn);
+    if (eg->pack)
+        k_group_pack<<<mg, 256, 0, L.stream>>>((const uint4*)upd, eg->head, eg->next, eg->heads, eg->next_cap,
+                                               eg->nheads, eg->pack, L.dyn);
     return hipGetLastError();
 }
 hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t* seg, uint32_t nseg,
@@ -1720,20 +1736,22 @@ static inline uint32_t sweep_grid(const LaunchCtx& L) {
 static inline SweepList sweep_list(const LaunchCtx& L) { return SweepList{L.list, L.list_count, L.list_cap}; }
 hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count) {
     if (kind >= M_STICKY || cap < L.V) return hipErrorInvalidValue;
-    if (L.ns == 1) {   // one graph: two passes over tiles of its own vertices (scratch behind the list)
-        const uint32_t tiles = (L.V + LIST_TILE_V - 1) / LIST_TILE_V;
-        uint32_t* tmp = list + cap;
-        uint32_t* tcount = tmp + (size_t)tiles * LIST_TILE_V;
-        k_list_tiles<<<tiles, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, tmp, tcount, L.active,
-                                                  L.gate);
-        k_list_gather<<<tiles, 256, 0, L.stream>>>(L.graphs, space(L), tmp, tcount, list, cap, count, L.active,
-                                                   L.gate);
+    // Two passes over 4096-vertex tiles, each tile inside one graph (scratch
+    // behind the list): one graph's own tiles, or the tiles of several graphs
+    // laid end to end (L.stile, L.tiles).
+    const bool multi = L.ns > 1;
+    const uint32_t tiles = multi ? L.tiles : (L.V + LIST_TILE_V - 1) / LIST_TILE_V;
+    if (multi && (!L.stile || tiles > list_tiles(cap, L.ng))) return hipErrorInvalidValue;
+    if (tiles == 0) {
+        k_zero_list_count<<<1, 1, 0, L.stream>>>(count);
         return hipGetLastError();
     }
-    k_zero_list_count<<<1, 1, 0, L.stream>>>(count);
-    uint32_t grid = grid_for_waves((L.V + WAVE * LIST_TILE - 1) / (WAVE * LIST_TILE), BLOCK / WAVE, 4096);
-    k_build_list<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), kind, L.long_row, list, cap, count, L.active,
-                                             L.gate);
+    uint32_t* tmp = list + cap;
+    uint32_t* tcount = tmp + (size_t)tiles * LIST_TILE_V;
+    k_list_tiles<<<tiles, 256, 0, L.stream>>>(L.graphs, space(L), multi ? L.stile : nullptr, kind, L.long_row, tmp,
+                                              tcount, L.active, L.gate);
+    k_list_gather<<<tiles, 256, 0, L.stream>>>(L.graphs, space(L), multi ? 1 : 0, tmp, tcount, list, cap, count,
+                                               L.active, L.gate);
     return hipGetLastError();
 }
 hipError_t launch_compact(const LaunchCtx& L) {

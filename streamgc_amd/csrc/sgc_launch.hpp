@@ -19,6 +19,8 @@ struct LaunchCtx {
     const uint32_t* sgid = nullptr;    // device, their graph ids (null = 0..ns-1)
     uint32_t ns = 1;
     uint32_t V = 0;                    // vertices of the graphs taking part
+    const uint32_t* stile = nullptr;   // device, ns + 1 prefix of their 4096-vertex tile counts (ns > 1)
+    uint32_t tiles = 0;                // their tiles
     const uint32_t* gbase = nullptr;   // device, ng global-vertex offsets (pool edge keys)
     uint64_t* totals = nullptr;        // device u64 counters
     uint32_t max_blocks = 16384;       // grid cap of per-update and full-sweep launches
@@ -43,6 +45,7 @@ struct EdgeGroups {
     uint32_t* heads = nullptr;    // next_cap entries: one per edge of the batch (slot, or position | 1 << 31),
                                   // long scans from the front, the others from the back
     uint32_t* nheads = nullptr;   // device words: entries at the front, at the back
+    uint4* pack = nullptr;        // next_cap x 4 records: each listed edge's updates in stream order (or null)
     uint32_t size = 0;
     uint32_t next_cap = 0;
 };
@@ -58,10 +61,12 @@ hipError_t launch_snapshot(const LaunchCtx& L);
 hipError_t launch_compact(const LaunchCtx& L);
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
 // Worklist of the vertices marked `kind` (or sticky).  `list` holds
-// list_words(cap) u32: the list, then scratch of the one-graph builder.
+// list_words(cap, graphs) u32: the list, then the builder's scratch of
+// 4096-vertex tiles (each inside one graph, so up to one more per graph).
 hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, uint32_t cap, uint32_t* count);
-inline size_t list_words(uint32_t cap) {
-    const size_t tiles = ((size_t)cap + 4095) / 4096;   // tiles of the one-graph builder
+inline size_t list_tiles(uint32_t cap, uint32_t graphs) { return ((size_t)cap + 4095) / 4096 + graphs; }
+inline size_t list_words(uint32_t cap, uint32_t graphs) {
+    const size_t tiles = list_tiles(cap, graphs);
     return (size_t)cap + tiles * 4096 + 2 * tiles;
 }
 hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist);

@@ -110,6 +110,7 @@ struct sgc_server {
     Clock::time_point first_submit, last_publish;
     bool any_submit = false;
     double busy_us = 0;
+    double intake_us = 0, pack_us = 0, apply_us = 0, epoch_us = 0;   // loop thread only until read under mu
 
     void run();
     int turn();
@@ -226,8 +227,10 @@ void sgc_server::take_front(uint32_t g, uint32_t take, std::vector<sgc_edge_upda
 
 // One launch over `batch`, then the tenants' update counts move on.
 int sgc_server::launch(const std::vector<sgc_edge_update>& batch, const std::vector<uint32_t>& taken) {
+    const auto t0 = Clock::now();
     int r = sgc_pool_apply(pool, batch.data(), batch.size(), 0);
     if (r) return r;
+    const double dt = us_between(t0, Clock::now());
     for (uint32_t g = 0; g < tenants.size(); g++) {
         tenants[g].since += taken[g];
         tenants[g].applied_total += taken[g];
@@ -236,6 +239,7 @@ int sgc_server::launch(const std::vector<sgc_edge_update>& batch, const std::vec
     updates_applied += batch.size();
     batches++;
     queued_updates -= batch.size();
+    apply_us += dt;
     return SGC_OK;
 }
 
@@ -317,10 +321,12 @@ int sgc_server::publish_due(bool flush, bool& work) {
     const bool plain = cfg.mode == SGC_SERVE_PLAIN;
     for (size_t k = 0; k < due.size(); k += plain ? 1 : due.size()) {
         const uint32_t cnt = plain ? 1u : (uint32_t)due.size();
+        const auto te = Clock::now();
         int r = sgc_pool_epoch(pool, due.data() + k, cnt, nullptr);
         if (r) return r;
         const auto tp = Clock::now();
         std::lock_guard<std::mutex> lk(mu);
+        epoch_us += us_between(te, tp);
         for (uint32_t j = 0; j < cnt; j++) {
             Tenant& T = tenants[due[k + j]];
             T.since = 0;
@@ -350,12 +356,21 @@ int sgc_server::turn() {
     bool flush = false, work = false;
     const auto t0 = Clock::now();
     int r = intake(flush, work);
+    const auto t1 = Clock::now();
+    double applied0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        applied0 = apply_us;
+    }
     if (r == SGC_OK) r = cfg.mode == SGC_SERVE_PLAIN ? apply_plain(work) : apply_batched(work);
+    const auto t2 = Clock::now();
     if (r == SGC_OK) r = publish_due(flush, work);
     if (r) return r;
     if (work) {
         std::lock_guard<std::mutex> lk(mu);
         busy_us += us_between(t0, Clock::now());
+        intake_us += us_between(t0, t1);
+        pack_us += us_between(t1, t2) - (apply_us - applied0);   // the launch call itself is apply_us
     }
     return work ? 1 : 0;
 }
@@ -574,6 +589,10 @@ int sgc_server_get_stats(sgc_server* s, sgc_server_stats* out) {
     out->queued_updates = s->queued_updates;
     out->span_us = s->any_submit && s->epoch_steps ? us_between(s->first_submit, s->last_publish) : 0;
     out->busy_us = s->busy_us;
+    out->intake_us = s->intake_us;
+    out->pack_us = s->pack_us;
+    out->apply_us = s->apply_us;
+    out->epoch_us = s->epoch_us;
     return SGC_OK;
 }
 

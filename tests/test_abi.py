@@ -116,3 +116,44 @@ def test_edge_intervals_give_the_live_set(graph_path):
         got = set(zip(eu[alive].tolist(), ev[alive].tolist()))
         lu, lv = O.live_edges_after(og, r["u"][:p], r["v"][:p], r["type"][:p])
         assert got == set(zip(lu.tolist(), lv.tolist())), p
+
+
+# Every ctypes structure of the binding against the header's own layout: a
+# program compiled from include/streamgc.h prints sizeof and the offset of
+# every field, and the binding must agree field for field.
+_STRUCTS = {"sgc_config": "Config", "sgc_epoch_stats": "EpochStats", "sgc_totals": "Totals",
+            "sgc_tuning": "Tuning", "sgc_pool_config": "PoolConfig", "sgc_graph_info": "GraphInfo",
+            "sgc_batch_config": "BatchConfig", "sgc_batch_stats": "BatchStats",
+            "sgc_server_config": "ServerConfig", "sgc_server_stats": "ServerStats",
+            "sgc_trace_item": "TraceItem", "sgc_replay_stats": "ReplayStats", "sgc_run_meta": "RunMeta"}
+
+
+def test_ctypes_structures_match_the_header(tmp_path):
+    import ctypes
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if not cxx:
+        pytest.skip("no g++")
+    lines = ["#include <cstdio>", "#include <cstddef>", '#include "streamgc.h"', "int main() {"]
+    for cname, pyname in _STRUCTS.items():
+        lines.append(f'  std::printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in getattr(_lib, pyname)._fields_:
+            lines.append(f'  std::printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([cxx, "-std=c++17", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    want = {}
+    for line in out:
+        if line:
+            c, f, v = line.split()
+            want[(c, f)] = int(v)
+    for cname, pyname in _STRUCTS.items():
+        cls = getattr(_lib, pyname)
+        assert ctypes.sizeof(cls) == want[(cname, "size")], cname
+        for f, _ in cls._fields_:
+            assert getattr(cls, f).offset == want[(cname, f)], (cname, f)

@@ -132,7 +132,13 @@ def test_parallel_batch_publishes_proper_snapshots(autotune, t1_sweeps, graph_pa
           f"ns/update {st['candidate_ns_per_update']}", flush=True)
     if autotune:
         assert st["candidates"] == 4 and 0 <= st["chosen"] < 4
-        assert all(v > 0 for v in st["candidate_ns_per_update"][:min(4, rounds // autotune)])
+        # a candidate is timed only on rounds that ran captured to the end
+        for n, v in zip(st["candidate_samples"], st["candidate_ns_per_update"]):
+            assert (n > 0) == (v > 0), st
+        if st["tune_rounds"] < rounds:   # tuning ended before the rounds did: every candidate has its samples
+            assert all(n >= autotune for n in st["candidate_samples"]), st
+        assert st["candidate_samples"][st["chosen"]] > 0
+        assert 0 < st["tune_ms"] <= st["ms"] and st["tune_updates"] <= st["updates"]
     for k, g in graphs.items():
         n_snap = pool.history_count(gid[k])
         epochs = -(-LENGTH[k] // EPOCH[k])

@@ -51,6 +51,8 @@ def test_c4_256_graphs_every_snapshot_validated(c4):
     st = pool.batch_run()
     pool.batch_release()
     assert st["rounds"] == rounds == max(epochs)
+    # the autotuner timed every candidate on a round that ran captured to the end
+    assert all(n >= 1 for n in st["candidate_samples"]) and all(v > 0 for v in st["candidate_ns_per_update"]), st
     clean = lossy = snaps = 0
     viol_lossy = lost_total = 0
     for (kind, g, L, _), gid, x, k, e in zip(spec, gids, churn, epochs, E):
