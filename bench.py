@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "c2": dict(name="Chung-Lu com-LiveJournal shape: 3,997,962 vertices, 34,681,189 edges, gamma 2.3, "
                     "max degree 15K, permuted (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
-               kind="chung_lu", n=3997962, m=34681189, gamma=2.3, max_degree=15000.0),
+               kind="chung_lu", n=3997962, m=34681189, gamma=2.3, max_degree=15000.0, stream=1000000),
     "c5": dict(name="served: 64 tenants (Delaunay 2^10-2^14, R-MAT s10-14 ef 8-40, Chung-Lu 2^12-2^15, "
                     "R-MAT s12 ef 48) + com-Orkut-shape Chung-Lu (3,072,441 vertices, 117M edges) resident in one "
                     "pool; churn (50% insert) batches of U[1,8192] updates to uniform tenants; E_g = 4,096",
@@ -58,9 +58,9 @@ WORKLOADS = {
                      "shared word", kind="microbench"),
     "c3": dict(name="R-MAT scale 20 (a,b,c = .57,.19,.19), edge factor 48 (seed 42); dense-growth stream inside the "
                     "90th-percentile k-core, 2 inserts per delete (seed 42); E=32,768; plain engine",
-               kind="rmat", scale=20, edge_factor=48.0),
+               kind="rmat", scale=20, edge_factor=48.0, stream=1000000),
     "c1": dict(name="Delaunay 2^17 uniform points (seed 42); churn 50% insert (seed 42); E=32,768; plain engine",
-               kind="delaunay", points=1 << 17),
+               kind="delaunay", points=1 << 17, stream=100000),
     "c4": dict(name="batched: 256 graphs (128 Delaunay with 2^U[10,16] points, 128 R-MAT scale U[10,16] edge "
                     "factor U[8,40], seeds 42+i) in one pool; per graph a churn of U[1K,64K] updates at U[30,70]% "
                     "insert, E_g = min(stream, 4096); rounds replayed as one captured hipGraph each",
@@ -141,30 +141,28 @@ def measured_traffic(workload, kernel):
     return (int(b), t["source"]) if b else (None, None)
 
 
+def scale_reference(workload):
+    """The oracle's colour counts on the same workload (tests/golden/scale_reference.json): the maximum
+    colour after each full epoch and after the tail's final repair, with exact histograms (this engine's
+    selection) and with the reference's own drifting histograms."""
+    path = os.path.join(ROOT, "tests", "golden", "scale_reference.json")
+    if not os.path.exists(path):
+        return None
+    t = json.load(open(path))
+    out = {}
+    for basis in ("exact", "reference"):
+        r = t.get(f"{workload}/{basis}")
+        if not r:
+            return None
+        out[basis] = {"stream": r["stream"], "epoch_max_color": r["epoch_max_color"],
+                      "final_max_color": r["final_max_color"]}
+    return out
+
+
 def tuning_knobs(args):
     """The sgc_tuning knobs given on the command line."""
     return {k: getattr(args, k) for k in ("long_row", "sweep_blocks", "serial_repair", "serial_conflicts",
                                           "fresh_repair") if getattr(args, k) is not None}
-
-
-def cpu_baseline(g, state, bench, E, batches, threads):
-    """The oracle (sequential updates, OpenMP sweeps) on the same batches from the same state."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    og = O.Csr(g.n, g.row_ptr, g.col_idx, g.degree)
-    cfg = O.default_config(flags=O.FLAG_REPAIR_EVERY_EPOCH)
-    eng = O.Engine(og, cfg, threads=threads)
-    words, col, bitmaps, hist = state
-    eng.set_col(col)
-    eng.set_words(words)
-    eng.set_bitmaps(bitmaps)
-    if hist.size:
-        eng.set_hist(hist)
-    sl = bench[:batches * E]
-    t0 = time.perf_counter()
-    eng.submit(sl["u"], sl["v"], sl["type"])
-    dt = time.perf_counter() - t0
-    return len(sl) / dt, dt
 
 
 def main():
@@ -173,7 +171,8 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-batches", type=int, default=2)
+    ap.add_argument("--updates", type=int, default=None,
+                    help="plain workloads: stream length (default: the config's, C1 100,000, C2 and C3 1,000,000)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--full-sweeps", action="store_true",
@@ -226,7 +225,6 @@ def main():
     if w["kind"] == "batched":
         return run_batched(S, args, w)
     E = 32768
-    K, W = args.steps, args.warmup
     t = time.perf_counter()
     g = make_graph(S, w)
     held = None
@@ -234,130 +232,134 @@ def main():
         g, held = S.holdout_stream(g, 10, 42)
     log(f"graph {g.n} vertices {g.num_edges // 2} edges max deg {int(g.degree.max())} ({time.perf_counter() - t:.1f}s)")
     t = time.perf_counter()
+    n_upd = args.updates or w["stream"]
     init = S.generate_init_stream(g)
-    bench = make_stream(S, w, g, (W + K) * E, args.stream_type, held)
-    log(f"streams: init {len(init)}, churn {len(bench)} ({time.perf_counter() - t:.1f}s)")
-
-    eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True,
-                                             full_sweeps=args.full_sweeps))
+    stream = make_stream(S, w, g, n_upd, args.stream_type, held)
+    n_upd = len(stream)
+    epochs = -(-n_upd // E)
+    iv = S.edge_intervals(g, stream)
+    pos = [min(E * (k + 1), n_upd) for k in range(epochs)]
+    log(f"streams: init {len(init)}, {n_upd} updates = {n_upd // E} full epochs + a {n_upd % E}-update tail "
+        f"({time.perf_counter() - t:.1f}s)")
     knobs = tuning_knobs(args)
-    if knobs:
-        eng.set_tuning(**knobs)
-    t = time.perf_counter()
-    ds_init = S.DeviceStream(init)
-    if args.init_batch and args.init_batch < E:
-        for k in range(0, len(init), E):
-            for j in range(k, min(k + E, len(init)), args.init_batch):
-                eng.apply_device(ds_init.at(j), min(args.init_batch, len(init) - j))
-            if k + E <= len(init):
-                eng.epoch()
-    else:
-        eng.process_stream(ds_init, max_epochs=0)
-    fin = eng.finish()
-    ds_init.close()
-    log(f"init phase: {time.perf_counter() - t:.1f}s, max colour {fin['max_color']}, t2_remaining {fin['t2_remaining']}")
 
-    eng.set_mode(repair_every_epoch=True)
-    eng.history_reserve(W + K)
-    ds = S.DeviceStream(bench)
-    for k in range(W):
-        eng.submit_device(ds.at(k * E), E)
+    def fresh_engine():
+        """The graph loaded, its init stream applied (untimed) and repaired, repair-every-epoch mode on."""
+        t = time.perf_counter()
+        eng = S.GPUPartition(g, S.StreamGCConfig(epoch_size=E, batch_size=E, exact_histogram=True,
+                                                 full_sweeps=args.full_sweeps))
+        if knobs:
+            eng.set_tuning(**knobs)
+        ds_init = S.DeviceStream(init)
+        if args.init_batch and args.init_batch < E:
+            for k in range(0, len(init), E):
+                for j in range(k, min(k + E, len(init)), args.init_batch):
+                    eng.apply_device(ds_init.at(j), min(args.init_batch, len(init) - j))
+                if k + E <= len(init):
+                    eng.epoch()
+        else:
+            eng.process_stream(ds_init, max_epochs=0)
+        fin = eng.finish()
+        ds_init.close()
+        eng.set_mode(repair_every_epoch=True)
+        eng.history_reserve(epochs)
+        log(f"init phase: {time.perf_counter() - t:.1f}s, max colour {fin['max_color']}, "
+            f"t2_remaining {fin['t2_remaining']}")
+        return eng
+
+    # Leg 1, host-driven: each epoch of E updates (resident in HBM) is one
+    # submit that applies it, repairs and publishes; the tail is applied and
+    # then finished (drain, repair, publish), as the reference ends a stream
+    # (A/src/streamgc.cu:1420-1426).  Gives the per-epoch update-to-snapshot
+    # latency, the per-kernel HIP-event times and the device counters.
+    eng = fresh_engine()
     state = None
     if not args.no_cpu_baseline:
         state = (eng.words(), eng.csr()[1], eng.bitmaps(), eng.hist())
+    ds = S.DeviceStream(stream)
     tot0 = eng.totals()
     eng.kernel_timing(True)
-    lat = []
-    epochs = []
+    lat, epoch_stats = [], []
     t0 = time.perf_counter()
-    for k in range(W, W + K):
+    for k in range(epochs):
         ts = time.perf_counter()
-        epochs += eng.submit_device(ds.at(k * E), E, max_epochs=1)
+        cnt = min(E, n_upd - k * E)
+        if cnt == E:
+            epoch_stats += eng.submit_device(ds.at(k * E), E, max_epochs=1)
+        else:
+            eng.apply_device(ds.at(k * E), cnt)
+            epoch_stats.append(eng.finish())
         lat.append(time.perf_counter() - ts)
-    elapsed = time.perf_counter() - t0
+    host_s = time.perf_counter() - t0
     ktimes = eng.kernel_times()
     eng.kernel_timing(False)
     tot1 = eng.totals()
-    value = K * E / elapsed
-    log(f"timed {K} steps: {elapsed * 1e3:.1f} ms, {value / 1e6:.2f} M updates/s")
-
-    # roofline of the dominant kernel
-    d = {k: tot1[k] - tot0[k] for k in tot1}
-    n = g.n
-    live_frac = d["live_slots"] / d["slots_compact"] if d["slots_compact"] else 0.25
-    per_kernel = {}
-    for name, (ms, launches) in ktimes.items():
-        b = kernel_bytes(name, launches, n, d, live_frac)
-        avg_ms = ms / max(launches, 1)
-        per_kernel[name] = dict(total_ms=round(ms, 3), launches=int(launches), avg_us=round(avg_ms * 1e3, 2),
-                                bytes_per_launch=int(b),
-                                gbs=round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 and b else None)
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["total_ms"])
-    pk = per_kernel[dom]
-    traffic, tsrc = measured_traffic(args.workload, dom)
-    roofline = dict(bound="hbm", kernel=dom, achieved=pk["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(pk["gbs"] / HBM_PEAK_GBS, 4) if pk["gbs"] else None, traffic=traffic,
-                    traffic_source=tsrc, bytes_per_launch=pk["bytes_per_launch"], avg_launch_us=pk["avg_us"])
-
-    parity = {"epochs_timed": len(epochs), "t2_remaining_max": max(e["t2_remaining"] for e in epochs),
-              "max_color_final": epochs[-1]["max_color"], "t1_iterations_mean":
-              round(float(np.mean([e["t1_iterations"] for e in epochs])), 2),
-              "tier2_runs": int(sum(e["t2_ran"] for e in epochs)), "grouped": int(d["grouped"]),
-              "overflow_events": int(d["overflow_events"])}
+    ds.close()
+    host_value = n_upd / host_s
+    log(f"host-driven: {n_upd} updates in {host_s * 1e3:.1f} ms = {host_value / 1e6:.2f} M updates/s")
+    parity = {"epochs": len(epoch_stats), "t2_remaining_max": max(e["t2_remaining"] for e in epoch_stats),
+              "max_color_per_epoch": [e["max_color"] for e in epoch_stats],
+              "t1_iterations_mean": round(float(np.mean([e["t1_iterations"] for e in epoch_stats])), 2),
+              "tier2_runs": int(sum(e["t2_ran"] for e in epoch_stats)),
+              "overflow_events": int(tot1["overflow_events"] - tot0["overflow_events"])}
+    ref = scale_reference(args.workload)
+    if ref:
+        parity["replay_exact_histograms"] = ref["exact"]
+        parity["replay_reference_semantics"] = ref["reference"]
     if not args.no_validate:
         t = time.perf_counter()
-        iv = S.edge_intervals(g, bench)
-        pos = [(k + 1) * E for k in range(eng.history_count())]
         bad = eng.validate_history(iv, pos)
         parity.update(snapshots_validated=len(pos), snapshots_improper=int(np.count_nonzero(bad)),
                       conflicting_edges_total=int(bad.sum()))
-        log(f"validated {len(pos)} snapshots against the live edge set: {int(bad.sum())} bad ({time.perf_counter() - t:.1f}s)")
+        log(f"host-driven: validated {len(pos)} snapshots against the live edge set: {int(bad.sum())} bad "
+            f"({time.perf_counter() - t:.1f}s)")
+    eng.close()
+
+    d = {k: tot1[k] - tot0[k] for k in tot1}
+    roofline, per_kernel = roofline_of(ktimes, d, g.n, args.workload)
 
     cpu = None
     if state is not None:
-        threads = min(16, os.cpu_count() or 1)
-        rate, dt = cpu_baseline(g, state, bench[W * E:], E, args.cpu_batches, threads)
-        cpu = dict(value=round(rate, 1), unit="updates/s", cores=threads, kind="port",
-                   sample=f"oracle engine, first {args.cpu_batches} measured batches ({args.cpu_batches * E} "
-                          f"updates incl. their epochs with Tier-1/Tier-2 and publish) from the same state; "
-                          f"{dt:.1f}s")
-        log(f"cpu baseline: {rate:.0f} updates/s on {threads} threads ({dt:.1f}s)")
+        cpu = cpu_baseline_both([(g, state, stream[:E], E)],
+                                f"oracle engine on the stream's first epoch ({E} updates, then compact, drain, "
+                                f"Tier-1, Tier-2 and publish) from the state after the init phase")
 
+    # Leg 2, the value: the same whole stream from a freshly loaded engine,
+    # its epochs replayed as captured hipGraph rounds (the last, partial one
+    # repaired and published like a finish).
     captured = None
+    value = host_value
     if not args.no_captured and not args.full_sweeps:
-        # the same number of epochs, from the state the timed steps left, as captured rounds
-        more = make_stream(S, w, g, (W + 2 * K) * E, args.stream_type, held)[(W + K) * E:]
-        eng.history_reserve(K)
-        rounds = eng.batch_prepare(more, use_graph=True)
+        eng = fresh_engine()
+        rounds = eng.batch_prepare(stream, use_graph=True)
         st = eng.batch_run()
         eng.batch_release()
-        iv = S.edge_intervals(g, make_stream(S, w, g, (W + 2 * K) * E, args.stream_type, held))
-        bad = eng.validate_history(iv, [(W + K) * E + E * (k + 1) for k in range(eng.history_count())])
-        captured = {"value": round(len(more) / st["ms"] * 1e3, 1), "ms_per_step": round(st["ms"] / rounds, 3),
-                    "rounds": rounds, "host_finished_rounds": st["host_rounds"],
-                    "snapshots_improper": int(np.count_nonzero(bad))}
-        log(f"captured rounds: {rounds} epochs in {st['ms']:.1f} ms = {captured['value'] / 1e6:.2f} M updates/s, "
-            f"{st['host_rounds']} finished by the host, {captured['snapshots_improper']} improper snapshots")
+        bad = eng.validate_history(iv, pos)
+        final = eng.validate()
+        captured = {"value": round(n_upd / st["ms"] * 1e3, 1), "ms": round(st["ms"], 3), "rounds": rounds,
+                    "host_finished_rounds": st["host_rounds"], "snapshots_validated": len(pos),
+                    "snapshots_improper": int(np.count_nonzero(bad)), "max_color_final": int(final[2])}
+        value = captured["value"]
+        eng.close()
+        log(f"captured rounds: {rounds} rounds, {n_upd} updates in {st['ms']:.1f} ms = {value / 1e6:.2f} M updates/s, "
+            f"{st['host_rounds']} finished by the host, {captured['snapshots_improper']} improper snapshots, "
+            f"final max colour {captured['max_color_final']}")
 
     lat_ms = np.array(lat) * 1e3
-    host_driven = {"value": round(value, 1), "ms_per_step": round(elapsed * 1e3 / K, 3)}
-    if captured:
-        value, ms_step = captured["value"], captured["ms_per_step"]
-    else:
-        ms_step = round(elapsed * 1e3 / K, 3)
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "updates/s", "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": round(value, 1), "unit": "updates/s", "n_gpus": 1, "steps": epochs, "warmup": 0,
+        "ms_per_step": round((n_upd / value) * 1e3 / epochs, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": w["name"] + (f"; {args.stream_type} stream instead" if args.stream_type else ""),
-                   "vertices": g.n, "edges": g.num_edges // 2, "epoch_size": E,
-                   "updates_timed": K * E,
+                   "vertices": g.n, "edges": g.num_edges // 2, "epoch_size": E, "updates_timed": n_upd,
+                   "span": "first update applied to the last snapshot published, the tail's final repair included",
                    "init": f"init stream ({len(init)} inserts, untimed) applied in launches of "
                            f"{min(args.init_batch or E, E)} updates, reference-order epochs every {E}, then repaired",
                    "maintenance": "full sweeps" if args.full_sweeps else "dirty worklists",
                    "epoch": "compact, drain, tier-1, tier-2 check (tier-2 when needed), publish",
-                   "execution": ("K epochs replayed as captured hipGraph rounds (value), after K host-driven "
-                                 "epochs (latency, per-kernel timing)") if captured else "host-driven epochs"},
+                   "execution": ("the stream's epochs replayed as captured hipGraph rounds (value); the same stream "
+                                 "host-driven from a fresh engine (latency, per-kernel timing)") if captured
+                   else "host-driven epochs"},
         "latency_ms": {"update_to_snapshot_p50": round(float(np.percentile(lat_ms, 50)), 3),
                        "update_to_snapshot_p99": round(float(np.percentile(lat_ms, 99)), 3)},
         "roofline": roofline,
@@ -369,7 +371,7 @@ def main():
                                                "recolor_cap_slots", "recolor_live", "grouped")},
         "tuning": knobs,
         "captured_rounds": captured,
-        "host_driven_steps": host_driven,
+        "host_driven": {"value": round(host_value, 1), "ms": round(host_s * 1e3, 3)},
     }
     print(json.dumps(out))
 

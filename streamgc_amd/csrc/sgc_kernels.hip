@@ -1194,67 +1194,149 @@ __global__ __launch_bounds__(256) void k_tier1_elem(const GraphDev* __restrict__
     block_add64(totals, T_SWEEP_SLOTS + 2, slots);
 }
 
-// Vertex order on one wavefront per graph (Gauss-Seidel, deterministic).
+// Whether tier1_vertex(g, v) would find v conflict-free (return 0, write
+// nothing): uncoloured with no neighbour in its row, or coloured with no
+// neighbour of its colour that has the higher priority.  Reads only.
+__device__ __forceinline__ bool tier1_clean(const GraphDev& g, uint32_t v) {
+    const uint16_t mine = color_of(bcast0(ld(&g.word[v])));
+    const uint32_t s = gp(g.row_ptr)[v], e = s + ld(&g.fill[v]);
+    uint32_t my_deg = 0;
+    if (mine != 0) my_deg = (g.flags & ABL_LIVE_DEGREE) ? live_row_count<false>(g, v) : gp(g.deg0)[v];
+    bool bad = false;
+    for (uint32_t base = s; base < e && !bad; base += ROW_UNROLL * WAVE) {
+        uint32_t nb[ROW_UNROLL], c[ROW_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+            const uint32_t j = base + u * WAVE + lane();
+            nb[u] = j < e ? ld(&g.col[j]) : INVALID;
+        }
+        bool hit = false;
+        if (mine == 0) {
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++) hit |= nb[u] != INVALID;
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++)
+                c[u] = (nb[u] != INVALID && nb[u] < g.n) ? color_of(ld(&g.word[nb[u]])) : 0;
+#pragma unroll
+            for (uint32_t u = 0; u < ROW_UNROLL; u++) {
+                if (c[u] == mine) {
+                    const uint32_t d = gp(g.deg0)[nb[u]];
+                    hit |= (my_deg < d) || (my_deg == d && v < nb[u]);  // pi(v) < pi(nb)
+                }
+            }
+        }
+        bad = ballot(hit) != 0;
+    }
+    return !bad;
+}
+
+// Vertex order per graph (Gauss-Seidel, deterministic), one block per graph.
 // With dirty tracking, a vertex is checked only if its check bit is set (or
-// it is sticky) when the sweep reaches it; the bit is read again after every
-// check, since a recolour marks later vertices of the same 64-vertex chunk.
-// A vertex whose bit is clear would be found conflict-free, so the sweep
-// changes exactly what the full sweep changes.
-__global__ __launch_bounds__(64) void k_tier1_serial(const GraphDev* __restrict__ graphs, uint32_t ng,
-                                                     const uint32_t* __restrict__ active, int fresh,
-                                                     uint64_t* totals) {
+// it is sticky) when the sweep reaches it; the bits are read again after
+// every vertex that changed (a recolour marks neighbours).  A vertex whose
+// bit is clear would be found conflict-free, so the sweep changes exactly
+// what the full sweep changes.
+// The block's SERIAL_WAVES wavefronts take the next SERIAL_WAVES marked
+// vertices of the current 1024-vertex chunk and find, at once, which of them
+// are conflict-free (tier1_clean, reads only).  Wavefront 0 then goes through
+// them in order: a conflict-free vertex changes nothing, so the next one's
+// verdict still holds; the first one that is not is checked and repaired
+// (tier1_vertex) in the state the sweep would have reached it in, and the
+// marks are read again from the vertex after it.  The result is the
+// one-vertex-at-a-time sweep's, in fewer round trips when most marked
+// vertices turn out clean.
+constexpr uint32_t SERIAL_WAVES = 4;
+__global__ __launch_bounds__(SERIAL_WAVES * WAVE) void k_tier1_serial(const GraphDev* __restrict__ graphs,
+                                                                     uint32_t ng,
+                                                                     const uint32_t* __restrict__ active,
+                                                                     int fresh, uint64_t* totals) {
     __shared__ uint32_t presence[PRESENCE_WORDS];
+    __shared__ uint32_t cand[SERIAL_WAVES], clean[SERIAL_WAVES], next_from;
+    const uint32_t wv = threadIdx.x / WAVE;
     uint64_t rows = 0, slots = 0;
     for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
         if (active && !active[gid]) continue;
         const GraphDev g = graphs[gid];
         uint32_t cnt = 0;
-        if (!g.dirty) {
-            for (uint32_t v = 0; v < g.n; v++) {
-                slots += gp(g.fill)[v];
-                cnt += tier1_vertex(g, v, presence, fresh);
+        if (!g.dirty) {   // every vertex, on wavefront 0 alone
+            if (wv == 0) {
+                for (uint32_t v = 0; v < g.n; v++) {
+                    slots += gp(g.fill)[v];
+                    cnt += tier1_vertex(g, v, presence, fresh);
+                }
+                rows += g.n;
             }
-            rows += g.n;
         } else {
-            // 1024 vertices per step: each lane reads the marks of 16 as two
-            // 8-byte words (mark arrays are padded to a multiple of 16), and
-            // again after each vertex it visits (a recolour marks neighbours)
+            // each lane of wavefront 0 reads the marks of 16 vertices of the
+            // chunk as two 8-byte words (mark arrays are padded to a multiple of 16)
             const size_t cst = (size_t)M_CHECK * g.dstride, sst = (size_t)M_STICKY * g.dstride;
             for (uint32_t base = 0; base < g.n; base += 16 * WAVE) {
                 const uint32_t v0 = base + 16 * lane();
                 const uint64_t* cw = (const uint64_t*)(g.dirty + cst + v0);
                 const uint64_t* sw = (const uint64_t*)(g.dirty + sst + v0);
-                const uint64_t s0 = v0 < g.n ? gp(sw)[0] : 0ull, s1 = v0 < g.n ? gp(sw)[1] : 0ull;
+                uint64_t s0 = 0, s1 = 0;
+                if (wv == 0 && v0 < g.n) {
+                    s0 = gp(sw)[0];
+                    s1 = gp(sw)[1];
+                }
                 const uint32_t live = v0 >= g.n ? 0u : g.n - v0 >= 16 ? 0xFFFFu : (1u << (g.n - v0)) - 1u;
-                uint32_t from = 0;   // chunk offset of the next vertex the sweep may visit
+                uint32_t from = 0;   // chunk offset of the next vertex the sweep may visit (block-uniform)
                 for (;;) {
-                    const uint64_t w0 = (v0 < g.n ? ld(&cw[0]) : 0ull) | s0;
-                    const uint64_t w1 = (v0 < g.n ? ld(&cw[1]) : 0ull) | s1;
-                    uint32_t m16 = 0;
+                    if (wv == 0) {   // the next marked vertices
+                        const uint64_t w0 = (v0 < g.n ? ld(&cw[0]) : 0ull) | s0;
+                        const uint64_t w1 = (v0 < g.n ? ld(&cw[1]) : 0ull) | s1;
+                        uint32_t m16 = 0;
 #pragma unroll
-                    for (uint32_t q = 0; q < 8; q++) {
-                        m16 |= (((w0 >> (8 * q)) & 0xFFull) != 0) << q;
-                        m16 |= (((w1 >> (8 * q)) & 0xFFull) != 0) << (q + 8);
+                        for (uint32_t q = 0; q < 8; q++) {
+                            m16 |= (((w0 >> (8 * q)) & 0xFFull) != 0) << q;
+                            m16 |= (((w1 >> (8 * q)) & 0xFFull) != 0) << (q + 8);
+                        }
+                        m16 &= live;
+                        const uint32_t at = 16 * lane();
+                        m16 &= from <= at ? 0xFFFFu : from >= at + 16 ? 0u : ~((1u << (from - at)) - 1u) & 0xFFFFu;
+                        for (uint32_t f = 0; f < SERIAL_WAVES; f++) {
+                            const uint64_t any = ballot(m16 != 0);
+                            uint32_t u = NONE;
+                            if (any) {
+                                const uint32_t k = (uint32_t)__builtin_ctzll(any);
+                                const uint32_t q = (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(m16, k));
+                                u = base + 16 * k + q;
+                                if (lane() == k) m16 &= m16 - 1u;   // q was its lowest bit
+                            }
+                            if (lane() == 0) cand[f] = u;
+                        }
                     }
-                    m16 &= live;
-                    const uint32_t at = 16 * lane();
-                    m16 &= from <= at ? 0xFFFFu : from >= at + 16 ? 0u : ~((1u << (from - at)) - 1u) & 0xFFFFu;
-                    const uint64_t any = ballot(m16 != 0);
-                    if (!any) break;
-                    const uint32_t k = (uint32_t)__builtin_ctzll(any);
-                    const uint32_t q = (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(m16, k));
-                    const uint32_t u = base + 16 * k + q;
-                    if (lane() == 0) gp(g.dirty)[cst + u] = 0;
-                    slots += gp(g.fill)[u];
-                    cnt += tier1_vertex(g, u, presence, fresh);
-                    rows++;
-                    from = 16 * k + q + 1;
+                    __syncthreads();
+                    const bool stop = cand[0] == NONE;
+                    const uint32_t mine = cand[wv];
+                    const bool ok = !stop && mine != NONE && tier1_clean(g, mine);
+                    if (lane() == 0) clean[wv] = ok ? 1u : 0u;
+                    __syncthreads();
+                    if (stop) break;
+                    if (wv == 0) {   // in vertex order, up to the first vertex that is not clean
+                        uint32_t last = base + from;
+                        for (uint32_t f = 0; f < SERIAL_WAVES; f++) {
+                            const uint32_t u = cand[f];
+                            if (u == NONE) break;
+                            if (lane() == 0) gp(g.dirty)[cst + u] = 0;
+                            slots += gp(g.fill)[u];
+                            rows++;
+                            last = u + 1;
+                            if (clean[f]) continue;
+                            cnt += tier1_vertex(g, u, presence, fresh);
+                            break;
+                        }
+                        if (lane() == 0) next_from = last - base;
+                    }
+                    __syncthreads();
+                    from = next_from;
                 }
             }
         }
-        if (lane() == 0 && cnt) add32(&g.ctr[C_CONFLICTS], cnt);
+        if (wv == 0 && lane() == 0 && cnt) add32(&g.ctr[C_CONFLICTS], cnt);
     }
-    if (lane() == 0 && totals && rows) {
+    if (wv == 0 && lane() == 0 && totals && rows) {
         tot_add(totals, T_SWEEP_ROWS + 4, rows);
         tot_add(totals, T_SWEEP_SLOTS + 4, slots);
     }
@@ -1778,7 +1860,7 @@ hipError_t launch_rebuild(const LaunchCtx& L, bool exact_hist) {
 hipError_t launch_tier1(const LaunchCtx& L, bool serial, int fresh) {
     if (serial) {
         const uint32_t grid = L.ng < 1024 ? L.ng : 1024;
-        k_tier1_serial<<<grid, 64, 0, L.stream>>>(L.graphs, L.ng, L.active, fresh, L.totals);
+        k_tier1_serial<<<grid, SERIAL_WAVES * WAVE, 0, L.stream>>>(L.graphs, L.ng, L.active, fresh, L.totals);
     } else {
         k_tier1_elem<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L), fresh,
                                                           L.totals);
