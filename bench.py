@@ -184,8 +184,10 @@ def main():
                     help="tuning: graphs of at most this many vertices run Tier-1 in vertex order")
     ap.add_argument("--serial-conflicts", type=int, default=None,
                     help="tuning: ... once two parallel sweeps have left at most this many conflicts")
+    ap.add_argument("--legs", type=int, default=2, help="profile summary: legs of the profiled run (1: --no-captured)")
     ap.add_argument("--profile-summary", default=None, metavar="DIR",
-                    help="summarise the rocprofv3 output under DIR (timed window = the last --steps steps) and exit")
+                    help="summarise the rocprofv3 output under DIR and exit (--steps: the epochs of a plain-workload "
+                         "run, whose two legs are then summarised apart)")
     ap.add_argument("--requests", type=int, default=2000, help="served workload: requests of each capacity phase")
     ap.add_argument("--point-seconds", type=float, default=1.0,
                     help="served workload: arrivals per sweep point span about this long (within the bounds below)")
@@ -383,56 +385,73 @@ def _short(name):
     return name.split("(")[0].replace("sgc::", "").replace("void ", "").strip()
 
 
-def _timed_window(rows, steps, which="captured"):
-    """Rows (dicts with Kernel_Name, ordered by dispatch) of the timed steps.
-    A default run has K host-driven steps, then K captured rounds: "captured"
-    is from the K-th last per-update launch to the end of the run, "host" the
-    K steps before it (the ones whose HIP-event timings give the roofline)."""
+def _windows(rows, steps, legs=2):
+    """The timed legs of a plain-workload bench run, by the per-update kernel's
+    dispatches: init (I), host-driven epochs (steps), then (legs = 2) init
+    again (I) and the captured rounds (steps).  host = from the leg's first
+    update launch to the validation that ends it (k_validate_history, else
+    the next leg); captured = from the captured leg's first update launch to
+    the end."""
     idx = [i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == "k_apply_elem"]
-    if which == "captured":
-        return rows[idx[-steps]:] if len(idx) >= steps else rows
-    return rows[idx[-2 * steps]:idx[-steps]] if len(idx) >= 2 * steps else rows
+    n_init = (len(idx) - legs * steps) // legs if steps and len(idx) >= legs * steps else None
+    if n_init is None or n_init < 0:
+        return {"all": rows}
+    if legs == 1:
+        a = idx[n_init]
+        b = next((i for i in range(a, len(rows)) if _short(rows[i]["Kernel_Name"]) == "k_validate_history"),
+                 len(rows))
+        return {"host": rows[a:b]}
+    a = idx[n_init]
+    b = next((i for i in range(a, len(rows)) if _short(rows[i]["Kernel_Name"]) == "k_validate_history"),
+             idx[n_init + steps] if n_init + steps < len(idx) else len(rows))
+    return {"host": rows[a:b], "captured": rows[idx[2 * n_init + steps]:]}
 
 
 def profile_summary(args):
-    """Summaries of the rocprofv3 output of a bench run: the kernel-trace
-    statistics over the timed steps, and per-dispatch FETCH_SIZE /
-    WRITE_SIZE (KiB) over the same window scaled by what the counters read
-    for a copy kernel of known size."""
+    """Summaries of the rocprofv3 output of a bench run under DIR: kernel-trace
+    statistics per kernel (over the legs of a plain-workload run when --steps
+    gives its epochs, else over the whole run), and the median per-dispatch
+    value of every collected counter per kernel, over the same window.
+    FETCH_SIZE and WRITE_SIZE (KiB) are scaled to bytes by what the same
+    counters read for a copy kernel of known size (a copy_* run under DIR)."""
     import csv
     out = {}
     d = args.profile_summary
     trace = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("kernel_trace.csv")]
-    if trace:
-        rows = sorted(csv.DictReader(open(trace[0])), key=lambda r: int(r["Start_Timestamp"]))
-        for which in ("host", "captured"):
-            win = _timed_window(rows, args.steps, which)
+    for path in trace:
+        rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+        tag = os.path.relpath(path, d)
+        res = {"dispatches": len(rows)}
+        for which, win in _windows(rows, args.steps, args.legs).items():
             per = {}
             for r in win:
-                k = _short(r["Kernel_Name"])
-                per.setdefault(k, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
-            out[f"{which}_window"] = {
-                k: {"calls": len(v), "total_us": round(sum(v), 2), "avg_us": round(sum(v) / len(v), 2),
-                    "min_us": round(min(v), 2), "max_us": round(max(v), 2)}
-                for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))}
+                per.setdefault(_short(r["Kernel_Name"]), []).append(
+                    (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+            res[which] = {k: {"calls": len(v), "total_us": round(sum(v), 2), "avg_us": round(sum(v) / len(v), 2),
+                              "median_us": round(float(np.median(v)), 2), "min_us": round(min(v), 2),
+                              "max_us": round(max(v), 2)}
+                          for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))}
             if win:
-                out[f"{which}_window_span_us"] = round((int(win[-1]["End_Timestamp"]) -
-                                                        int(win[0]["Start_Timestamp"])) / 1e3, 1)
-        out["dispatches_in_trace"] = len(rows)
-    for tag in ("FETCH_SIZE", "WRITE_SIZE"):
-        files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs
-                 if f.endswith("counter_collection.csv") and tag.lower().split("_")[0] in r.lower()]
-        for f in files:
-            rows = list(csv.DictReader(open(f)))
-            rows = [r for r in rows if r.get("Counter_Name") == tag]
-            rows.sort(key=lambda r: int(r["Dispatch_Id"]))
-            kind = "copy" if "copy" in f else "bench"
-            win = rows if kind == "copy" else _timed_window(rows, args.steps, "host")
+                res[f"{which}_span_us"] = round((int(win[-1]["End_Timestamp"]) - int(win[0]["Start_Timestamp"])) / 1e3, 1)
+        out[tag] = res
+    counters = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+    for path in counters:
+        rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
+        tag = os.path.relpath(path, d)
+        copy = "copy" in tag or "gather" in tag
+        by_counter = {}
+        for r in rows:
+            by_counter.setdefault(r["Counter_Name"], []).append(r)
+        res = {}
+        for cname, crow in by_counter.items():
+            wins = {"all": crow} if copy else _windows(crow, args.steps, args.legs)
+            win = wins.get("host", wins.get("all"))
             per = {}
             for r in win:
                 per.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
-            out.setdefault(kind, {})[tag] = {k: {"dispatches": len(v), "kib_avg": round(sum(v) / len(v), 1)}
-                                             for k, v in per.items()}
+            res[cname] = {k: {"dispatches": len(v), "median": round(float(np.median(v)), 1),
+                              "mean": round(float(np.mean(v)), 1)} for k, v in per.items()}
+        out[tag] = res
     print(json.dumps(out, indent=1))
 
 

@@ -27,6 +27,10 @@ constexpr uint32_t WAVE = 64;
 // Row scans issue ROW_UNROLL 64-slot loads before consuming any of them, so a
 // long row costs ROW_UNROLL times fewer memory round trips.
 constexpr uint32_t ROW_UNROLL = 4;
+// An insert's or delete's scan of its two rows loads ROW_UNROLL 64-slot
+// chunks of each per round trip, and SCAN_CHUNKS of the one row left once the
+// other is done: a hub row's long search takes SCAN_CHUNKS x 64 slots per trip.
+constexpr uint32_t SCAN_CHUNKS = 16;
 constexpr int MAX_CAS_RETRIES = 64;          // recolour CAS retries before leaving it to repair
 // Ablation flags carried by the descriptors (the values of SGC_ABL_* in streamgc.h).
 constexpr uint32_t ABL_FORCE_BITMAP = 1u << 5, ABL_NO_OVERFLOW = 1u << 7, ABL_LIVE_DEGREE = 1u << 8;
@@ -321,7 +325,7 @@ __device__ __forceinline__ void on_neighbor_recolor(const GraphDev& g, uint32_t 
 // overflowed edge's (sticky, and counted in C_ASYM).
 // Whether slots [from, end) hold nb (wave-uniform).
 __device__ __forceinline__ bool row_holds(const GraphDev& g, uint32_t nb, uint32_t from, uint32_t end) {
-    constexpr uint32_t CH = 2 * ROW_UNROLL;   // 64-slot chunks per round trip
+    constexpr uint32_t CH = SCAN_CHUNKS;   // 64-slot chunks per round trip
     bool dup = false;
     for (uint32_t base = from; base < end && !dup; base += CH * WAVE) {
         uint32_t val[CH];
@@ -431,8 +435,8 @@ __device__ __forceinline__ void mark_neighbors(const GraphDev& g, uint32_t v, ui
 // looks for the first slot that is free or already holds the neighbour
 // (A/src/kernels/local_update.cu:12-39), a delete for the first slot holding
 // it (A/src/kernels/local_update.cu:42-56), row 0 and row 1 each on its own.
-// A round trip loads 4 x 64 slots of each row still searching (8 x 64 when
-// one row is left).  The rows' matches are then claimed together, lane 0
+// A round trip loads ROW_UNROLL x 64 slots of each row still searching
+// (SCAN_CHUNKS x 64 when one row is left).  The rows' matches are then claimed together, lane 0
 // for row 0 and lane 1 for row 1: one CAS each, and for an insert the raise
 // of the row's fill to the claimed position (harmless when the CAS loses:
 // the slot's winner raised it as far).  A row whose CAS lost resumes its scan
@@ -449,7 +453,7 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
                                           uint32_t end1, uint32_t& pos0, uint32_t& pos1, uint32_t& pf0,
                                           uint32_t& pf1, bool& had0, bool& had1, uint32_t& free0,
                                           uint32_t& free1) {
-    constexpr uint32_t CH = 2 * ROW_UNROLL;   // 64-slot chunks per round trip
+    constexpr uint32_t CH = SCAN_CHUNKS;   // 64-slot chunks per round trip
     uint32_t b0 = s0, b1 = s1;
     pos0 = pos1 = pf0 = pf1 = 0;
     had0 = had1 = false;
@@ -457,13 +461,15 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
     run0 = run0 && b0 < end0;
     run1 = run1 && b1 < end1;
     while (run0 || run1) {
-        const uint32_t per0 = run0 ? (run1 ? ROW_UNROLL : CH) : 0;   // chunks of row 0; the rest are row 1's
+        // chunks [0, per0) are row 0's, [per0, per0 + per1) row 1's
+        const uint32_t per0 = run0 ? (run1 ? ROW_UNROLL : CH) : 0;
+        const uint32_t per1 = run1 ? (run0 ? ROW_UNROLL : CH) : 0;
         uint32_t val[CH];
 #pragma unroll
         for (uint32_t q = 0; q < CH; q++) {
             const bool r1 = q >= per0;
             const uint32_t j = (r1 ? b1 + (q - per0) * WAVE : b0 + q * WAVE) + lane();
-            val[q] = j < (r1 ? end1 : end0) ? ld(&g.col[j]) : 0u;
+            val[q] = q < per0 + per1 && j < (r1 ? end1 : end0) ? ld(&g.col[j]) : 0u;
         }
         uint32_t c0 = INVALID, c1 = INVALID;   // first match of each row in this trip
         bool eq0 = false, eq1 = false;         // it already holds the neighbour (insert)
@@ -472,7 +478,7 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
             const bool r1 = q >= per0;
             const uint32_t b = r1 ? b1 + (q - per0) * WAVE : b0 + q * WAVE;
             const uint32_t want = r1 ? nb1 : nb0;
-            const bool in = b + lane() < (r1 ? end1 : end0);
+            const bool in = q < per0 + per1 && b + lane() < (r1 ? end1 : end0);
             const uint64_t m = ballot(in && (val[q] == want || (INSERT && val[q] == INVALID)));
             if (!INSERT) {
                 const uint64_t mf = ballot(in && val[q] == INVALID);
@@ -498,7 +504,7 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
             run0 = b0 < end0;
         }
         if (run1 && c1 == INVALID) {
-            b1 += (CH - per0) * WAVE;
+            b1 += per1 * WAVE;
             run1 = b1 < end1;
         }
         if (INSERT && c0 != INVALID && eq0) {   // the neighbour is there already
