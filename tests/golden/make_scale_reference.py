@@ -3,7 +3,8 @@
 The oracle replays the same seeded workload the benchmark runs -- the init
 stream in the reference's epoch order, then the churn stream with a repair and
 publish every epoch -- and records the maximum colour (the reference's colour
-count, A/src/streamgc.cu:549-553) after the init repair and after every epoch.
+count, A/src/streamgc.cu:549-553) after the init repair, after every full
+epoch, and after the final repair of the stream's tail (final_max_color).
 The GPU engine's parallel results are compared against these numbers.
 
 Usage: python tests/golden/make_scale_reference.py c1 [c2 c3 ...]   (CPU; c2 and c3 take minutes)
@@ -28,10 +29,10 @@ E = 32768
 def graph(name):
     if name == "c1":
         return S.gen_delaunay(1 << 17, seed=42), 100000
-    if name == "c2":
-        return S.gen_chung_lu(3997962, 34681189, 2.3, 15000.0, seed=42, permute=True), 31 * E
+    if name == "c2":   # 1,000,000 updates: 30 full epochs and a 16,960-update tail
+        return S.gen_chung_lu(3997962, 34681189, 2.3, 15000.0, seed=42, permute=True), 1000000
     if name == "c3":
-        return S.gen_rmat(20, 48.0, seed=42), 31 * E
+        return S.gen_rmat(20, 48.0, seed=42), 1000000
     raise SystemExit(f"unknown workload {name}")
 
 
