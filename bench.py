@@ -184,10 +184,8 @@ def main():
                     help="tuning: graphs of at most this many vertices run Tier-1 in vertex order")
     ap.add_argument("--serial-conflicts", type=int, default=None,
                     help="tuning: ... once two parallel sweeps have left at most this many conflicts")
-    ap.add_argument("--legs", type=int, default=2, help="profile summary: legs of the profiled run (1: --no-captured)")
     ap.add_argument("--profile-summary", default=None, metavar="DIR",
-                    help="summarise the rocprofv3 output under DIR and exit (--steps: the epochs of a plain-workload "
-                         "run, whose two legs are then summarised apart)")
+                    help="summarise the rocprofv3 output under DIR and exit (the legs of a plain-workload run apart)")
     ap.add_argument("--requests", type=int, default=2000, help="served workload: requests of each capacity phase")
     ap.add_argument("--point-seconds", type=float, default=1.0,
                     help="served workload: arrivals per sweep point span about this long (within the bounds below)")
@@ -282,6 +280,7 @@ def main():
     tot0 = eng.totals()
     eng.kernel_timing(True)
     lat, epoch_stats = [], []
+    leg_marker(eng)
     t0 = time.perf_counter()
     for k in range(epochs):
         ts = time.perf_counter()
@@ -293,6 +292,7 @@ def main():
             epoch_stats.append(eng.finish())
         lat.append(time.perf_counter() - ts)
     host_s = time.perf_counter() - t0
+    leg_marker(eng)
     ktimes = eng.kernel_times()
     eng.kernel_timing(False)
     tot1 = eng.totals()
@@ -334,7 +334,9 @@ def main():
     if not args.no_captured and not args.full_sweeps:
         eng = fresh_engine()
         rounds = eng.batch_prepare(stream, use_graph=True)
+        leg_marker(eng)
         st = eng.batch_run()
+        leg_marker(eng)
         eng.batch_release()
         bad = eng.validate_history(iv, pos)
         final = eng.validate()
@@ -382,29 +384,28 @@ def main():
 
 # ---------------------------------------------------------------- profile summaries
 def _short(name):
+    name = name.replace("(anonymous namespace)::", "")
     return name.split("(")[0].replace("sgc::", "").replace("void ", "").strip()
 
 
-def _windows(rows, steps, legs=2):
-    """The timed legs of a plain-workload bench run, by the per-update kernel's
-    dispatches: init (I), host-driven epochs (steps), then (legs = 2) init
-    again (I) and the captured rounds (steps).  host = from the leg's first
-    update launch to the validation that ends it (k_validate_history, else
-    the next leg); captured = from the captured leg's first update launch to
-    the end."""
-    idx = [i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == "k_apply_elem"]
-    n_init = (len(idx) - legs * steps) // legs if steps and len(idx) >= legs * steps else None
-    if n_init is None or n_init < 0:
+def _windows(rows):
+    """The timed legs of a bench run: the bench brackets each leg with a
+    one-vertex query (k_gather16, which nothing else launches before the served
+    sweep's own queries), so the first two markers bound leg 1 (plain
+    workloads: host-driven; served: plain capacity) and the next two leg 2
+    (captured rounds; served capacity); without markers, the whole run."""
+    marks = [i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == "k_gather16"]
+    if len(marks) < 2:
         return {"all": rows}
-    if legs == 1:
-        a = idx[n_init]
-        b = next((i for i in range(a, len(rows)) if _short(rows[i]["Kernel_Name"]) == "k_validate_history"),
-                 len(rows))
-        return {"host": rows[a:b]}
-    a = idx[n_init]
-    b = next((i for i in range(a, len(rows)) if _short(rows[i]["Kernel_Name"]) == "k_validate_history"),
-             idx[n_init + steps] if n_init + steps < len(idx) else len(rows))
-    return {"host": rows[a:b], "captured": rows[idx[2 * n_init + steps]:]}
+    out = {"leg1": rows[marks[0] + 1:marks[1]]}
+    if len(marks) >= 4:
+        out["leg2"] = rows[marks[2] + 1:marks[3]]
+    return out
+
+
+def leg_marker(eng):
+    """A one-vertex query: the marker that bounds a timed leg in a kernel trace."""
+    eng.query(np.zeros(1, np.uint32))
 
 
 def profile_summary(args):
@@ -422,7 +423,7 @@ def profile_summary(args):
         rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
         tag = os.path.relpath(path, d)
         res = {"dispatches": len(rows)}
-        for which, win in _windows(rows, args.steps, args.legs).items():
+        for which, win in _windows(rows).items():
             per = {}
             for r in win:
                 per.setdefault(_short(r["Kernel_Name"]), []).append(
@@ -444,13 +445,14 @@ def profile_summary(args):
             by_counter.setdefault(r["Counter_Name"], []).append(r)
         res = {}
         for cname, crow in by_counter.items():
-            wins = {"all": crow} if copy else _windows(crow, args.steps, args.legs)
-            win = wins.get("host", wins.get("all"))
-            per = {}
-            for r in win:
-                per.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
-            res[cname] = {k: {"dispatches": len(v), "median": round(float(np.median(v)), 1),
-                              "mean": round(float(np.mean(v)), 1)} for k, v in per.items()}
+            wins = {"all": crow} if copy else _windows(crow)
+            for which, win in wins.items():
+                per = {}
+                for r in win:
+                    per.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+                res.setdefault(which, {})[cname] = {
+                    k: {"dispatches": len(v), "median": round(float(np.median(v)), 1),
+                        "mean": round(float(np.mean(v)), 1)} for k, v in per.items()}
         out[tag] = res
     print(json.dumps(out, indent=1))
 
@@ -800,11 +802,13 @@ def run_served(S, args, w):
         srv = new_server(mode)
         tot0 = pool.totals()
         pool.kernel_timing(True)
+        pool.query(gids[0], np.zeros(1, np.uint32))   # marks the phase in a kernel trace
         t0 = time.perf_counter()
         for k, recs in reqs:
             srv.submit(gids[k], recs)
         srv.flush()
         dt = time.perf_counter() - t0
+        pool.query(gids[0], np.zeros(1, np.uint32))
         st = srv.stats()
         kt = pool.kernel_times()
         pool.kernel_timing(False)

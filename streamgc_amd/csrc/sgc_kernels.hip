@@ -133,12 +133,16 @@ __device__ __forceinline__ void list_tile(const GraphDev& g, uint32_t kind, uint
                                           uint32_t words, ListTile& out) {
     SGC_G uint32_t* mk = (SGC_G uint32_t*)(gp(g.dirty) + (size_t)kind * g.dstride);
     const SGC_G uint32_t* sk = (const SGC_G uint32_t*)(gp(g.dirty) + (size_t)M_STICKY * g.dstride);
+    // Sticky marks come only with an event counted in C_ASYM (overflow,
+    // duplicate): a graph that never had one has none, and its sticky array
+    // is not read.
+    const bool any_sticky = ld(&g.ctr[C_ASYM]) != 0;
     uint32_t m[LIST_WORDS], h[LIST_WORDS];
 #pragma unroll
     for (uint32_t k = 0; k < LIST_WORDS; k++) {
         const uint32_t w = tile * (BLOCK * LIST_WORDS) + k * BLOCK + threadIdx.x;
         m[k] = w < words ? mk[w] : 0u;
-        h[k] = w < words ? sk[w] : 0u;
+        h[k] = any_sticky && w < words ? sk[w] : 0u;
     }
 #pragma unroll
     for (uint32_t k = 0; k < LIST_WORDS; k++) {
