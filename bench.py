@@ -392,20 +392,22 @@ def _windows(rows):
     """The timed legs of a bench run: the bench brackets each leg with a
     one-vertex query (k_gather16, which nothing else launches before the served
     sweep's own queries), so the first two markers bound leg 1 (plain
-    workloads: host-driven; served: plain capacity) and the next two leg 2
-    (captured rounds; served capacity); without markers, the whole run."""
+    workloads: host-driven; served: plain capacity; batched: one graph at a
+    time), the next two leg 2 (captured rounds; served capacity; batched:
+    host-driven rounds) and the next two leg 3 (batched: captured rounds);
+    without markers, the whole run."""
     marks = [i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == "k_gather16"]
     if len(marks) < 2:
         return {"all": rows}
-    out = {"leg1": rows[marks[0] + 1:marks[1]]}
-    if len(marks) >= 4:
-        out["leg2"] = rows[marks[2] + 1:marks[3]]
-    return out
+    return {f"leg{j // 2 + 1}": rows[marks[j] + 1:marks[j + 1]] for j in range(0, len(marks) - 1, 2)}
 
 
-def leg_marker(eng):
+def leg_marker(eng, gid=None):
     """A one-vertex query: the marker that bounds a timed leg in a kernel trace."""
-    eng.query(np.zeros(1, np.uint32))
+    if gid is None:
+        eng.query(np.zeros(1, np.uint32))
+    else:
+        eng.query(gid, np.zeros(1, np.uint32))
 
 
 def profile_summary(args):
@@ -511,11 +513,13 @@ def run_batched(S, args, w):
     # 1. one graph at a time
     pool, gids = fresh_pool()
     c0 = counters(pool, gids)
+    leg_marker(pool, gids[0])
     t0 = time.perf_counter()
     for gid, x in zip(gids, churn):
         pool.submit(S.tag_graph(x, gid), max_stats=0)
         pool.finish([gid])
     alone_s = time.perf_counter() - t0
+    leg_marker(pool, gids[0])
     legs["one_graph_at_a_time"] = n_upd / alone_s
     stats["one_graph_at_a_time"] = finish_leg(pool, gids, c0)
     log(f"one graph at a time: {alone_s:.2f}s = {legs['one_graph_at_a_time'] / 1e6:.2f} M updates/s")
@@ -526,7 +530,9 @@ def run_batched(S, args, w):
     mixed = np.concatenate([S.tag_graph(x, gid) for x, gid in zip(churn, gids)])
     rounds = pool.batch_prepare(mixed, use_graph=False)
     pool.kernel_timing(True)
+    leg_marker(pool, gids[0])
     st_host = pool.batch_run()
+    leg_marker(pool, gids[0])
     host_k = pool.kernel_times()
     pool.kernel_timing(False)
     legs["host_driven_rounds"] = n_upd / st_host["ms"] * 1e3
@@ -541,7 +547,9 @@ def run_batched(S, args, w):
     for gid, k in zip(gids, epochs):
         pool.history_reserve(gid, k)
     rounds = pool.batch_prepare(mixed, use_graph=True, autotune_rounds=args.autotune_rounds)
+    leg_marker(pool, gids[0])
     st = pool.batch_run()
+    leg_marker(pool, gids[0])
     value = n_upd / (st["ms"] / 1e3)
     legs["captured_rounds"] = value
     stats["captured_rounds"] = finish_leg(pool, gids, c0)

@@ -217,6 +217,8 @@ struct BatchRun {
     uint16_t** d_snapbase = nullptr;
     HistDev* d_hist = nullptr;
     uint32_t* h_ctl = nullptr;                 // pinned copy of the control block
+    uint32_t* h_rd = nullptr;                  // pinned: the control block after each of two chunks in flight
+    hipEvent_t rd_ev[2] = {nullptr, nullptr};  // each chunk's reading has arrived
     std::vector<sgc_tuning> cands;
     std::vector<hipGraphExec_t> execs;
 
@@ -228,6 +230,9 @@ struct BatchRun {
                         (void*)d_inround, (void*)d_state, (void*)d_rg, (void*)d_snapbase, (void*)d_hist})
             if (p) (void)hipFree(p);
         if (h_ctl) (void)hipHostFree(h_ctl);
+        if (h_rd) (void)hipHostFree(h_rd);
+        for (hipEvent_t x : rd_ev)
+            if (x) (void)hipEventDestroy(x);
         *this = BatchRun();
     }
 };
@@ -1303,6 +1308,9 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_snapbase, (size_t)ng * sizeof(uint16_t*));
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_hist, (size_t)ng * sizeof(HistDev));
     if (e == hipSuccess) e = hipHostMalloc((void**)&B->h_ctl, RC_WORDS * 4);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&B->h_rd, 2 * RC_WORDS * 4);
+    for (hipEvent_t& x : B->rd_ev)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
     auto up = [&](void* d, const void* h, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream);
     };
@@ -1560,6 +1568,9 @@ int sgc_pool::batch_run_host(BatchRun& B) {
     return SGC_OK;
 }
 
+// Rounds per replay chunk once the autotuner has chosen (two chunks in flight).
+constexpr uint32_t ROUND_CHUNK = 2;
+
 int sgc_pool::batch_run(sgc_batch_stats* out) {
     if (!batch) return fail(SGC_ERR_INVALID, "batch run: nothing prepared");
     BatchRun& B = *batch;
@@ -1606,22 +1617,69 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
                     if (cand_n[k] < need) who.push_back(k);
                 std::stable_sort(who.begin(), who.end(), [&](uint32_t a, uint32_t b) { return cand_n[a] < cand_n[b]; });
             }
-            const uint32_t end = tuned ? B.rounds : std::min<uint32_t>(B.rounds, r + (uint32_t)who.size());
-            for (uint32_t k = r; k < end && e == hipSuccess; k++) {
-                if (!tuned) e = hipEventRecord(ev[2 * (k - r)], stream);
-                if (e == hipSuccess) e = hipGraphLaunch(B.execs[tuned ? best : who[k - r]], stream);
-                if (e == hipSuccess && !tuned) e = hipEventRecord(ev[2 * (k - r) + 1], stream);
-            }
-            if (e == hipSuccess) e = hipMemcpyAsync(B.h_ctl, B.d_ctl, RC_WORDS * 4, hipMemcpyDeviceToHost, stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) break;
-            // the replays either all ran or stopped at one round in [r, end);
-            // any other reading ends the run (each accepted one moves r on)
             RoundReading rd;
-            if (!read_rounds(r, end, B.rounds, B.h_ctl[RC_ROUND], B.h_ctl[RC_STOP], B.h_ctl[RC_STOP_ROUND], &rd)) {
-                rc = fail(SGC_ERR_STATE, "batch run: inconsistent round control after replays of rounds " +
-                                             std::to_string(r) + ".." + std::to_string(end));
-                break;
+            uint32_t end = r;
+            if (!tuned) {   // one round per candidate still short of samples, each between events
+                end = std::min<uint32_t>(B.rounds, r + (uint32_t)who.size());
+                for (uint32_t k = r; k < end && e == hipSuccess; k++) {
+                    e = hipEventRecord(ev[2 * (k - r)], stream);
+                    if (e == hipSuccess) e = hipGraphLaunch(B.execs[who[k - r]], stream);
+                    if (e == hipSuccess) e = hipEventRecord(ev[2 * (k - r) + 1], stream);
+                }
+                if (e == hipSuccess) e = hipMemcpyAsync(B.h_ctl, B.d_ctl, RC_WORDS * 4, hipMemcpyDeviceToHost, stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(stream);
+                if (e != hipSuccess) break;
+                // the replays either all ran or stopped at one round in [r, end);
+                // any other reading ends the run (each accepted one moves r on)
+                if (!read_rounds(r, end, B.rounds, B.h_ctl[RC_ROUND], B.h_ctl[RC_STOP], B.h_ctl[RC_STOP_ROUND], &rd)) {
+                    rc = fail(SGC_ERR_STATE, "batch run: inconsistent round control after replays of rounds " +
+                                                 std::to_string(r) + ".." + std::to_string(end));
+                    break;
+                }
+            } else {
+                // The chosen body in chunks of ROUND_CHUNK rounds, two chunks in
+                // flight: each chunk's control block comes back while the next
+                // one runs, so the GPU never waits for the host, and a round
+                // that stops costs at most the rest of its chunk and the next
+                // chunk as replays that find the stop and end at once.
+                uint32_t c_end[2] = {0, 0}, next = r;
+                auto chunk = [&](int slot) {
+                    c_end[slot] = std::min<uint32_t>(B.rounds, next + ROUND_CHUNK);
+                    for (uint32_t k = next; k < c_end[slot] && e == hipSuccess; k++)
+                        e = hipGraphLaunch(B.execs[best], stream);
+                    if (e == hipSuccess)
+                        e = hipMemcpyAsync(B.h_rd + slot * RC_WORDS, B.d_ctl, RC_WORDS * 4, hipMemcpyDeviceToHost,
+                                           stream);
+                    if (e == hipSuccess) e = hipEventRecord(B.rd_ev[slot], stream);
+                    next = c_end[slot];
+                };
+                chunk(0);
+                if (next < B.rounds) chunk(1);
+                int slot = 0;
+                bool ok = true;
+                while (e == hipSuccess) {
+                    e = hipEventSynchronize(B.rd_ev[slot]);
+                    if (e != hipSuccess) break;
+                    const uint32_t* c = B.h_rd + slot * RC_WORDS;
+                    end = c_end[slot];
+                    ok = read_rounds(r, end, B.rounds, c[RC_ROUND], c[RC_STOP], c[RC_STOP_ROUND], &rd);
+                    if (!ok || rd.stopped || end >= B.rounds) {
+                        // the chunk queued behind a stop finds it and ends; the host
+                        // round below reads the control block as the stop left it
+                        memcpy(B.h_ctl, c, RC_WORDS * 4);
+                        e = hipStreamSynchronize(stream);
+                        break;
+                    }
+                    r = end;
+                    if (next < B.rounds) chunk(slot);   // this slot's reading is consumed
+                    slot ^= 1;
+                }
+                if (e != hipSuccess) break;
+                if (!ok) {
+                    rc = fail(SGC_ERR_STATE, "batch run: inconsistent round control after replays of rounds " +
+                                                 std::to_string(r) + ".." + std::to_string(end));
+                    break;
+                }
             }
             const bool stopped = rd.stopped;
             const uint32_t done = rd.done;
@@ -1645,6 +1703,7 @@ int sgc_pool::batch_run(sgc_batch_stats* out) {
                     if (more_t2) B.lean = false;
                     if (more_t1) B.sweeps = std::min(2 * B.sweeps, B.sweeps_max);
                     for (uint32_t k = 0; k < C && rc == SGC_OK; k++) {
+                        if (tuned && k != best) continue;   // only the chosen body replays from here on
                         hipGraphExec_t x = nullptr;
                         rc = batch_capture(B.cands[k], &x);
                         if (rc == SGC_OK) {
