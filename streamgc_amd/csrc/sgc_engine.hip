@@ -211,6 +211,7 @@ struct BatchRun {
     std::vector<std::vector<uint32_t>> gids;   // graphs of each round
     sgc_edge_update* d_stream = nullptr;
     uint32_t *d_tab = nullptr, *d_seg = nullptr, *d_ctl = nullptr, *d_fl = nullptr, *d_pub = nullptr;
+    uint32_t* d_space = nullptr;               // vertex space of the graphs running device Tier-2 (dspace_words)
     uint8_t* d_inround = nullptr;
     RoundState* d_state = nullptr;
     RoundGraph* d_rg = nullptr;
@@ -227,7 +228,8 @@ struct BatchRun {
             if (x) (void)hipGraphExecDestroy(x);
         execs.clear();
         for (void* p : {(void*)d_stream, (void*)d_tab, (void*)d_seg, (void*)d_ctl, (void*)d_fl, (void*)d_pub,
-                        (void*)d_inround, (void*)d_state, (void*)d_rg, (void*)d_snapbase, (void*)d_hist})
+                        (void*)d_space, (void*)d_inround, (void*)d_state, (void*)d_rg, (void*)d_snapbase,
+                        (void*)d_hist})
             if (p) (void)hipFree(p);
         if (h_ctl) (void)hipHostFree(h_ctl);
         if (h_rd) (void)hipHostFree(h_rd);
@@ -1303,6 +1305,7 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_ctl, RC_WORDS * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_fl, (size_t)F_KINDS * ng * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_pub, (size_t)ng * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_space, (size_t)dspace_words(ng) * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_state, (size_t)ng * sizeof(RoundState));
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_rg, (size_t)ng * sizeof(RoundGraph));
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_snapbase, (size_t)ng * sizeof(uint16_t*));
@@ -1424,19 +1427,21 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
     if (e == hipSuccess && !B.lean) e = launch_build_list(Lt, M_T2, d_list, list_cap, d_list_count);
     if (e == hipSuccess && !B.lean) e = launch_t2_check(Lt);
     if (B.t2_passes && !B.lean) {   // Tier-2 on the device for the graphs whose check found bad slots
-        LaunchCtx L2 = base;
+        LaunchCtx L2 = base;   // the graphs with bad slots, laid out by k_t2_control
         L2.active = F(F_T2RUN);
         L2.gate = B.d_ctl + RC_T2_ANY;
-        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 0, 0);
+        L2.dspace = B.d_space;
+        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 0, 0, B.d_space);
         if (e == hipSuccess) e = launch_t2_begin(L2, d_rev, d_rev_count, rev_cap);
-        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 1, 0);
+        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 1, 0, B.d_space);
         if (e == hipSuccess) e = launch_t2_pass(L2, d_rev, d_rev_count, rev_cap, true);
         for (uint32_t p = 0; p < B.t2_passes && e == hipSuccess; p++) {
-            e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 2, p);
+            e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 2, p, B.d_space);
             if (e == hipSuccess) e = launch_t2_pass(L2, d_rev, d_rev_count, rev_cap, false);
-            if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 3, p);
+            if (e == hipSuccess)
+                e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 3, p, B.d_space);
         }
-        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 4, 0);
+        if (e == hipSuccess) e = launch_t2_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_rg, 4, 0, B.d_space);
         LaunchCtx Lc = base;   // the remaining bad slots, as the host's check after a repair counts them
         Lc.active = F(F_T2RAN);
         Lc.gate = B.d_ctl + RC_T2_RAN;

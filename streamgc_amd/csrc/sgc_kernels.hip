@@ -39,6 +39,12 @@ struct Space {
     }
 };
 
+// A vertex space written on the device (a captured round's control kernel
+// lays out the graphs still taking part; see dspace_words).
+__device__ __forceinline__ Space device_space(const uint32_t* dspace, uint32_t ng) {
+    return Space{dspace + 4, dspace + 4 + ng + 1, dspace[0], dspace[1]};
+}
+
 // ---------------------------------------------------------------- sweep sets
 // A maintenance sweep visits either every virtual vertex (list == null, one
 // wavefront per row) or the worklist a k_build_list launch left on the device.
@@ -1400,8 +1406,10 @@ __global__ __launch_bounds__(256) void k_t2_clear(const GraphDev* __restrict__ g
 __global__ __launch_bounds__(256) void k_t2_rev(const GraphDev* __restrict__ graphs, Space sp,
                                                 const uint32_t* __restrict__ active, RevPair* __restrict__ rev,
                                                 uint32_t* __restrict__ rev_count, uint32_t cap,
+                                                const uint32_t* __restrict__ dspace, uint32_t ng,
                                                 const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0) return;
+    if (dspace) sp = device_space(dspace, ng);
     for (uint32_t i = wave_id(); i < sp.total; i += wave_count()) {
         uint32_t gid, u;
         sp.locate(i, gid, u);
@@ -1568,8 +1576,11 @@ __global__ __launch_bounds__(T2_BLOCK) void k_t2_serial(const GraphDev* __restri
 // The next pass's list: the higher-id endpoint of every same-coloured slot
 // pair of every row.  One wavefront per row; C_T2TODO becomes 1 if any.
 __global__ __launch_bounds__(256) void k_t2_flags(const GraphDev* __restrict__ graphs, Space sp,
-                                                  const uint32_t* __restrict__ active, const uint32_t* __restrict__ gate) {
+                                                  const uint32_t* __restrict__ active,
+                                                  const uint32_t* __restrict__ dspace, uint32_t ng,
+                                                  const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0) return;
+    if (dspace) sp = device_space(dspace, ng);
     for (uint32_t i = wave_id(); i < sp.total; i += wave_count()) {
         uint32_t gid, x;
         sp.locate(i, gid, x);
@@ -1885,14 +1896,15 @@ hipError_t launch_t2_begin(const LaunchCtx& L, void* rev, uint32_t* rev_count, u
     k_zero_list_count<<<1, 1, 0, L.stream>>>(rev_count);   // [0] is the pair count
     k_t2_clear<<<grid_for_waves(L.V, 4 * WAVE, 4096), 256, 0, L.stream>>>(L.graphs, space(L), L.active, L.gate);
     k_t2_rev<<<grid_for_waves(L.V, 4, L.max_blocks), 256, 0, L.stream>>>(L.graphs, space(L), L.active,
-                                                                         (RevPair*)rev, rev_count, cap, L.gate);
+                                                                         (RevPair*)rev, rev_count, cap, L.dspace,
+                                                                         L.ng, L.gate);
     return hipGetLastError();
 }
 hipError_t launch_t2_pass(const LaunchCtx& L, const void* rev, const uint32_t* rev_count, uint32_t cap,
                           bool uncoloured) {
     if (!uncoloured)
         k_t2_flags<<<grid_for_waves(L.V, 4, L.max_blocks), 256, 0, L.stream>>>(L.graphs, space(L), L.active,
-                                                                               L.gate);
+                                                                               L.dspace, L.ng, L.gate);
     k_t2_serial<<<L.ng < 1024 ? L.ng : 1024, T2_BLOCK, 0, L.stream>>>(L.graphs, L.ng, L.active, (const RevPair*)rev,
                                                                       rev_count, cap, uncoloured ? 1 : 0, L.gate);
     return hipGetLastError();

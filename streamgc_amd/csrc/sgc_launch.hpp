@@ -33,7 +33,11 @@ struct LaunchCtx {
     const uint32_t* dyn = nullptr;     // [offset, count, segment offset] of the batch in the resident stream;
                                        // the launchers' count then only sizes the grid
     const uint32_t* gate = nullptr;    // worklist builds do nothing while *gate == 0
+    const uint32_t* dspace = nullptr;  // device vertex space of the graphs taking part ([ns, total, -, -, base[ng + 1],
+                                       // gid[ng]], written by a control kernel; replaces sbase/sgid/ns/V)
 };
+// Words of a device vertex space over up to ng graphs.
+inline uint32_t dspace_words(uint32_t ng) { return 4 + 2 * ng + 1; }
 
 // Per-edge grouping of a parallel batch: hash table (size: power of two >=
 // 2 x batch) of edge keys with list heads, a next link per update, and the
@@ -134,8 +138,10 @@ hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs
 // run it), 1 after the reverse-only list (a graph whose list overflowed stops
 // the round), 2 before pass `pass`, 3 after it, 4 at the end (graphs still
 // running stop the round; the graphs that ran are checked again).
+// Steps 0, 1 and 3 also write into dspace the vertex space of the graphs
+// still running Tier-2 (the Tier-2 row kernels of the round launch over it).
 hipError_t launch_t2_control(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
-                             const RoundGraph* rg, int step, uint32_t pass);
+                             const RoundGraph* rg, int step, uint32_t pass, uint32_t* dspace);
 hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, uint32_t max_n, const uint32_t* fl_round,
                                 uint16_t* const* snap_base, uint32_t* pub_half, HistDev* hist, bool any_history);
 

@@ -39,6 +39,61 @@ __device__ __forceinline__ uint32_t* flag(uint32_t* fl, uint32_t ng, uint32_t ki
     return fl + (size_t)kind * ng + g;
 }
 
+// Block-wide exclusive prefix sum (single-block control kernels); *total
+// gets the sum.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_xscan(uint32_t x, uint32_t* total) {
+    __shared__ uint32_t part[CTL_THREADS / WAVE];
+    const uint32_t w = threadIdx.x / WAVE;
+    uint32_t inc = x;
+    for (uint32_t d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, d);
+        if (lane() >= d) inc += y;
+    }
+    if (lane() == WAVE - 1) part[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t k = 0; k < blockDim.x / WAVE; k++) {
+        if (k < w) before += part[k];
+        all += part[k];
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - x;
+}
+
+// The vertex space of the graphs whose `kind` flag is set, as the host's
+// active sets lay it out: [ns, total, -, -, base[0..ns], gid[0..ns-1]] with
+// base the prefix of their vertex counts, graphs in id order.
+__device__ __forceinline__ void write_space(const GraphDev* __restrict__ graphs, uint32_t ng, uint32_t* fl,
+                                            uint32_t kind, uint32_t* dspace) {
+    const uint32_t per = (ng + blockDim.x - 1) / blockDim.x;
+    const uint32_t g0 = threadIdx.x * per < ng ? threadIdx.x * per : ng;
+    const uint32_t g1 = g0 + per < ng ? g0 + per : ng;
+    uint32_t cg = 0, cv = 0;
+    for (uint32_t g = g0; g < g1; g++)
+        if (*flag(fl, ng, kind, g)) {
+            cg++;
+            cv += graphs[g].n;
+        }
+    uint32_t tg, tv;
+    uint32_t pg = block_xscan(cg, &tg);
+    uint32_t pv = block_xscan(cv, &tv);
+    uint32_t* base = dspace + 4;
+    uint32_t* gid = dspace + 4 + ng + 1;
+    for (uint32_t g = g0; g < g1; g++)
+        if (*flag(fl, ng, kind, g)) {
+            base[pg] = pv;
+            gid[pg] = g;
+            pg++;
+            pv += graphs[g].n;
+        }
+    if (threadIdx.x == 0) {
+        dspace[0] = tg;
+        dspace[1] = tv;
+        base[tg] = tv;
+    }
+}
+
 // Start of a round: the graphs taking part, their Tier-1 state, the slice.
 // After a stop (or past the last round) every flag is cleared and the slice
 // is empty, and the stopped round's Tier-1 state is left for the host.
@@ -165,7 +220,8 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ 
 // host's device_tier2 controls it.
 __global__ __launch_bounds__(CTL_THREADS) void k_t2_control(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
                                                             uint32_t ng, uint32_t* __restrict__ fl,
-                                                            const RoundGraph* __restrict__ rg, int step, uint32_t pass) {
+                                                            const RoundGraph* __restrict__ rg, int step, uint32_t pass,
+                                                            uint32_t* __restrict__ dspace) {
     const bool stopped = ctl[RC_STOP] != 0;
     uint32_t any = 0, halt = 0;
     for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
@@ -208,6 +264,10 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t2_control(uint32_t* __restrict
             *flag(fl, ng, F_T2RUN, g) = 0;
             *flag(fl, ng, F_T2RAN, g) = 0;
         }
+    if (dspace && (step == 0 || step == 1 || step == 3)) {
+        __syncthreads();   // this block's flag writes above, read by other threads below
+        write_space(graphs, ng, fl, F_T2RUN, dspace);
+    }
     if (threadIdx.x == 0) {
         if (halt && !stopped) {
             ctl[RC_STOP] = 1;
@@ -284,8 +344,8 @@ hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs
     return hipGetLastError();
 }
 hipError_t launch_t2_control(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
-                             const RoundGraph* rg, int step, uint32_t pass) {
-    k_t2_control<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, rg, step, pass);
+                             const RoundGraph* rg, int step, uint32_t pass, uint32_t* dspace) {
+    k_t2_control<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, rg, step, pass, dspace);
     return hipGetLastError();
 }
 hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, uint32_t max_n, const uint32_t* fl_round,
