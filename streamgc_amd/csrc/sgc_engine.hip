@@ -244,6 +244,7 @@ struct sgc_pool {
     sgc_tuning tune{1024, 2048, 16384, 65536, 64, 2};
     uint64_t budget = 0, used_bytes = 0;
     hipStream_t stream = nullptr, qstream = nullptr;
+    hipEvent_t batch_up = nullptr;        // a host batch's upload has been read
     std::vector<GraphState> gs;
     Staged<GraphDev> graphs;              // descriptors
     Staged<uint32_t> vbase, gbase, vtile;    // all graphs: vertex prefix, edge-key offsets, 4096-vertex tile prefix
@@ -695,6 +696,7 @@ int sgc_pool::apply(const sgc_edge_update* upd, uint32_t count, bool on_device, 
     hipError_t e = hipSuccess;
     if (!on_device) {
         e = hipMemcpyAsync(d_batch, upd, (size_t)count * sizeof(sgc_edge_update), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipEventRecord(batch_up, stream);
         if (e != hipSuccess) return hip_fail(e, "batch upload");
         src = d_batch;
     }
@@ -757,8 +759,9 @@ int sgc_pool::apply(const sgc_edge_update* upd, uint32_t count, bool on_device, 
     }
     if (e != hipSuccess) return hip_fail(e, "apply launch");
     if (!on_device) {
-        // the staging buffer is reused by the next batch
-        e = hipStreamSynchronize(stream);
+        // the caller may reuse its records once the upload has read them; the
+        // launches run on (the next batch's upload waits for them in stream order)
+        e = hipEventSynchronize(batch_up);
         if (e != hipSuccess) return hip_fail(e, "apply");
     }
     for (uint32_t g = 0; g < ng; g++) gs[g].since_epoch += per_graph[g];
@@ -1168,6 +1171,8 @@ void sgc_pool::release() {
         (void)hipEventDestroy(p.b);
     }
     for (hipEvent_t ev : free_events) (void)hipEventDestroy(ev);
+    if (batch_up) (void)hipEventDestroy(batch_up);
+    batch_up = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     if (qstream) (void)hipStreamDestroy(qstream);
     stream = qstream = nullptr;
@@ -1418,10 +1423,9 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
             e = launch_tier1(Ls, true, det ? 0 : fresh);
         }
     }
-    if (e == hipSuccess)
-        e = launch_t1_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.d_rg, true, false, det,
-                              t.serial_repair, t.serial_conflicts);
-    if (e == hipSuccess) e = launch_t2_prep(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.lean);
+    if (e == hipSuccess)   // the last sweep's counts, the Tier-2 check's graphs (lean: and the round's end)
+        e = launch_t1_finish(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.d_rg, det, t.serial_repair,
+                             t.serial_conflicts, B.lean);
     LaunchCtx Lt = base;
     Lt.active = F(F_T2);
     if (e == hipSuccess && !B.lean) e = launch_build_list(Lt, M_T2, d_list, list_cap, d_list_count);
@@ -1448,7 +1452,7 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
         if (e == hipSuccess) e = launch_build_list(Lc, M_T2, d_list, list_cap, d_list_count);
         if (e == hipSuccess) e = launch_t2_check(Lc);
     }
-    if (e == hipSuccess) e = launch_round_end(stream, B.d_ctl, graphs.d, ng, B.d_fl);
+    if (e == hipSuccess && !B.lean) e = launch_round_end(stream, B.d_ctl, graphs.d, ng, B.d_fl);
     if (e == hipSuccess && !det) {   // bitmaps and histograms exact again for the next updates
         e = launch_build_list(Lr, M_BITS, d_list, list_cap, d_list_count);
         if (e == hipSuccess) e = launch_rebuild(Lr, exact_hist());
@@ -1804,6 +1808,7 @@ int pool_init(sgc_pool* P, uint32_t max_graphs, uint32_t batch_cap, uint32_t fla
     P->gs.resize(max_graphs);
     hipError_t e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&P->qstream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&P->batch_up, hipEventDisableTiming);
     if (e == hipSuccess) e = P->graphs.alloc(max_graphs);
     if (e == hipSuccess) e = P->vbase.alloc(max_graphs + 1);
     if (e == hipSuccess) e = P->act.alloc(4 * (size_t)max_graphs + 2);

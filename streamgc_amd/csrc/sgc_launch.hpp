@@ -134,6 +134,10 @@ hipError_t launch_t1_control(hipStream_t s, uint32_t* ctl, const GraphDev* graph
 hipError_t launch_t2_prep(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
                           const RoundState* st, bool lean);
 hipError_t launch_round_end(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl);
+// launch_t1_control(after, !before), launch_t2_prep and, when lean, launch_round_end in one launch.
+hipError_t launch_t1_finish(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
+                            RoundState* st, const RoundGraph* rg, bool deterministic, uint32_t serial_repair,
+                            uint32_t serial_conflicts, bool lean);
 // Device Tier-2 control of a round: 0 after the check (graphs with bad slots
 // run it), 1 after the reverse-only list (a graph whose list overflowed stops
 // the round), 2 before pass `pass`, 3 after it, 4 at the end (graphs still

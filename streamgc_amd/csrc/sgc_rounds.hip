@@ -140,12 +140,10 @@ __global__ __launch_bounds__(CTL_THREADS) void k_round_begin(uint32_t* __restric
 // the next sweep of each graph still iterating -- serial (vertex order, one
 // wavefront) when deterministic or when the graph is small and two parallel
 // sweeps have left few conflicts, else parallel -- and clear its counter.
-__global__ __launch_bounds__(CTL_THREADS) void k_t1_control(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
-                                                            uint32_t ng, uint32_t* __restrict__ fl,
-                                                            RoundState* __restrict__ st,
-                                                            const RoundGraph* __restrict__ rg, int after, int before,
-                                                            int deterministic, uint32_t serial_repair,
-                                                            uint32_t serial_conflicts) {
+__device__ __forceinline__ void t1_control(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs, uint32_t ng,
+                                           uint32_t* __restrict__ fl, RoundState* __restrict__ st,
+                                           const RoundGraph* __restrict__ rg, int after, int before, int deterministic,
+                                           uint32_t serial_repair, uint32_t serial_conflicts) {
     uint32_t any = 0;
     for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
         uint32_t it = *flag(fl, ng, F_ITER, g);
@@ -183,14 +181,21 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t1_control(uint32_t* __restrict
     any = block_any(any);
     if (threadIdx.x == 0) ctl[RC_T1_ANY] = any;
 }
+__global__ __launch_bounds__(CTL_THREADS) void k_t1_control(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
+                                                            uint32_t ng, uint32_t* __restrict__ fl,
+                                                            RoundState* __restrict__ st,
+                                                            const RoundGraph* __restrict__ rg, int after, int before,
+                                                            int deterministic, uint32_t serial_repair,
+                                                            uint32_t serial_conflicts) {
+    t1_control(ctl, graphs, ng, fl, st, rg, after, before, deterministic, serial_repair, serial_conflicts);
+}
 
 // After the captured Tier-1 sweeps: a graph still iterating stops the round
 // (the host finishes its repair).  The others need the Tier-2 check unless
 // their last sweep was clean and no overflow ever left an edge in one row; in
 // a capture without the Tier-2 section (lean) such a graph stops the round.
-__global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
-                                                         uint32_t ng, uint32_t* __restrict__ fl,
-                                                         const RoundState* __restrict__ st, int lean) {
+__device__ __forceinline__ void t2_prep(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs, uint32_t ng,
+                                        uint32_t* __restrict__ fl, const RoundState* __restrict__ st, int lean) {
     uint32_t stop = 0, missing = 0;
     for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) {
         uint32_t need = 0;
@@ -214,6 +219,11 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ 
         ctl[RC_T2_MISSING] = missing;
         ctl[RC_T1_SHORT] = stop;
     }
+}
+__global__ __launch_bounds__(CTL_THREADS) void k_t2_prep(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
+                                                         uint32_t ng, uint32_t* __restrict__ fl,
+                                                         const RoundState* __restrict__ st, int lean) {
+    t2_prep(ctl, graphs, ng, fl, st, lean);
 }
 
 // Device Tier-2 inside a round (k_t2_* kernels), controlled per graph as the
@@ -281,14 +291,34 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t2_control(uint32_t* __restrict
 
 // End of the repair: a stopped round publishes nothing (its graphs leave the
 // round); otherwise the round index advances.
-__global__ __launch_bounds__(CTL_THREADS) void k_round_end(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
-                                                           uint32_t ng, uint32_t* __restrict__ fl) {
+__device__ __forceinline__ void round_end(uint32_t* __restrict__ ctl, uint32_t ng, uint32_t* __restrict__ fl) {
     const uint32_t stopped = ctl[RC_STOP];
     const uint32_t r = ctl[RC_ROUND];
     __syncthreads();
     if (stopped)
         for (uint32_t g = threadIdx.x; g < ng; g += blockDim.x) *flag(fl, ng, F_ROUND, g) = 0;
     if (threadIdx.x == 0 && !stopped && r < ctl[RC_ROUNDS]) ctl[RC_ROUND] = r + 1;
+}
+__global__ __launch_bounds__(CTL_THREADS) void k_round_end(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
+                                                           uint32_t ng, uint32_t* __restrict__ fl) {
+    round_end(ctl, ng, fl);
+}
+
+// The three steps after the last captured Tier-1 sweep in one launch: fold
+// its counts (k_t1_control after), prepare the Tier-2 check (k_t2_prep) and,
+// in a capture without the Tier-2 section, end the round (k_round_end).  Each
+// step reads what the one before wrote, after a barrier of the one block.
+__global__ __launch_bounds__(CTL_THREADS) void k_t1_finish(uint32_t* __restrict__ ctl, const GraphDev* __restrict__ graphs,
+                                                           uint32_t ng, uint32_t* __restrict__ fl,
+                                                           RoundState* __restrict__ st,
+                                                           const RoundGraph* __restrict__ rg, int deterministic,
+                                                           uint32_t serial_repair, uint32_t serial_conflicts,
+                                                           int lean) {
+    t1_control(ctl, graphs, ng, fl, st, rg, 1, 0, deterministic, serial_repair, serial_conflicts);
+    __syncthreads();
+    t2_prep(ctl, graphs, ng, fl, st, lean);
+    __syncthreads();
+    if (lean) round_end(ctl, ng, fl);
 }
 
 // Keep a copy of the snapshot just written (the graph's history, if any).
@@ -332,6 +362,13 @@ hipError_t launch_t1_control(hipStream_t s, uint32_t* ctl, const GraphDev* graph
                              uint32_t serial_repair, uint32_t serial_conflicts) {
     k_t1_control<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, st, rg, after ? 1 : 0, before ? 1 : 0,
                                            deterministic ? 1 : 0, serial_repair, serial_conflicts);
+    return hipGetLastError();
+}
+hipError_t launch_t1_finish(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
+                            RoundState* st, const RoundGraph* rg, bool deterministic, uint32_t serial_repair,
+                            uint32_t serial_conflicts, bool lean) {
+    k_t1_finish<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, st, rg, deterministic ? 1 : 0, serial_repair,
+                                          serial_conflicts, lean ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t launch_t2_prep(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
