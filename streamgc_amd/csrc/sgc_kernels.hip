@@ -1262,7 +1262,7 @@ __global__ __launch_bounds__(SERIAL_WAVES * WAVE) void k_tier1_serial(const Grap
                                                                      const uint32_t* __restrict__ active,
                                                                      int fresh, uint64_t* totals) {
     __shared__ uint32_t presence[PRESENCE_WORDS];
-    __shared__ uint32_t cand[SERIAL_WAVES], clean[SERIAL_WAVES], next_from;
+    __shared__ uint32_t cand[SERIAL_WAVES], clean[SERIAL_WAVES], next_from, marked[SERIAL_WAVES];
     const uint32_t wv = threadIdx.x / WAVE;
     uint64_t rows = 0, slots = 0;
     for (uint32_t gid = blockIdx.x; gid < ng; gid += gridDim.x) {
@@ -1281,7 +1281,38 @@ __global__ __launch_bounds__(SERIAL_WAVES * WAVE) void k_tier1_serial(const Grap
             // each lane of wavefront 0 reads the marks of 16 vertices of the
             // chunk as two 8-byte words (mark arrays are padded to a multiple of 16)
             const size_t cst = (size_t)M_CHECK * g.dstride, sst = (size_t)M_STICKY * g.dstride;
+            bool look = true;   // the last chunk had nothing marked (block-uniform)
             for (uint32_t base = 0; base < g.n; base += 16 * WAVE) {
+                // After a chunk with nothing marked, skip unmarked chunks
+                // SERIAL_WAVES at a time: each wavefront reads the marks of one
+                // of the next chunks in the same round trip, and the sweep goes
+                // on at the first chunk with any.  The chunks skipped stay
+                // unmarked: only this block's repairs mark this graph's
+                // vertices, and none ran since the read.
+                if (look) {
+                    const uint32_t c0 = base + 16 * WAVE * wv, a = c0 + 16 * lane();
+                    bool any = false;
+                    if (a < g.n) {
+                        const uint64_t* cw = (const uint64_t*)(g.dirty + cst + a);
+                        const uint64_t* sw = (const uint64_t*)(g.dirty + sst + a);
+                        const uint32_t k = g.n - a >= 16 ? 16u : g.n - a;   // live vertices of the 16
+                        const uint64_t lo = k >= 8 ? ~0ull : (1ull << (8 * k)) - 1ull;
+                        const uint64_t hi = k >= 16 ? ~0ull : k <= 8 ? 0ull : (1ull << (8 * (k - 8))) - 1ull;
+                        any = (((ld(&cw[0]) | gp(sw)[0]) & lo) | ((ld(&cw[1]) | gp(sw)[1]) & hi)) != 0;
+                    }
+                    const uint64_t m = ballot(any);
+                    if (lane() == 0) marked[wv] = m != 0;
+                    __syncthreads();
+                    uint32_t first = SERIAL_WAVES;
+                    for (uint32_t f = SERIAL_WAVES; f-- > 0;)
+                        if (marked[f]) first = f;
+                    __syncthreads();
+                    if (first == SERIAL_WAVES) {   // nothing marked in the next SERIAL_WAVES chunks
+                        base += (SERIAL_WAVES - 1) * 16 * WAVE;
+                        continue;
+                    }
+                    base += first * 16 * WAVE;
+                }
                 const uint32_t v0 = base + 16 * lane();
                 const uint64_t* cw = (const uint64_t*)(g.dirty + cst + v0);
                 const uint64_t* sw = (const uint64_t*)(g.dirty + sst + v0);
@@ -1292,6 +1323,7 @@ __global__ __launch_bounds__(SERIAL_WAVES * WAVE) void k_tier1_serial(const Grap
                 }
                 const uint32_t live = v0 >= g.n ? 0u : g.n - v0 >= 16 ? 0xFFFFu : (1u << (g.n - v0)) - 1u;
                 uint32_t from = 0;   // chunk offset of the next vertex the sweep may visit (block-uniform)
+                look = true;
                 for (;;) {
                     if (wv == 0) {   // the next marked vertices
                         const uint64_t w0 = (v0 < g.n ? ld(&cw[0]) : 0ull) | s0;
@@ -1324,6 +1356,7 @@ __global__ __launch_bounds__(SERIAL_WAVES * WAVE) void k_tier1_serial(const Grap
                     if (lane() == 0) clean[wv] = ok ? 1u : 0u;
                     __syncthreads();
                     if (stop) break;
+                    look = false;
                     if (wv == 0) {   // in vertex order, up to the first vertex that is not clean
                         uint32_t last = base + from;
                         for (uint32_t f = 0; f < SERIAL_WAVES; f++) {
