@@ -134,8 +134,10 @@ ROCPROF_NAMES = {"apply": "k_apply_elem", "group": "k_group_link", "rebuild": "k
 
 
 def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (copy-kernel calibrated), if any."""
+    """HBM bytes per launch of `kernel` from the committed summary of this
+    workload's L2 memory-side request counters (read = 128/64/32-byte read
+    requests by size, write = 64/32-byte write requests by size; medians per
+    dispatch), if any."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     if not os.path.exists(path):
         return None, None
@@ -417,9 +419,9 @@ def profile_summary(args):
     """Summaries of the rocprofv3 output of a bench run under DIR: kernel-trace
     statistics per kernel (over the legs of a plain-workload run when --steps
     gives its epochs, else over the whole run), and the median per-dispatch
-    value of every collected counter per kernel, over the same window.
-    FETCH_SIZE and WRITE_SIZE (KiB) are scaled to bytes by what the same
-    counters read for a copy kernel of known size (a copy_* run under DIR)."""
+    value of every collected counter per kernel, over the same window (a
+    run whose directory names copy or gather -- the calibration kernels --
+    is summarised over the whole run)."""
     import csv
     out = {}
     d = args.profile_summary

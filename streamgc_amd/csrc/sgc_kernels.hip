@@ -47,7 +47,7 @@ __device__ __forceinline__ Space device_space(const uint32_t* dspace, uint32_t n
 
 // ---------------------------------------------------------------- sweep sets
 // A maintenance sweep visits either every virtual vertex (list == null, one
-// wavefront per row) or the worklist a k_build_list launch left on the device.
+// wavefront per row) or the worklist launch_build_list left on the device.
 // The worklist keeps rows of at most `long_row` filled slots at its front
 // (count[0] of them, one wavefront each) and longer rows at its back
 // (count[1], list[cap - 1 - k], one 256-thread block each), so a hub row
