@@ -30,7 +30,10 @@ constexpr uint32_t ROW_UNROLL = 4;
 // An insert's or delete's scan of its two rows loads ROW_UNROLL 64-slot
 // chunks of each per round trip, and SCAN_CHUNKS of the one row left once the
 // other is done: a hub row's long search takes SCAN_CHUNKS x 64 slots per trip.
-constexpr uint32_t SCAN_CHUNKS = 16;
+// 8 rather than 16: every trip runs the code of all SCAN_CHUNKS chunks, and
+// the shorter trips made the per-update kernel 8% faster on a 4M-vertex
+// power-law graph although hub rows take twice as many of them.
+constexpr uint32_t SCAN_CHUNKS = 8;
 constexpr int MAX_CAS_RETRIES = 64;          // recolour CAS retries before leaving it to repair
 // Ablation flags carried by the descriptors (the values of SGC_ABL_* in streamgc.h).
 constexpr uint32_t ABL_FORCE_BITMAP = 1u << 5, ABL_NO_OVERFLOW = 1u << 7, ABL_LIVE_DEGREE = 1u << 8;

@@ -421,11 +421,18 @@ __global__ __launch_bounds__(256) void k_group_pack(const uint4* __restrict__ up
         cswap(p0, p2);
         cswap(p1, p3);
         cswap(p1, p2);
+        // (each record loaded under its own branch: selects between whole
+        // vector records put them in scratch memory)
         const uint4 none = make_uint4(PACK_NONE, PACK_END, 0u, 0u);
-        out[0] = p0 != NONE ? upd[p0] : none;
-        out[1] = p1 != NONE ? upd[p1] : none;
-        out[2] = p2 != NONE ? upd[p2] : none;
-        out[3] = p3 != NONE ? upd[p3] : none;
+        uint4 r0 = none, r1 = none, r2 = none, r3 = none;
+        if (p0 != NONE) r0 = upd[p0];
+        if (p1 != NONE) r1 = upd[p1];
+        if (p2 != NONE) r2 = upd[p2];
+        if (p3 != NONE) r3 = upd[p3];
+        out[0] = r0;
+        out[1] = r1;
+        out[2] = r2;
+        out[3] = r3;
     }
 }
 
