@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace sgc {
 
 constexpr uint32_t INVALID = 0xFFFFFFFFu;   // empty slot / no histogram row (A/include/types.h:24-25)
@@ -463,21 +465,20 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
     free0 = free1 = INVALID;
     run0 = run0 && b0 < end0;
     run1 = run1 && b1 < end1;
-    while (run0 || run1) {
-        // chunks [0, per0) are row 0's, [per0, per0 + per1) row 1's
-        const uint32_t per0 = run0 ? (run1 ? ROW_UNROLL : CH) : 0;
-        const uint32_t per1 = run1 ? (run0 ? ROW_UNROLL : CH) : 0;
-        uint32_t val[CH];
+    uint32_t c0, c1;   // first match of each row in this trip
+    bool eq0, eq1;     // it already holds the neighbour (insert)
+    // One trip over N chunks: [0, per0) are row 0's, [per0, per0 + per1) row 1's.
+    auto trip = [&](auto nc, uint32_t per0, uint32_t per1) {
+        constexpr uint32_t N = decltype(nc)::value;
+        uint32_t val[N];
 #pragma unroll
-        for (uint32_t q = 0; q < CH; q++) {
+        for (uint32_t q = 0; q < N; q++) {
             const bool r1 = q >= per0;
             const uint32_t j = (r1 ? b1 + (q - per0) * WAVE : b0 + q * WAVE) + lane();
             val[q] = q < per0 + per1 && j < (r1 ? end1 : end0) ? ld(&g.col[j]) : 0u;
         }
-        uint32_t c0 = INVALID, c1 = INVALID;   // first match of each row in this trip
-        bool eq0 = false, eq1 = false;         // it already holds the neighbour (insert)
 #pragma unroll
-        for (uint32_t q = 0; q < CH; q++) {
+        for (uint32_t q = 0; q < N; q++) {
             const bool r1 = q >= per0;
             const uint32_t b = r1 ? b1 + (q - per0) * WAVE : b0 + q * WAVE;
             const uint32_t want = r1 ? nb1 : nb0;
@@ -501,6 +502,21 @@ __device__ __forceinline__ void scan_pair(const GraphDev& g, bool run0, uint32_t
                     eq0 = eq;
                 }
             }
+        }
+    };
+    while (run0 || run1) {
+        c0 = c1 = INVALID;
+        eq0 = eq1 = false;
+        uint32_t per0, per1;
+        if (run0 && run1 && end0 - b0 <= WAVE && end1 - b1 <= WAVE) {
+            // Both rows end within one chunk: a trip of two chunks (light
+            // rows, the common case, run far fewer instructions).
+            per0 = per1 = 1;
+            trip(std::integral_constant<uint32_t, 2>{}, per0, per1);
+        } else {
+            per0 = run0 ? (run1 ? ROW_UNROLL : CH) : 0;
+            per1 = run1 ? (run0 ? ROW_UNROLL : CH) : 0;
+            trip(std::integral_constant<uint32_t, CH>{}, per0, per1);
         }
         if (run0 && c0 == INVALID) {
             b0 += per0 * WAVE;
