@@ -200,7 +200,7 @@ struct Staged {
 // candidate.
 struct BatchRun {
     sgc_batch_config cfg{};
-    uint32_t rounds = 0, ng = 0, max_count = 0, max_n = 0, sweeps = 0, ovf_cap = 0, t2_passes = 0;
+    uint32_t rounds = 0, ng = 0, max_count = 0, sweeps = 0, ovf_cap = 0, t2_passes = 0;
     uint32_t sweeps_max = 0;                   // most Tier-1 sweeps a capture may hold
     uint64_t updates = 0;
     bool multi = false, any_history = false, all_settled = true;
@@ -1274,7 +1274,6 @@ int sgc_pool::batch_plan(const sgc_edge_update* upd, uint64_t count, const sgc_b
         rg[g].t2_passes = gs[g].cfg.tier2_passes;
         base[g] = gs[g].snap;
         max_iters = std::max(max_iters, gs[g].cfg.max_repair_iters);
-        B->max_n = std::max(B->max_n, gs[g].n);
         B->ovf_cap = std::max(B->ovf_cap, gs[g].cfg.overflow_capacity);
         B->any_history |= gs[g].history != nullptr;
         B->all_settled &= gs[g].settled;
@@ -1457,10 +1456,10 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
         e = launch_build_list(Lr, M_BITS, d_list, list_cap, d_list_count);
         if (e == hipSuccess) e = launch_rebuild(Lr, exact_hist());
     }
-    if (e == hipSuccess) e = launch_snapshot(Lr);
+    // the snapshot also fills each round graph's next history slot
+    if (e == hipSuccess) e = launch_snapshot(Lr, B.any_history ? F(F_ROUND) : nullptr, B.d_hist);
     if (e == hipSuccess)
-        e = launch_round_publish(stream, graphs.d, ng, B.max_n, F(F_ROUND), B.d_snapbase, B.d_pub, B.d_hist,
-                                 B.any_history);
+        e = launch_round_publish(stream, graphs.d, ng, F(F_ROUND), B.d_snapbase, B.d_pub, B.d_hist);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(stream, &g);
     if (e == hipSuccess) e = e2;

@@ -594,38 +594,65 @@ __global__ __launch_bounds__(256) void k_max_color1(const GraphDev* __restrict__
 }
 
 // ---------------------------------------------------------------- snapshot
-// A/src/kernels/epoch_maintenance.cu:5-16, into the back buffer.
+// A/src/kernels/epoch_maintenance.cu:5-16, into the back buffer.  In a
+// captured round (fl_round and hist set) the same colours also go to the
+// graph's next history slot, if it keeps a history and has one free, so the
+// snapshot is not read back for it.
+__device__ __forceinline__ SGC_G uint16_t* history_slot(const uint32_t* fl_round, const HistDev* hist, uint32_t gid,
+                                                       uint32_t n) {
+    if (!hist || !fl_round[gid]) return nullptr;
+    const HistDev h = hist[gid];
+    return h.base && h.count < h.cap ? gp(h.base + (size_t)h.count * n) : nullptr;
+}
+
 // One graph: each thread copies the colours of 4 consecutive vertices (one
 // 16-byte read of their words; the snapshot half may start at any even
 // address, so four 2-byte stores).
 __global__ __launch_bounds__(256) void k_snapshot1(const GraphDev* __restrict__ graphs, Space sp,
-                                                   const uint32_t* __restrict__ active) {
+                                                   const uint32_t* __restrict__ active,
+                                                   const uint32_t* __restrict__ fl_round,
+                                                   const HistDev* __restrict__ hist) {
     const uint32_t gid = sp.gid ? sp.gid[0] : 0u;
     if (active && !active[gid]) return;
     const GraphDev g = graphs[gid];
     SGC_G uint16_t* out = gp(g.snap_write);
+    SGC_G uint16_t* keep = history_slot(fl_round, hist, gid, g.n);
     const uint32_t quads = g.n / 4;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
         const uint4 w = *reinterpret_cast<const uint4*>(g.word + 4 * (size_t)q);
-        out[4 * q] = color_of(w.x);
-        out[4 * q + 1] = color_of(w.y);
-        out[4 * q + 2] = color_of(w.z);
-        out[4 * q + 3] = color_of(w.w);
+        const uint16_t c0 = color_of(w.x), c1 = color_of(w.y), c2 = color_of(w.z), c3 = color_of(w.w);
+        out[4 * q] = c0;
+        out[4 * q + 1] = c1;
+        out[4 * q + 2] = c2;
+        out[4 * q + 3] = c3;
+        if (keep) {
+            keep[4 * q] = c0;
+            keep[4 * q + 1] = c1;
+            keep[4 * q + 2] = c2;
+            keep[4 * q + 3] = c3;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x < g.n % 4) {
         const uint32_t v = 4 * quads + threadIdx.x;
-        out[v] = color_of(gp(g.word)[v]);
+        const uint16_t c = color_of(gp(g.word)[v]);
+        out[v] = c;
+        if (keep) keep[v] = c;
     }
 }
 
 __global__ __launch_bounds__(256) void k_snapshot(const GraphDev* __restrict__ graphs, Space sp,
-                                                  const uint32_t* __restrict__ active) {
+                                                  const uint32_t* __restrict__ active,
+                                                  const uint32_t* __restrict__ fl_round,
+                                                  const HistDev* __restrict__ hist) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sp.total; i += gridDim.x * blockDim.x) {
         uint32_t gid, v;
         sp.locate(i, gid, v);
         if (active && !active[gid]) continue;
         const GraphDev& g = graphs[gid];
-        gp(g.snap_write)[v] = color_of(gp(g.word)[v]);
+        const uint16_t c = color_of(gp(g.word)[v]);
+        gp(g.snap_write)[v] = c;
+        SGC_G uint16_t* keep = history_slot(fl_round, hist, gid, g.n);
+        if (keep) keep[v] = c;
     }
 }
 
@@ -1854,14 +1881,15 @@ hipError_t launch_max_color(const LaunchCtx& L) {
     k_max_color<<<grid, 256, 0, L.stream>>>(L.graphs, space(L));
     return hipGetLastError();
 }
-hipError_t launch_snapshot(const LaunchCtx& L) {
+hipError_t launch_snapshot(const LaunchCtx& L, const uint32_t* fl_round, const HistDev* hist) {
+    if (!fl_round) hist = nullptr;
     if (L.ns == 1) {
         const uint32_t grid = grid_for_waves(L.V / 4 + 1, 4 * WAVE, L.max_blocks);
-        k_snapshot1<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), L.active);
+        k_snapshot1<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), L.active, fl_round, hist);
         return hipGetLastError();
     }
     uint32_t grid = grid_for_waves(L.V, 4 * WAVE, L.max_blocks);
-    k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), L.active);
+    k_snapshot<<<grid, 256, 0, L.stream>>>(L.graphs, space(L), L.active, fl_round, hist);
     return hipGetLastError();
 }
 // Sweep grids: one wavefront per row (blocks first take the long rows of a

@@ -61,7 +61,11 @@ hipError_t launch_apply_seg(const LaunchCtx& L, const void* upd, const uint32_t*
                             const uint32_t* perm, int multi);
 hipError_t launch_zero_ctr(const LaunchCtx& L, uint32_t idx);
 hipError_t launch_max_color(const LaunchCtx& L);
-hipError_t launch_snapshot(const LaunchCtx& L);
+struct HistDev;
+// The snapshot of the active graphs into their back buffers; with hist and
+// fl_round (a captured round), also into the history slot of each graph of
+// the round that keeps a history.
+hipError_t launch_snapshot(const LaunchCtx& L, const uint32_t* fl_round = nullptr, const HistDev* hist = nullptr);
 hipError_t launch_compact(const LaunchCtx& L);
 hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
 // Worklist of the vertices marked `kind` (or sticky).  `list` holds
@@ -146,8 +150,11 @@ hipError_t launch_t1_finish(hipStream_t s, uint32_t* ctl, const GraphDev* graphs
 // still running Tier-2 (the Tier-2 row kernels of the round launch over it).
 hipError_t launch_t2_control(hipStream_t s, uint32_t* ctl, const GraphDev* graphs, uint32_t ng, uint32_t* fl,
                              const RoundGraph* rg, int step, uint32_t pass, uint32_t* dspace);
-hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, uint32_t max_n, const uint32_t* fl_round,
-                                uint16_t* const* snap_base, uint32_t* pub_half, HistDev* hist, bool any_history);
+// After the round's snapshot (which also filled the history slots): each
+// round graph's snapshot becomes the published one and its history count
+// moves on.
+hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, const uint32_t* fl_round,
+                                uint16_t* const* snap_base, uint32_t* pub_half, HistDev* hist);
 
 hipError_t launch_fill_rows(hipStream_t s, uint32_t n, const uint32_t* rp0, const uint32_t* col0, const uint32_t* rp,
                             uint32_t* col, uint32_t* fill);

@@ -321,21 +321,6 @@ __global__ __launch_bounds__(CTL_THREADS) void k_t1_finish(uint32_t* __restrict_
     if (lean) round_end(ctl, ng, fl);
 }
 
-// Keep a copy of the snapshot just written (the graph's history, if any).
-// Grid: x over 2048-vertex chunks, y over graphs.
-__global__ __launch_bounds__(256) void k_round_history(const GraphDev* __restrict__ graphs,
-                                                       const uint32_t* __restrict__ fl_round,
-                                                       const HistDev* __restrict__ hist) {
-    const uint32_t g = blockIdx.y;
-    if (!fl_round[g]) return;
-    const HistDev h = hist[g];
-    if (!h.base || h.count >= h.cap) return;
-    const uint32_t n = graphs[g].n;
-    const uint16_t* src = graphs[g].snap_write;
-    uint16_t* dst = h.base + (size_t)h.count * n;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) dst[v] = src[v];
-}
-
 // The snapshot just written becomes the published one; the next publish
 // writes the other buffer.
 __global__ void k_round_flip(GraphDev* __restrict__ graphs, uint32_t ng, const uint32_t* __restrict__ fl_round,
@@ -385,13 +370,8 @@ hipError_t launch_t2_control(hipStream_t s, uint32_t* ctl, const GraphDev* graph
     k_t2_control<<<1, CTL_THREADS, 0, s>>>(ctl, graphs, ng, fl, rg, step, pass, dspace);
     return hipGetLastError();
 }
-hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, uint32_t max_n, const uint32_t* fl_round,
-                                uint16_t* const* snap_base, uint32_t* pub_half, HistDev* hist, bool any_history) {
-    if (any_history) {
-        uint32_t chunks = (max_n + 2047) / 2048;
-        dim3 grid(chunks ? chunks : 1, ng);
-        k_round_history<<<grid, 256, 0, s>>>(graphs, fl_round, hist);
-    }
+hipError_t launch_round_publish(hipStream_t s, GraphDev* graphs, uint32_t ng, const uint32_t* fl_round,
+                                uint16_t* const* snap_base, uint32_t* pub_half, HistDev* hist) {
     k_round_flip<<<(ng + 255) / 256, 256, 0, s>>>(graphs, ng, fl_round, snap_base, pub_half, hist);
     return hipGetLastError();
 }
