@@ -1113,10 +1113,10 @@ int sgc_pool::epoch(const std::vector<uint32_t>& gids, bool finish, std::vector<
     }
     LaunchCtx Lc = L;
     if (e == hipSuccess) e = sweep(Lc, M_ROW);
-    if (e == hipSuccess) e = timed("compact", [&] { return launch_compact(Lc); });
+    if (e == hipSuccess) e = timed("compact", [&] { return launch_compact(Lc, true); });
     uint32_t ovf_cap = 0;
     for (uint32_t g : gids) ovf_cap = std::max(ovf_cap, gs[g].cfg.overflow_capacity);
-    if (e == hipSuccess) e = timed("drain", [&] { return launch_drain(L, deterministic(), ovf_cap); });
+    if (e == hipSuccess) e = timed("drain", [&] { return launch_drain(L, deterministic(), ovf_cap, true); });
     if (e != hipSuccess) return hip_fail(e, "compact/drain");
     if (out)   // drain statistics (the repair below reads the counters it needs itself)
         if ((r = read_drain(gids))) return r;
@@ -1397,8 +1397,8 @@ int sgc_pool::batch_capture(const sgc_tuning& t, hipGraphExec_t* out) {
     LaunchCtx Lr = base;
     Lr.active = F(F_ROUND);
     if (e == hipSuccess) e = launch_build_list(Lr, M_ROW, d_list, list_cap, d_list_count);
-    if (e == hipSuccess) e = launch_compact(Lr);
-    if (e == hipSuccess) e = launch_drain(Lr, det, B.ovf_cap);
+    if (e == hipSuccess) e = launch_compact(Lr, true);
+    if (e == hipSuccess) e = launch_drain(Lr, det, B.ovf_cap, true);
     for (uint32_t it = 0; it < B.sweeps && e == hipSuccess; it++) {
         e = launch_t1_control(stream, B.d_ctl, graphs.d, ng, B.d_fl, B.d_state, B.d_rg, it > 0, true, det,
                               t.serial_repair, t.serial_conflicts);

@@ -725,8 +725,28 @@ __device__ __forceinline__ uint32_t compact_row_wave(const GraphDev& g, uint32_t
     return w - s;
 }
 
+// The drain's counter snapshot of one graph (A/src/streamgc.cu:972-983): the
+// overflow entries it retries (capped at the buffer), those lost past the
+// buffer, counters reset.
+__device__ __forceinline__ void drain_prep_graph(const GraphDev* __restrict__ graphs, uint32_t gid) {
+    SGC_G uint32_t* c = gp(graphs[gid].ctr);
+    uint32_t raw = c[C_OVF];
+    c[C_OVF_RAW] = raw;
+    c[C_OVF_SNAP] = raw < graphs[gid].ovf_cap ? raw : graphs[gid].ovf_cap;
+    if (raw > graphs[gid].ovf_cap) c[C_LOST] += raw - graphs[gid].ovf_cap;   // never buffered
+    c[C_OVF] = 0;
+    c[C_DRAIN_FAILED] = 0;
+}
+
+// prep_ng > 0: block 0 also takes the drain's counter snapshot of the
+// active graphs (k_drain_prep), which the drain launched next reads;
+// compaction touches none of those counters.
 __global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ graphs, Space sp, SweepList sl,
-                                                 uint64_t* totals) {
+                                                 uint64_t* totals, uint32_t prep_ng,
+                                                 const uint32_t* __restrict__ prep_active) {
+    if (blockIdx.x == 0)
+        for (uint32_t gid = threadIdx.x; gid < prep_ng; gid += blockDim.x)
+            if (!prep_active || prep_active[gid]) drain_prep_graph(graphs, gid);
     uint64_t live_total = 0, rows = 0, slots = 0;
     const uint32_t nbig = sweep_big(sl);
     for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
@@ -762,13 +782,7 @@ __global__ __launch_bounds__(256) void k_compact(const GraphDev* __restrict__ gr
 __global__ void k_drain_prep(const GraphDev* __restrict__ graphs, uint32_t ng, const uint32_t* __restrict__ active) {
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= ng || (active && !active[gid])) return;
-    SGC_G uint32_t* c = gp(graphs[gid].ctr);
-    uint32_t raw = c[C_OVF];
-    c[C_OVF_RAW] = raw;
-    c[C_OVF_SNAP] = raw < graphs[gid].ovf_cap ? raw : graphs[gid].ovf_cap;
-    if (raw > graphs[gid].ovf_cap) c[C_LOST] += raw - graphs[gid].ovf_cap;   // never buffered
-    c[C_OVF] = 0;
-    c[C_DRAIN_FAILED] = 0;
+    drain_prep_graph(graphs, gid);
 }
 
 // A/src/kernels/epoch_maintenance.cu:66-87: one wavefront per entry, graph = blockIdx.y.
@@ -1919,12 +1933,13 @@ hipError_t launch_build_list(const LaunchCtx& L, uint32_t kind, uint32_t* list, 
                                                L.active, L.gate);
     return hipGetLastError();
 }
-hipError_t launch_compact(const LaunchCtx& L) {
-    k_compact<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L), L.totals);
+hipError_t launch_compact(const LaunchCtx& L, bool drain_prep) {
+    k_compact<<<sweep_grid(L), 256, 0, L.stream>>>(L.graphs, space(L), sweep_list(L), L.totals,
+                                                   drain_prep ? L.ng : 0u, L.active);
     return hipGetLastError();
 }
-hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count) {
-    k_drain_prep<<<(L.ng + 63) / 64, 64, 0, L.stream>>>(L.graphs, L.ng, L.active);
+hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count, bool prepped) {
+    if (!prepped) k_drain_prep<<<(L.ng + 63) / 64, 64, 0, L.stream>>>(L.graphs, L.ng, L.active);
     if (serial) {
         k_drain_serial<<<L.ng < 1024 ? L.ng : 1024, 64, 0, L.stream>>>(L.graphs, L.ng, L.active);
     } else {

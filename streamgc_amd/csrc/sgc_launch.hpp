@@ -66,8 +66,10 @@ struct HistDev;
 // fl_round (a captured round), also into the history slot of each graph of
 // the round that keeps a history.
 hipError_t launch_snapshot(const LaunchCtx& L, const uint32_t* fl_round = nullptr, const HistDev* hist = nullptr);
-hipError_t launch_compact(const LaunchCtx& L);
-hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count);
+// drain_prep: the compaction launch also takes the drain's counter snapshot
+// (the launch_drain that follows then passes prepped).
+hipError_t launch_compact(const LaunchCtx& L, bool drain_prep = false);
+hipError_t launch_drain(const LaunchCtx& L, bool serial, uint32_t max_count, bool prepped = false);
 // Worklist of the vertices marked `kind` (or sticky).  `list` holds
 // list_words(cap, graphs) u32: the list, then the builder's scratch of
 // 4096-vertex tiles (each inside one graph, so up to one more per graph).
