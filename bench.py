@@ -170,7 +170,9 @@ def tuning_knobs(args):
                                           "fresh_repair") if getattr(args, k) is not None}
 
 
-def main():
+def arg_parser():
+    """The command line of bench.py (also used to check the commands the
+    profile summaries quote)."""
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
@@ -210,7 +212,11 @@ def main():
     ap.add_argument("--graphs", type=int, default=256, help="batched workload: graphs in the pool")
     ap.add_argument("--init-batch", type=int, default=256,
                     help="apply the (untimed) init stream in launches of this many updates, epochs still every E")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = arg_parser().parse_args()
     if args.profile_summary:
         return profile_summary(args)
     if args.gpus != 1:
